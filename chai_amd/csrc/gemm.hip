@@ -1,0 +1,609 @@
+// gemm.hip -- the GEMM family behind chai_hip_gemm.
+//
+// Replaces the Evergreen "MM" OpenCL generator (autotune/EvergreenMatmulMM.cpp:
+// 304-384 parameters, 1077-1630 kernel text) and its enqueue path
+// (enqueue/EnqueueMatmul.cpp:24-161, enqueue/EnqueueMatmulAuto.cpp:21-402).
+// Semantics kept: C = op(A) op(B) or C = alpha op(A) op(B) + beta C, row-major,
+// op() folding a transpose, accumulation precision max(precA, precB), batches of
+// independent instances with optional shared A or B (sameDataA/B).
+//
+// Kernels (all gfx950):
+//   split_kernel    : f32/i32/u32 operand -> bf16 hi/lo panels, K contiguous,
+//                     zero padded to the tile grid (an HBM-bound pre-pass).
+//   gemm_bf16x3     : 256x256x32 tiles, 8 waves (2 x 4), each wave 128x64 built
+//                     from 32x32x16 bf16 MFMAs; three products hi*lo + lo*hi +
+//                     hi*hi accumulate in FP32; double-buffered XOR-swizzled LDS.
+//   sgemm_fma       : plain FP32 VALU FMA, 128x128x16 tiles, 8x8 outputs/thread.
+//   dgemm_mfma      : FP64 with 16x16x4 f64 MFMAs, 128x128x16 tiles, 4 waves.
+#include "common.hpp"
+#include "gemm_device.hpp"
+#include "gemm_internal.hpp"
+
+#include <type_traits>
+
+namespace chai_hip {
+
+// =====================================================================
+// split / pack pre-pass
+// =====================================================================
+
+template <typename T>
+__device__ __forceinline__ float to_f32(T v) { return (float)v; }
+
+// op(X) element (r, k) lives at src[r * ld + k] (TRANS = false) or
+// src[k * ld + r] (TRANS = true).  Output panels hi/lo are [Rp][Kp] bf16.
+// 64 x 64 tile per 256-thread block; the transposed case goes through LDS.
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, uint64_t ld,
+                                                    uint64_t R, uint64_t K, uint64_t Kp,
+                                                    uint64_t src_bstride, uint64_t dst_bstride,
+                                                    uint16_t* __restrict__ hi,
+                                                    uint16_t* __restrict__ lo)
+{
+    __shared__ float tile[64][65];
+    const uint64_t r0 = (uint64_t)blockIdx.y * 64, k0 = (uint64_t)blockIdx.x * 64;
+    src += (uint64_t)blockIdx.z * src_bstride;
+    hi += (uint64_t)blockIdx.z * dst_bstride;
+    lo += (uint64_t)blockIdx.z * dst_bstride;
+    const int t = threadIdx.x;
+    if (TRANS) {
+        // coalesced along r: 64 lanes read 64 consecutive r at one k
+        for (int i = 0; i < 16; i++) {
+            const int kk = i * 4 + (t >> 6), rr = t & 63;
+            const uint64_t r = r0 + rr, k = k0 + kk;
+            tile[kk][rr] = (r < R && k < K) ? to_f32(src[k * ld + r]) : 0.0f;
+        }
+    } else {
+        for (int i = 0; i < 16; i++) {
+            const int rr = i * 4 + (t >> 6), kk = t & 63;
+            const uint64_t r = r0 + rr, k = k0 + kk;
+            tile[kk][rr] = (r < R && k < K) ? to_f32(src[r * ld + k]) : 0.0f;
+        }
+    }
+    __syncthreads();
+    // each thread writes 16 consecutive k of one row: 2 x 16 bytes per panel
+    const int rr = t >> 2, kq = (t & 3) * 16;
+    const uint64_t r = r0 + rr;
+    uint16_t h[16], l[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) split_bf16(tile[kq + j][rr], h[j], l[j]);
+    if (k0 + kq < Kp) {
+        uint4* dh = reinterpret_cast<uint4*>(hi + r * Kp + k0 + kq);
+        uint4* dl = reinterpret_cast<uint4*>(lo + r * Kp + k0 + kq);
+        uint4 vh0, vh1, vl0, vl1;
+        vh0.x = h[0] | ((uint32_t)h[1] << 16);   vh0.y = h[2] | ((uint32_t)h[3] << 16);
+        vh0.z = h[4] | ((uint32_t)h[5] << 16);   vh0.w = h[6] | ((uint32_t)h[7] << 16);
+        vh1.x = h[8] | ((uint32_t)h[9] << 16);   vh1.y = h[10] | ((uint32_t)h[11] << 16);
+        vh1.z = h[12] | ((uint32_t)h[13] << 16); vh1.w = h[14] | ((uint32_t)h[15] << 16);
+        vl0.x = l[0] | ((uint32_t)l[1] << 16);   vl0.y = l[2] | ((uint32_t)l[3] << 16);
+        vl0.z = l[4] | ((uint32_t)l[5] << 16);   vl0.w = l[6] | ((uint32_t)l[7] << 16);
+        vl1.x = l[8] | ((uint32_t)l[9] << 16);   vl1.y = l[10] | ((uint32_t)l[11] << 16);
+        vl1.z = l[12] | ((uint32_t)l[13] << 16); vl1.w = l[14] | ((uint32_t)l[15] << 16);
+        dh[0] = vh0;
+        if (k0 + kq + 8 < Kp) dh[1] = vh1;
+        dl[0] = vl0;
+        if (k0 + kq + 8 < Kp) dl[1] = vl1;
+    }
+}
+
+// Pack op(X) into a dense zero-padded [Rp][Cp] array of TO (element (r, c) of
+// op(X) at src[r*ld + c] or, transposed, src[c*ld + r]).
+template <typename T, typename TO, bool TRANS>
+__global__ __launch_bounds__(256) void pack_kernel(const T* __restrict__ src, uint64_t ld, uint64_t R,
+                                                   uint64_t Cn, uint64_t Cp, uint64_t src_bstride,
+                                                   uint64_t dst_bstride, TO* __restrict__ dst)
+{
+    __shared__ TO tile[32][33];
+    const uint64_t r0 = (uint64_t)blockIdx.y * 32, c0 = (uint64_t)blockIdx.x * 32;
+    src += (uint64_t)blockIdx.z * src_bstride;
+    dst += (uint64_t)blockIdx.z * dst_bstride;
+    const int t = threadIdx.x;
+    for (int i = 0; i < 4; i++) {
+        const int a = i * 8 + (t >> 5), b = t & 31;
+        if (TRANS) {   // read along r (contiguous in src)
+            const uint64_t r = r0 + b, c = c0 + a;
+            tile[a][b] = (r < R && c < Cn) ? (TO)src[c * ld + r] : (TO)0;
+        } else {
+            const uint64_t r = r0 + a, c = c0 + b;
+            tile[b][a] = (r < R && c < Cn) ? (TO)src[r * ld + c] : (TO)0;
+        }
+    }
+    __syncthreads();
+    for (int i = 0; i < 4; i++) {
+        const int a = i * 8 + (t >> 5), b = t & 31;   // write row r0+a, col c0+b
+        const uint64_t c = c0 + b;
+        if (c < Cp) dst[(r0 + a) * Cp + c] = tile[b][a];
+    }
+}
+
+// =====================================================================
+// split-bf16x3 MFMA GEMM
+// =====================================================================
+
+constexpr int X3_BM = 256, X3_BN = 256, X3_BK = 32, X3_THREADS = 512;
+constexpr int X3_PANEL = X3_BM * X3_BK / 8;           // uint4 chunks per panel per stage
+constexpr int X3_STAGE = 4 * X3_PANEL;                 // Ah, Al, Bh, Bl
+constexpr size_t X3_LDS_BYTES = 2 * X3_STAGE * sizeof(uint4);
+
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
+{
+    bf16x8 r;
+    __builtin_memcpy(&r, &v, 16);
+    return r;
+}
+
+// Epilogue shared by the MFMA kernels: C = v, or alpha*v + beta*C in double.
+__device__ __forceinline__ void store_c(float* C, uint64_t idx, float v, const GemmArgs& a)
+{
+    if (a.general) C[idx] = (float)(a.alpha * (double)v + a.beta * (double)C[idx]);
+    else C[idx] = v;
+}
+
+__global__ __launch_bounds__(X3_THREADS, 1) void gemm_bf16x3(GemmArgs a)
+{
+    extern __shared__ uint4 lds[];
+    uint32_t tm, tn;
+    tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    const uint64_t bz = blockIdx.z;
+    const uint64_t Kp = a.Kp;
+    const uint16_t* src[4] = {
+        a.Ah + bz * a.pa_bstride + (uint64_t)tm * X3_BM * Kp,
+        a.Al + bz * a.pa_bstride + (uint64_t)tm * X3_BM * Kp,
+        a.Bh + bz * a.pb_bstride + (uint64_t)tn * X3_BN * Kp,
+        a.Bl + bz * a.pb_bstride + (uint64_t)tn * X3_BN * Kp};
+    float* C = a.C + bz * a.c_bstride;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 2) * 128;
+    const int wn = (wave & 3) * 64;
+
+    floatx16 acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
+
+    uint4 st[8];
+    // chunk j of this thread: panel j>>1, row (t + 512*(j&1)) >> 2, 16-byte column q = t & 3
+    const int q_ld = t & 3;
+    auto gload = [&](int kt) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int row = (t + 512 * (j & 1)) >> 2;
+            st[j] = *(reinterpret_cast<const uint4*>(src[j >> 1] + (uint64_t)row * Kp + (uint64_t)kt * X3_BK) + q_ld);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int row = (t + 512 * (j & 1)) >> 2;
+            const int p = q_ld ^ ((row >> 2) & 3);
+            lds[buf * X3_STAGE + (j >> 1) * X3_PANEL + row * 4 + p] = st[j];
+        }
+    };
+
+    const int nk = (int)(Kp / X3_BK);
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+        const uint4* L = lds + buf * X3_STAGE;
+#pragma unroll
+        for (int kb = 0; kb < 2; kb++) {
+            const int q = kb * 2 + (lane >> 5);
+            bf16x8 ah[4], al[4], bh[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int row = wm + i * 32 + (lane & 31);
+                const int p = q ^ ((row >> 2) & 3);
+                ah[i] = as_bf16x8(L[0 * X3_PANEL + row * 4 + p]);
+                al[i] = as_bf16x8(L[1 * X3_PANEL + row * 4 + p]);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int row = wn + j * 32 + (lane & 31);
+                const int p = q ^ ((row >> 2) & 3);
+                bh[j] = as_bf16x8(L[2 * X3_PANEL + row * 4 + p]);
+                bl[j] = as_bf16x8(L[3 * X3_PANEL + row * 4 + p]);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+        if (kt + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+
+    // 32x32 output layout: lane -> column lane%32; register r -> row 8*(r/4) + 4*(lane/32) + r%4
+    const uint64_t row_base = (uint64_t)tm * X3_BM + wm;
+    const uint64_t col_base = (uint64_t)tn * X3_BN + wn;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint64_t col = col_base + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint64_t row = row_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
+            }
+        }
+}
+
+// =====================================================================
+// plain FP32 VALU FMA GEMM
+// =====================================================================
+
+constexpr int F_BM = 128, F_BN = 128, F_BK = 16;
+
+// A packed [Mp][Kp] f32, B packed [Kp][Np] f32 (both zero padded).
+__global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
+{
+    __shared__ float As[2][F_BK][F_BM + 4];
+    __shared__ float Bs[2][F_BK][F_BN + 4];
+    uint32_t tm, tn;
+    tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    const uint64_t bz = blockIdx.z;
+    const uint64_t Kp = a.Kp, Np = a.Np;
+    const float* A = a.Af + bz * a.pa_bstride + (uint64_t)tm * F_BM * Kp;
+    const float* B = a.Bf + bz * a.pb_bstride + (uint64_t)tn * F_BN;
+    float* C = a.C + bz * a.c_bstride;
+    const int t = threadIdx.x;
+    const int tx = t & 15, ty = t >> 4;
+
+    float acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 8; j++) acc[i][j] = 0.0f;
+
+    // A tile 128 x 16: thread loads rows (t>>2) and (t>>2)+64, 4 consecutive k at (t&3)*4
+    // B tile 16 x 128: thread loads k rows (t>>5) and (t>>5)+8, 4 consecutive n at (t&31)*4
+    float4 ra[2], rb[2];
+    auto gload = [&](int kt) {
+        const uint64_t k0 = (uint64_t)kt * F_BK;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int row = (t >> 2) + h * 64;
+            ra[h] = *reinterpret_cast<const float4*>(A + (uint64_t)row * Kp + k0 + (t & 3) * 4);
+            const int kr = (t >> 5) + h * 8;
+            rb[h] = *reinterpret_cast<const float4*>(B + (k0 + kr) * Np + (t & 31) * 4);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int row = (t >> 2) + h * 64, kc = (t & 3) * 4;
+            As[buf][kc + 0][row] = ra[h].x;
+            As[buf][kc + 1][row] = ra[h].y;
+            As[buf][kc + 2][row] = ra[h].z;
+            As[buf][kc + 3][row] = ra[h].w;
+            const int kr = (t >> 5) + h * 8;
+            *reinterpret_cast<float4*>(&Bs[buf][kr][(t & 31) * 4]) = rb[h];
+        }
+    };
+    const int nk = (int)(Kp / F_BK);
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+#pragma unroll 2
+        for (int k = 0; k < F_BK; k++) {
+            const float4 a0 = *reinterpret_cast<const float4*>(&As[buf][k][ty * 4]);
+            const float4 a1 = *reinterpret_cast<const float4*>(&As[buf][k][ty * 4 + 64]);
+            const float4 b0 = *reinterpret_cast<const float4*>(&Bs[buf][k][tx * 4]);
+            const float4 b1 = *reinterpret_cast<const float4*>(&Bs[buf][k][tx * 4 + 64]);
+            const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+            const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+            for (int i = 0; i < 8; i++)
+#pragma unroll
+                for (int j = 0; j < 8; j++) acc[i][j] = __builtin_fmaf(av[i], bv[j], acc[i][j]);
+        }
+        if (kt + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint64_t row = (uint64_t)tm * F_BM + ty * 4 + (i & 3) + (i >> 2) * 64;
+        if (row >= a.M) continue;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const uint64_t col = (uint64_t)tn * F_BN + tx * 4 + (j & 3) + (j >> 2) * 64;
+            if (col < a.N) store_c(C, row * a.ldc + col, acc[i][j], a);
+        }
+    }
+}
+
+// =====================================================================
+// FP64 MFMA GEMM
+// =====================================================================
+
+constexpr int D_BM = 128, D_BN = 128, D_BK = 8;
+
+// A packed [Mp][Kp] f64, B packed [Kp][Np] f64.
+__global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
+{
+    __shared__ double As[2][D_BK][D_BM + 2];   // k-major (A transposed on the way in)
+    __shared__ double Bs[2][D_BK][D_BN + 2];
+    uint32_t tm, tn;
+    tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    const uint64_t bz = blockIdx.z;
+    const uint64_t Kp = a.Kp, Np = a.Np;
+    const double* A = a.Ad + bz * a.pa_bstride + (uint64_t)tm * D_BM * Kp;
+    const double* B = a.Bd + bz * a.pb_bstride + (uint64_t)tn * D_BN;
+    double* C = a.Cd + bz * a.c_bstride;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+
+    doublex4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (doublex4){0.0, 0.0, 0.0, 0.0};
+
+    // A tile 128 x 8 doubles: thread loads row t>>1, 4 consecutive k at (t&1)*4
+    // B tile 8 x 128 doubles: thread loads k row t>>5, 4 consecutive n at (t&31)*4
+    double2 ra[2], rb[2];
+    auto gload = [&](int kt) {
+        const uint64_t k0 = (uint64_t)kt * D_BK;
+        const double2* pa = reinterpret_cast<const double2*>(A + (uint64_t)(t >> 1) * Kp + k0 + (t & 1) * 4);
+        const double2* pb = reinterpret_cast<const double2*>(B + (k0 + (t >> 5)) * Np + (t & 31) * 4);
+#pragma unroll
+        for (int h = 0; h < 2; h++) { ra[h] = pa[h]; rb[h] = pb[h]; }
+    };
+    auto lstore = [&](int buf) {
+        const int row = t >> 1, kc = (t & 1) * 4;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            As[buf][kc + 2 * h][row] = ra[h].x;
+            As[buf][kc + 2 * h + 1][row] = ra[h].y;
+            Bs[buf][t >> 5][(t & 31) * 4 + 2 * h] = rb[h].x;
+            Bs[buf][t >> 5][(t & 31) * 4 + 2 * h + 1] = rb[h].y;
+        }
+    };
+    const int nk = (int)(Kp / D_BK);
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+#pragma unroll
+        for (int ks = 0; ks < D_BK / 4; ks++) {
+            const int k = ks * 4 + (lane >> 4);
+            double av[4], bv[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) av[i] = As[buf][k][wm + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < 4; j++) bv[j] = Bs[buf][k][wn + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+        if (kt + 1 < nk) lstore(buf ^ 1);
+        __syncthreads();
+    }
+    // 16x16 output layout: lane -> column lane%16; register r -> row 4*(lane/16) + r
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t col = (uint64_t)tn * D_BN + wn + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const uint64_t row = (uint64_t)tm * D_BM + wm + i * 16 + 4 * (lane >> 4) + r;
+                if (row < a.M && col < a.N) {
+                    const uint64_t idx = row * a.ldc + col;
+                    const double v = acc[i][j][r];
+                    C[idx] = a.general ? a.alpha * v + a.beta * C[idx] : v;
+                }
+            }
+        }
+}
+
+// =====================================================================
+// host side
+// =====================================================================
+
+static uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
+
+// ---- optional per-kernel timing of the calling thread's last GEMM
+static volatile int g_profile = 0;
+struct ProfEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    bool valid = false;
+    ~ProfEvents()
+    {
+        for (auto& x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+static thread_local ProfEvents t_prof;
+
+static void prof_mark(int i, hipStream_t st)
+{
+    if (!g_profile) return;
+    if (!t_prof.e[i] && hipEventCreate(&t_prof.e[i]) != hipSuccess) { t_prof.e[i] = nullptr; return; }
+    (void)hipEventRecord(t_prof.e[i], st);
+    t_prof.valid = (i == 2) ? true : (i == 0 ? false : t_prof.valid);
+}
+
+template <typename T>
+static void launch_split(const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
+                         uint64_t Rp, uint64_t Kp, uint64_t src_bstride, uint64_t nb,
+                         uint16_t* hi, uint16_t* lo, hipStream_t st)
+{
+    dim3 grid((unsigned)(Kp / 64 + (Kp % 64 ? 1 : 0)), (unsigned)(Rp / 64), (unsigned)nb);
+    if (trans)
+        split_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Kp, src_bstride, Rp * Kp, hi, lo);
+    else
+        split_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Kp, src_bstride, Rp * Kp, hi, lo);
+}
+
+static void split_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
+                      uint64_t Rp, uint64_t Kp, uint64_t src_bstride, uint64_t nb, uint16_t* hi,
+                      uint16_t* lo, hipStream_t st)
+{
+    switch (prec) {
+        case CHAI_F32: launch_split<float>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, st); break;
+        case CHAI_I32: launch_split<int32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, st); break;
+        default: launch_split<uint32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, st); break;
+    }
+}
+
+template <typename T, typename TO>
+static void launch_pack(const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t Cn,
+                        uint64_t Rp, uint64_t Cp, uint64_t src_bstride, uint64_t nb, TO* dst,
+                        hipStream_t st)
+{
+    dim3 grid((unsigned)(Cp / 32), (unsigned)(Rp / 32), (unsigned)nb);
+    if (trans)
+        pack_kernel<T, TO, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, Cn, Cp, src_bstride, Rp * Cp, dst);
+    else
+        pack_kernel<T, TO, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, Cn, Cp, src_bstride, Rp * Cp, dst);
+}
+
+template <typename TO>
+static void pack_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t Cn,
+                     uint64_t Rp, uint64_t Cp, uint64_t src_bstride, uint64_t nb, TO* dst, hipStream_t st)
+{
+    switch (prec) {
+        case CHAI_F32: launch_pack<float, TO>(src, ld, trans, R, Cn, Rp, Cp, src_bstride, nb, dst, st); break;
+        case CHAI_F64: launch_pack<double, TO>(src, ld, trans, R, Cn, Rp, Cp, src_bstride, nb, dst, st); break;
+        case CHAI_I32: launch_pack<int32_t, TO>(src, ld, trans, R, Cn, Rp, Cp, src_bstride, nb, dst, st); break;
+        default: launch_pack<uint32_t, TO>(src, ld, trans, R, Cn, Rp, Cp, src_bstride, nb, dst, st); break;
+    }
+}
+
+int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
+                hipStream_t st)
+{
+    const uint64_t M = d.M, N = d.N, K = d.K, nb = d.batch ? d.batch : 1;
+    const uint64_t a_rows = d.trans_a ? K : M;   // stored layout of A
+    (void)a_rows;
+    if (mode == CHAI_GEMM_BF16X3) {
+        if (d.prec_c != CHAI_F32) { set_error("gemm: bf16x3 writes f32 C"); return CHAI_ERR_ARG; }
+        const uint64_t Mp = round_up(M, X3_BM), Np = round_up(N, X3_BN), Kp = round_up(K, 64);
+        const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
+        const size_t bytes = 2 * sizeof(uint16_t) * (Mp * Kp * na + Np * Kp * nbb);
+        uint16_t* ws = nullptr;
+        CHAI_HIP_TRY(hipMallocAsync((void**)&ws, bytes, st));
+        uint16_t* Ah = ws;
+        uint16_t* Al = Ah + Mp * Kp * na;
+        uint16_t* Bh = Al + Mp * Kp * na;
+        uint16_t* Bl = Bh + Np * Kp * nbb;
+        prof_mark(0, st);
+        // op(A): element (m,k) at A[m*lda+k] or, transposed, A[k*lda+m]
+        split_any(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, Ah, Al, st);
+        // B^T panels: element (n,k) = op(B)[k][n] at B[k*ldb+n] (transposed read) or B[n*ldb+k]
+        split_any(d.prec_b, B, d.ldb, d.trans_b == 0, N, K, Np, Kp, d.stride_b, nbb, Bh, Bl, st);
+        prof_mark(1, st);
+        GemmArgs g{};
+        g.Ah = Ah; g.Al = Al; g.Bh = Bh; g.Bl = Bl;
+        g.Kp = Kp; g.Np = Np;
+        g.pa_bstride = d.same_a ? 0 : Mp * Kp;
+        g.pb_bstride = d.same_b ? 0 : Np * Kp;
+        g.C = (float*)C; g.c_bstride = d.stride_c; g.ldc = d.ldc;
+        g.M = M; g.N = N;
+        g.tiles_m = (uint32_t)(Mp / X3_BM); g.tiles_n = (uint32_t)(Np / X3_BN);
+        g.general = d.general; g.alpha = d.alpha; g.beta = d.beta;
+        static bool attr_set = false;
+        if (!attr_set) {
+            CHAI_HIP_TRY(hipFuncSetAttribute((const void*)gemm_bf16x3,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)X3_LDS_BYTES));
+            attr_set = true;
+        }
+        dim3 grid(g.tiles_m * g.tiles_n, 1, (unsigned)nb);
+        gemm_bf16x3<<<grid, X3_THREADS, X3_LDS_BYTES, st>>>(g);
+        hipError_t e = hipGetLastError();
+        prof_mark(2, st);
+        CHAI_HIP_TRY(hipFreeAsync(ws, st));
+        if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3 launch");
+        return CHAI_OK;
+    }
+    if (mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_F64) {
+        const bool f64 = mode == CHAI_GEMM_F64;
+        const uint64_t BMx = f64 ? D_BM : F_BM, BNx = f64 ? D_BN : F_BN, BKx = f64 ? D_BK : F_BK;
+        const uint64_t Mp = round_up(M, BMx), Np = round_up(N, BNx), Kp = round_up(K, 32);
+        const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
+        const size_t es = f64 ? 8 : 4;
+        const size_t bytes = es * (Mp * Kp * na + Kp * Np * nbb);
+        void* ws = nullptr;
+        CHAI_HIP_TRY(hipMallocAsync(&ws, bytes, st));
+        char* pa = (char*)ws;
+        char* pb = pa + es * Mp * Kp * na;
+        // A packed [Mp][Kp] (op(A) row-major), B packed [Kp][Np] (op(B) row-major)
+        prof_mark(0, st);
+        if (f64) {
+            pack_any<double>(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, (double*)pa, st);
+            pack_any<double>(d.prec_b, B, d.ldb, d.trans_b != 0, K, N, Kp, Np, d.stride_b, nbb, (double*)pb, st);
+        } else {
+            pack_any<float>(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, (float*)pa, st);
+            pack_any<float>(d.prec_b, B, d.ldb, d.trans_b != 0, K, N, Kp, Np, d.stride_b, nbb, (float*)pb, st);
+        }
+        GemmArgs g{};
+        g.Kp = Kp; g.Np = Np;
+        g.pa_bstride = d.same_a ? 0 : Mp * Kp;
+        g.pb_bstride = d.same_b ? 0 : Kp * Np;
+        g.c_bstride = d.stride_c; g.ldc = d.ldc;
+        g.M = M; g.N = N;
+        g.tiles_m = (uint32_t)(Mp / BMx); g.tiles_n = (uint32_t)(Np / BNx);
+        g.general = d.general; g.alpha = d.alpha; g.beta = d.beta;
+        dim3 grid(g.tiles_m * g.tiles_n, 1, (unsigned)nb);
+        (void)BKx;
+        prof_mark(1, st);
+        if (f64) {
+            g.Ad = (const double*)pa; g.Bd = (const double*)pb; g.Cd = (double*)C;
+            dgemm_mfma<<<grid, 256, 0, st>>>(g);
+        } else {
+            g.Af = (const float*)pa; g.Bf = (const float*)pb; g.C = (float*)C;
+            sgemm_fma<<<grid, 256, 0, st>>>(g);
+        }
+        hipError_t e = hipGetLastError();
+        prof_mark(2, st);
+        CHAI_HIP_TRY(hipFreeAsync(ws, st));
+        if (e != hipSuccess) return hip_fail(e, f64 ? "dgemm_mfma launch" : "sgemm_fma launch");
+        return CHAI_OK;
+    }
+    set_error("gemm: unknown mode %d", mode);
+    return CHAI_ERR_ARG;
+}
+
+}  // namespace chai_hip
+
+using namespace chai_hip;
+
+extern "C" int chai_hip_profile_enable(int on)
+{
+    g_profile = on ? 1 : 0;
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_gemm_last_timing(float* prep_ms, float* core_ms)
+{
+    if (!t_prof.valid || !t_prof.e[0] || !t_prof.e[1] || !t_prof.e[2]) {
+        set_error("gemm_last_timing: no profiled GEMM on this thread (enable profiling first)");
+        return CHAI_ERR_STATE;
+    }
+    CHAI_HIP_TRY(hipEventSynchronize(t_prof.e[2]));
+    float a = 0.f, b = 0.f;
+    CHAI_HIP_TRY(hipEventElapsedTime(&a, t_prof.e[0], t_prof.e[1]));
+    CHAI_HIP_TRY(hipEventElapsedTime(&b, t_prof.e[1], t_prof.e[2]));
+    if (prep_ms) *prep_ms = a;
+    if (core_ms) *core_ms = b;
+    return CHAI_OK;
+}

@@ -1,0 +1,49 @@
+// gemm_api.hip -- chai_hip_gemm: argument validation and kernel-family choice.
+#include "common.hpp"
+#include "gemm_internal.hpp"
+
+using namespace chai_hip;
+
+extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
+                             void* stream)
+{
+    if (!desc) { set_error("gemm: null descriptor"); return CHAI_ERR_ARG; }
+    const chai_gemm_desc& d = *desc;
+    if (!valid_prec(d.prec_a) || !valid_prec(d.prec_b) || !valid_prec(d.prec_c)) {
+        set_error("gemm: bad precision (%d, %d, %d)", d.prec_a, d.prec_b, d.prec_c);
+        return CHAI_ERR_ARG;
+    }
+    if (d.M == 0 || d.N == 0) return CHAI_OK;
+    if (!A || !B || !C) { set_error("gemm: null operand"); return CHAI_ERR_ARG; }
+    const size_t a_cols = d.trans_a ? d.M : d.K, b_cols = d.trans_b ? d.K : d.N;
+    if (d.lda < a_cols || d.ldb < b_cols || d.ldc < d.N) {
+        set_error("gemm: leading dimension smaller than the row length");
+        return CHAI_ERR_ARG;
+    }
+    if (d.batch > 65535) { set_error("gemm: at most 65535 batch instances per call"); return CHAI_ERR_ARG; }
+    const bool any_f64 = d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64;
+    const bool ints_only = d.prec_a <= CHAI_I32 && d.prec_b <= CHAI_I32;
+    if (ints_only) {
+        set_error("gemm: integer x integer products are evaluated by the interpreter path");
+        return CHAI_ERR_ARG;
+    }
+    int mode = d.mode;
+    if (mode == CHAI_GEMM_AUTO) mode = any_f64 ? CHAI_GEMM_F64 : CHAI_GEMM_BF16X3;
+    if (any_f64 && mode != CHAI_GEMM_F64) {
+        set_error("gemm: f64 operands need CHAI_GEMM_F64");
+        return CHAI_ERR_ARG;
+    }
+    if (mode == CHAI_GEMM_F64 && d.prec_c != CHAI_F64) {
+        set_error("gemm: CHAI_GEMM_F64 writes an f64 C");
+        return CHAI_ERR_ARG;
+    }
+    if ((mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_BF16X3) && d.prec_c != CHAI_F32) {
+        set_error("gemm: FMA / BF16X3 write an f32 C");
+        return CHAI_ERR_ARG;
+    }
+    if (d.K == 0) {
+        set_error("gemm: K == 0");
+        return CHAI_ERR_ARG;
+    }
+    return gemm_launch(d, A, B, C, mode, as_stream(stream));
+}

@@ -1,0 +1,55 @@
+// gemm_device.hpp -- shared device helpers of the GEMM family.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace chai_hip {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+// Round-to-nearest-even f32 -> bf16 bits (finite inputs; NaN kept quiet).
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// a = hi + lo + r with hi = bf16(a), lo = bf16(a - hi); |r| <= 2^-17 |a|.
+__device__ __forceinline__ void split_bf16(float a, uint16_t& hi, uint16_t& lo)
+{
+    hi = f32_to_bf16_rne(a);
+    const float rem = a - bf16_bits_to_f32(hi);   // exact (Sterbenz)
+    lo = f32_to_bf16_rne(rem);
+}
+
+// XCD-aware tile order: the dispatcher hands consecutive workgroups to the 8
+// XCDs round-robin; remap so that the tiles one XCD runs together are
+// neighbours (shared A row panels / B column panels stay in that XCD's L2),
+// then walk tiles in groups of GROUP_M tile rows.
+template <int GROUP_M>
+__device__ __forceinline__ void tile_coords(uint32_t bid, uint32_t tiles_m, uint32_t tiles_n,
+                                            uint32_t& tm, uint32_t& tn)
+{
+    const uint32_t total = tiles_m * tiles_n;
+    uint32_t id = bid;
+    if ((total & 7u) == 0) {
+        const uint32_t per_xcd = total >> 3;
+        id = (bid & 7u) * per_xcd + (bid >> 3);
+    }
+    const uint32_t group_tiles = GROUP_M * tiles_n;
+    const uint32_t group = id / group_tiles;
+    const uint32_t first_m = group * GROUP_M;
+    const uint32_t gm = (tiles_m - first_m) < GROUP_M ? (tiles_m - first_m) : GROUP_M;
+    const uint32_t in_group = id - group * group_tiles;
+    tm = first_m + in_group % gm;
+    tn = in_group / gm;
+}
+
+}  // namespace chai_hip

@@ -1,0 +1,30 @@
+// gemm_internal.hpp -- kernel argument block and internal launchers of the
+// GEMM family (not part of the C ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "../../include/chai_mi355x.h"
+
+namespace chai_hip {
+
+struct GemmArgs {
+    // split-bf16 panels, K contiguous: A [Mp][Kp], B^T [Np][Kp]
+    const uint16_t *Ah, *Al, *Bh, *Bl;
+    // packed operands for the FMA / FP64 kernels: A [Mp][Kp], B [Kp][Np]
+    const float *Af, *Bf;
+    const double *Ad, *Bd;
+    float* C;
+    double* Cd;
+    uint64_t Kp, Np;
+    uint64_t pa_bstride, pb_bstride, c_bstride;   // elements between batch instances
+    uint64_t ldc, M, N;
+    uint32_t tiles_m, tiles_n;
+    int general;
+    double alpha, beta;
+};
+
+int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
+                hipStream_t st);
+
+}  // namespace chai_hip

@@ -1,0 +1,156 @@
+// reduce.hip -- sum / mean / dot over each slot, FP64 cross-block combine.
+//
+// The reference reduces with one work item looping over the whole array in
+// element precision (enqueue/EnqueueReduce.cpp:45-103); its interpreter
+// accumulates in double (interp/InterpAccum.cpp:34-63).  Here every thread
+// streams 16-byte vectors from HBM and accumulates in FP64 (integers in
+// int64, exact), waves reduce with cross-lane shuffles, blocks write one FP64
+// partial each, and a second single-block pass combines the partials of each
+// slot in a fixed order -- so results are run-to-run deterministic and agree
+// with the interpreter's double accumulation to within its own rounding.
+#include "common.hpp"
+#include "reduce_device.hpp"
+
+namespace chai_hip {
+
+constexpr int kRedThreads = 256;
+
+template <typename T>
+__device__ __forceinline__ double as_acc(T v) { return (double)v; }
+
+// Integer inputs accumulate exactly in int64 and are returned as double.
+template <typename T, bool IS_INT>
+__global__ __launch_bounds__(kRedThreads) void reduce_partial(const T* __restrict__ in, uint64_t n,
+                                                              uint32_t blocks_per_slot,
+                                                              double* __restrict__ partial)
+{
+    const uint32_t slot = blockIdx.y;
+    const T* p = in + (uint64_t)slot * n;
+    const uint64_t tid = (uint64_t)blockIdx.x * kRedThreads + threadIdx.x;
+    const uint64_t stride = (uint64_t)blocks_per_slot * kRedThreads;
+    double accd = 0.0;
+    long long acci = 0;
+    constexpr int VEC = 16 / sizeof(T);
+    const bool vec_ok = ((uintptr_t)p % 16) == 0;
+    uint64_t done = 0;
+    if (vec_ok) {
+        typedef T TV __attribute__((ext_vector_type(VEC)));
+        const TV* pv = reinterpret_cast<const TV*>(p);
+        const uint64_t nv = n / VEC;
+        for (uint64_t i = tid; i < nv; i += stride) {
+            TV v = __builtin_nontemporal_load(pv + i);
+#pragma unroll
+            for (int j = 0; j < VEC; j++) {
+                if (IS_INT) acci += (long long)v[j];
+                else accd += (double)v[j];
+            }
+        }
+        done = nv * VEC;
+    }
+    for (uint64_t i = done + tid; i < n; i += stride) {
+        if (IS_INT) acci += (long long)p[i];
+        else accd += (double)p[i];
+    }
+    double r;
+    if (IS_INT) r = (double)block_sum_i64<kRedThreads>(acci);
+    else r = block_sum_f64<kRedThreads>(accd);
+    if (threadIdx.x == 0) partial[(uint64_t)slot * blocks_per_slot + blockIdx.x] = r;
+}
+
+// dot(a, b) with the product formed in the promoted type (float*float -> float).
+template <typename TA, typename TB, typename TP>
+__global__ __launch_bounds__(kRedThreads) void dot_partial(const TA* __restrict__ a,
+                                                           const TB* __restrict__ b, uint64_t n,
+                                                           uint32_t blocks_per_slot,
+                                                           double* __restrict__ partial)
+{
+    const uint32_t slot = blockIdx.y;
+    const TA* pa = a + (uint64_t)slot * n;
+    const TB* pb = b + (uint64_t)slot * n;
+    const uint64_t tid = (uint64_t)blockIdx.x * kRedThreads + threadIdx.x;
+    const uint64_t stride = (uint64_t)blocks_per_slot * kRedThreads;
+    double acc = 0.0;
+    for (uint64_t i = tid; i < n; i += stride) {
+        const TP prod = (TP)pa[i] * (TP)pb[i];
+        acc += (double)prod;
+    }
+    const double r = block_sum_f64<kRedThreads>(acc);
+    if (threadIdx.x == 0) partial[(uint64_t)slot * blocks_per_slot + blockIdx.x] = r;
+}
+
+template <typename T>
+__device__ __forceinline__ T from_double(double v)
+{
+    if constexpr (std::is_same<T, uint32_t>::value) return (uint32_t)(long long)v;
+    else if constexpr (std::is_same<T, int32_t>::value) return (int32_t)(long long)v;
+    else return (T)v;
+}
+
+// Fixed-order combine of one slot's partials (one block per slot).
+template <typename T>
+__global__ __launch_bounds__(kRedThreads) void reduce_final(const double* __restrict__ partial,
+                                                            uint32_t blocks_per_slot, double divisor,
+                                                            T* __restrict__ out,
+                                                            double* __restrict__ out_f64)
+{
+    const uint32_t slot = blockIdx.x;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < blocks_per_slot; i += kRedThreads)
+        acc += partial[(uint64_t)slot * blocks_per_slot + i];
+    double r = block_sum_f64<kRedThreads>(acc);
+    if (threadIdx.x == 0) {
+        if (divisor != 1.0) r = r / divisor;
+        out[slot] = from_double<T>(r);
+        if (out_f64) out_f64[slot] = r;
+    }
+}
+
+static uint32_t pick_blocks(uint64_t n, size_t slots)
+{
+    // enough blocks to cover 256 CUs x 8 waves when there is one slot; each
+    // thread should see at least ~64 bytes of input.
+    uint64_t want = (n + kRedThreads * 16 - 1) / (kRedThreads * 16);
+    uint64_t cap = 2048 / (slots < 8 ? slots : 8);
+    if (want > cap) want = cap;
+    if (want < 1) want = 1;
+    return (uint32_t)want;
+}
+
+template <typename T>
+static int run_reduce(int op, const void* in, size_t n, size_t slots, void* out, double* out_f64,
+                      hipStream_t st)
+{
+    const uint32_t bps = pick_blocks(n, slots);
+    double* partial = nullptr;
+    CHAI_HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * bps * slots, st));
+    dim3 grid(bps, (unsigned)slots);
+    reduce_partial<T, std::is_integral<T>::value><<<grid, kRedThreads, 0, st>>>((const T*)in, n, bps, partial);
+    const double div = op == CHAI_OP_MEAN ? (double)n : 1.0;
+    reduce_final<T><<<(unsigned)slots, kRedThreads, 0, st>>>(partial, bps, div, (T*)out, out_f64);
+    hipError_t e = hipGetLastError();
+    CHAI_HIP_TRY(hipFreeAsync(partial, st));
+    if (e != hipSuccess) return hip_fail(e, "reduce launch");
+    return CHAI_OK;
+}
+
+}  // namespace chai_hip
+
+using namespace chai_hip;
+
+extern "C" int chai_hip_reduce(int op, int prec, const void* in, size_t n, size_t slots, void* out,
+                               double* out_f64, void* stream)
+{
+    if (op != CHAI_OP_SUM && op != CHAI_OP_MEAN) { set_error("reduce: op %d is not sum/mean", op); return CHAI_ERR_ARG; }
+    if (!valid_prec(prec)) { set_error("reduce: bad precision %d", prec); return CHAI_ERR_ARG; }
+    if (slots == 0) return CHAI_OK;
+    if (n == 0) { set_error("reduce: empty array"); return CHAI_ERR_ARG; }
+    if (slots > 65535) { set_error("reduce: at most 65535 slots per call"); return CHAI_ERR_ARG; }
+    if (!in || !out) { set_error("reduce: null pointer"); return CHAI_ERR_ARG; }
+    hipStream_t st = as_stream(stream);
+    switch (prec) {
+        case CHAI_U32: return run_reduce<uint32_t>(op, in, n, slots, out, out_f64, st);
+        case CHAI_I32: return run_reduce<int32_t>(op, in, n, slots, out, out_f64, st);
+        case CHAI_F32: return run_reduce<float>(op, in, n, slots, out, out_f64, st);
+        default: return run_reduce<double>(op, in, n, slots, out, out_f64, st);
+    }
+}

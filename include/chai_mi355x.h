@@ -1,0 +1,151 @@
+/*
+ * chai_mi355x.h -- C ABI of the MI355X (gfx950) back end for Chai.
+ *
+ * This library replaces, behind Chai's DeviceBase plug point
+ * (runtime/DeviceBase.hpp:55-60, runtime/DeviceMap.cpp:26-36), the OpenCL
+ * code emitters and their dispatch/caching layers:
+ *   autotune/Code*, autotune/EvergreenMatmulMM/MV  -> chai_hip_gemm / chai_hip_gemv
+ *   enqueue/Enqueue*, memmgr/MemTransJIT           -> chai_hip_trace_* / chai_hip_rng_fill / chai_hip_reduce
+ *   autotune/AutoTuneBench/EM, StandardEM           -> chai_hip_tune_*
+ *   autotune/AutoTuneMemo, enqueue/JustInTimeMemo   -> chai_hip_memo_* / chai_hip_jit_*
+ *   vendor/OCLdevice (buffers, queue, timing)       -> chai_hip_init / alloc / h2d / d2h / timer
+ *
+ * Conventions (kept from the reference, SURVEY-level contract):
+ *   - Every entry point returns an int status: CHAI_OK (0) or a negative code.
+ *     No exception crosses the ABI.  A failure maps to sub_evaluate() returning
+ *     false, which makes the scheduler re-queue the trace on another device
+ *     (runtime/Scheduler.cpp:221-254).  chai_hip_last_error() describes the
+ *     last failure of the calling thread.
+ *   - Precision codes are Chai's PrecType codes (misc/PrecType.hpp:18-21).
+ *   - Arrays are dense row-major; "slots" are independent instances stacked
+ *     one after another (a VectorTrace's gathered traces, common/VectorTrace.cpp).
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Entry points enqueue work and return; call chai_hip_sync() or
+ *     synchronize the stream before reading results on the host.
+ *   - Device pointers come from chai_hip_alloc() or any HIP allocation.
+ */
+#ifndef CHAI_MI355X_H
+#define CHAI_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ codes */
+enum {
+    CHAI_OK = 0,
+    CHAI_ERR_ARG = -1,        /* invalid argument / unsupported combination */
+    CHAI_ERR_HIP = -2,        /* HIP runtime error                          */
+    CHAI_ERR_NOMEM = -3,      /* allocation failed                          */
+    CHAI_ERR_STATE = -4,      /* library not initialised / bad handle       */
+    CHAI_ERR_IO = -5          /* journal file could not be read or written  */
+};
+
+/* PrecType codes (misc/PrecType.hpp:18-21) */
+enum { CHAI_U32 = 0, CHAI_I32 = 1, CHAI_F32 = 4, CHAI_F64 = 8 };
+
+/* RNG variants (api/chai.h:412-416) */
+enum { CHAI_RNG_PHILOX = 1230, CHAI_RNG_THREEFRY = 1231 };
+
+/* reduction op codes = Chai bytecodes (bytecode/ByteCodes.hpp:256-258) */
+enum { CHAI_OP_DOT = 130, CHAI_OP_MEAN = 131, CHAI_OP_SUM = 132 };
+
+/* ------------------------------------------------------------- lifecycle */
+/* Replaces OCLinit + Translator::sub_initDevice (runtime/Translator.cpp:353-364). */
+int chai_hip_init(int device_ordinal);
+int chai_hip_shutdown(void);
+const char* chai_hip_last_error(void);
+/* Device properties: CU count, LDS bytes per CU, clock (kHz), arch name. */
+int chai_hip_device_info(int* cu_count, int* lds_per_cu, int* clock_khz,
+                         char* arch_name, size_t arch_len);
+
+/* ---------------------------------------------------------------- memory */
+/* Replaces the OCLdevice buffer heap (vendor/OCLdevice.cpp) and
+ * MemTrans host<->device transfers (memmgr/MemTrans.hpp). */
+int chai_hip_alloc(size_t bytes, void** out);
+int chai_hip_free(void* ptr);
+int chai_hip_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int chai_hip_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int chai_hip_memset(void* dst, int value, size_t bytes, void* stream);
+int chai_hip_sync(void* stream);
+
+/* ----------------------------------------------------------------- timing */
+/* hipEvent timer on a stream; replaces gettimeofday timing
+ * (autotune/AutoTuneBench.cpp:116-123).  Handles are opaque. */
+int chai_hip_timer_create(void** timer);
+int chai_hip_timer_start(void* timer, void* stream);
+int chai_hip_timer_stop(void* timer, void* stream);
+int chai_hip_timer_elapsed_ms(void* timer, float* ms);   /* waits for stop */
+int chai_hip_timer_destroy(void* timer);
+
+/* -------------------------------------------------------------------- RNG */
+/* Fills out[slots][H][W] with Chai's counter-based stream.
+ * Replaces Enqueue::visit(AstRNGuniform/AstRNGnormal) (enqueue/EnqueueAST.cpp:496-546)
+ * and the Random123 text emitters (Random123/R123Philox.cpp:288-520,
+ * Random123/R123Threefry.cpp:330-520).
+ *   variant : CHAI_RNG_PHILOX | CHAI_RNG_THREEFRY
+ *   prec    : CHAI_U32 | CHAI_I32 | CHAI_F32 | CHAI_F64
+ *   vec_len : 1, 2, 4 (32-bit) or 1, 2 (64-bit); 0 = PrecType default (4 / 2)
+ *   seed    : generator seed of this call (api/RngFun.cpp:24-30)
+ *   normal  : Box-Muller normals (CHAI_F32 / CHAI_F64 only)
+ *   has_limits, lo, hi : uniform in [lo, hi] as u*(hi-lo)+lo
+ */
+int chai_hip_rng_fill(int variant, int prec, int vec_len, uint64_t seed,
+                      size_t W, size_t H, size_t slots, int normal,
+                      int has_limits, double lo, double hi,
+                      void* out, void* stream);
+
+/* ------------------------------------------------------------- reductions */
+/* out[s] = sum / mean of in[s][0..n) (interp/InterpAccum.cpp:13-70 semantics:
+ * accumulation is exact-in-FP64-combine, result in prec's type).  Replaces the
+ * serial one-work-item reduction of enqueue/EnqueueReduce.cpp:45-103.
+ * out_f64 (nullable) also receives the FP64 value before rounding. */
+int chai_hip_reduce(int op, int prec, const void* in, size_t n, size_t slots,
+                    void* out, double* out_f64, void* stream);
+
+/* ------------------------------------------------------------------- GEMM */
+/* Kernel family selector.  AUTO picks BF16X3 for f32/int operands with an f32
+ * result and F64 when any operand is f64. */
+enum {
+    CHAI_GEMM_AUTO = 0,
+    CHAI_GEMM_FMA = 1,      /* plain FP32 VALU FMA                        */
+    CHAI_GEMM_BF16X3 = 2,   /* split-BF16 three-product MFMA, FP32 accum  */
+    CHAI_GEMM_F64 = 3       /* FP64 MFMA                                  */
+};
+
+/* One GEMM call (or a batch of identical-shape instances).  Mirrors the
+ * exogenous parameters of autotune/KernelParams.hpp:16-371 (precA/B/C, M, N, K,
+ * transA/B, general, batching, sameDataA/B) and StmtMatmul's
+ * C = op(A) op(B) / C = alpha op(A) op(B) + beta C (api/MatmulFun.cpp:26-54). */
+typedef struct chai_gemm_desc {
+    int prec_a, prec_b, prec_c;     /* CHAI_U32/I32/F32/F64                     */
+    int trans_a, trans_b;           /* op(X) = X^T when set                     */
+    size_t M, N, K;                 /* op(A) is M x K, op(B) is K x N           */
+    size_t lda, ldb, ldc;           /* row strides of the stored arrays (elems) */
+    int general;                    /* 1: C = alpha AB + beta C                 */
+    double alpha, beta;
+    size_t batch;                   /* instances (0 or 1 = single)              */
+    int same_a, same_b;             /* batch shares A / B (sameDataA/B)         */
+    size_t stride_a, stride_b, stride_c; /* elements between instances          */
+    int mode;                       /* CHAI_GEMM_*                              */
+} chai_gemm_desc;
+
+/* Replaces Enqueue::visit(StmtMatmul/StmtMatmulAuto) and the Evergreen MM
+ * kernels (enqueue/EnqueueMatmul.cpp:24-161, autotune/EvergreenMatmulMM.cpp:1077-1630). */
+int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C, void* stream);
+
+/* Per-kernel timing (the replacement for AutoTuneBench's timer hooks,
+ * autotune/AutoTuneBench.cpp:116-123): when enabled, chai_hip_gemm records
+ * hipEvents on its launch stream around the operand pre-pass (split / pack)
+ * and around the core GEMM kernel; chai_hip_gemm_last_timing waits for the
+ * calling thread's most recent profiled call and returns both durations. */
+int chai_hip_profile_enable(int on);
+int chai_hip_gemm_last_timing(float* prep_ms, float* core_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CHAI_MI355X_H */

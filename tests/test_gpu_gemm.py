@@ -1,0 +1,147 @@
+"""GPU parity: chai_hip_gemm (FP32-FMA, split-BF16x3 MFMA, FP64 MFMA, mixed
+int x float) against the oracle's FP64 GEMM and interpreter restatement.
+
+Tolerances (stated per element, deterministic error bounds):
+  FMA     |C - C64| <= gamma_K * sum_k |a_k b_k|,  gamma_K = K u / (1 - K u), u = 2^-24
+  BF16X3  |C - C64| <= (3 * 2^-18 + gamma_K) * sum_k |a_k b_k|   (split residual
+          |x - hi - lo| <= 2^-18 |x| per operand, dropped lo*lo term)
+  F64     |C - C64| <= 2 K 2^-53 * sum_k |a_k b_k|
+and the interpreter's own result (product rounded to f32, double sum) lies
+within u * sum|ab| + u |C| of C64, so GPU-vs-interpreter uses the sum of both.
+"""
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+U = 2.0 ** -24
+
+
+def unif(oracle, seed, rows, cols, dtype=np.float32):
+    prec = 8 if dtype == np.float64 else 4
+    return oracle.rng_fill(1230, prec, 0, seed, cols, rows, 1, limits=(-1.0, 1.0)).reshape(rows, cols)
+
+
+def bound(mode, K):
+    g = K * U / (1 - K * U)
+    if mode == "fma":
+        return g
+    if mode == "bf16x3":
+        return 3 * 2.0 ** -18 + g
+    return 2 * K * 2.0 ** -53
+
+
+def check(mode, C, A, B, oracle, ta=False, tb=False):
+    A64 = (A.T if ta else A).astype(np.float64)
+    B64 = (B.T if tb else B).astype(np.float64)
+    C64 = A64 @ B64
+    absprod = np.abs(A64) @ np.abs(B64)
+    K = A64.shape[1]
+    err = np.abs(C.astype(np.float64) - C64)
+    assert np.all(err <= bound(mode, K) * absprod + 1e-300), (mode, float(err.max()))
+    return err
+
+
+MODES = {"fma": 1, "bf16x3": 2}
+
+
+@pytest.mark.parametrize("mode", ["fma", "bf16x3"])
+@pytest.mark.parametrize("M,N,K", [(256, 256, 256), (1, 1, 1), (17, 33, 65), (300, 129, 1000),
+                                   (512, 768, 96), (1024, 1024, 1024)])
+def test_gemm_f32(chai, oracle, mode, M, N, K):
+    print("case", mode, M, N, K, flush=True)
+    A = unif(oracle, 42, M, K); B = unif(oracle, 43, K, N)
+    C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), mode=MODES[mode])
+    C = C.cpu().numpy()
+    check(mode, C, A, B, oracle)
+    if M * N * K <= 64 ** 3:
+        I = oracle.matmul_interp(A, B)
+        tol = bound(mode, K) * (np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64)))
+        tol = tol + U * (np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))) + U * np.abs(I)
+        assert np.all(np.abs(C.astype(np.float64) - I) <= tol)
+
+
+@pytest.mark.parametrize("mode", ["fma", "bf16x3"])
+@pytest.mark.parametrize("ta,tb", [(True, False), (False, True), (True, True)])
+def test_gemm_transposes(chai, oracle, mode, ta, tb):
+    M, N, K = 130, 70, 300
+    A = unif(oracle, 1, K, M) if ta else unif(oracle, 1, M, K)
+    B = unif(oracle, 2, N, K) if tb else unif(oracle, 2, K, N)
+    C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), trans_a=ta, trans_b=tb,
+                  mode=MODES[mode]).cpu().numpy()
+    check(mode, C, A, B, oracle, ta, tb)
+
+
+@pytest.mark.parametrize("mode", ["fma", "bf16x3"])
+def test_gemm_general_and_batch(chai, oracle, mode):
+    M, N, K, nb = 96, 200, 64, 3
+    A = np.stack([unif(oracle, 10 + i, M, K) for i in range(nb)])
+    B = unif(oracle, 20, K, N)                        # shared B (sameDataB)
+    C0 = np.stack([unif(oracle, 30 + i, M, N) for i in range(nb)])
+    Ct = torch.from_numpy(C0.copy()).cuda()
+    chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), C=Ct, general=True,
+              alpha=0.5, beta=-1.5, mode=MODES[mode])
+    C = Ct.cpu().numpy()
+    for i in range(nb):
+        ab = A[i].astype(np.float64) @ B.astype(np.float64)
+        ref = 0.5 * ab - 1.5 * C0[i].astype(np.float64)
+        absprod = np.abs(A[i].astype(np.float64)) @ np.abs(B.astype(np.float64))
+        tol = 0.5 * bound(mode, K) * absprod + U * (np.abs(ref) + 1.5 * np.abs(C0[i])) + 1e-30
+        assert np.all(np.abs(C[i] - ref) <= tol)
+
+
+def test_gemm_sentinel_ones(chai):
+    """Autotune sentinel (EvergreenMatmulMM.cpp:800-822): A = B = 1 => C = K exactly."""
+    for mode in (1, 2):
+        K = 777
+        A = torch.ones(300, K, device="cuda"); B = torch.ones(K, 260, device="cuda")
+        C = chai.gemm(A, B, mode=mode)
+        assert torch.all(C == K).item()
+
+
+def test_gemm_mixed_int_float(chai, oracle):
+    rng = np.random.default_rng(7)
+    M = N = K = 512
+    Ai = rng.integers(-128, 128, size=(M, K)).astype(np.int32)
+    B = unif(oracle, 3, K, N)
+    C = chai.gemm(torch.from_numpy(Ai).cuda(), torch.from_numpy(B).cuda(), mode=2,
+                  out_dtype=torch.float32).cpu().numpy()
+    C64 = Ai.astype(np.float64) @ B.astype(np.float64)
+    absprod = np.abs(Ai.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    # the int operand is exact in bf16 (|a| < 256): only B's split residual and f32 accumulation
+    assert np.all(np.abs(C - C64) <= (2.0 ** -17 + K * U) * absprod)
+
+
+@pytest.mark.parametrize("M,N,K", [(128, 128, 128), (65, 97, 33), (512, 384, 640)])
+def test_dgemm(chai, oracle, M, N, K):
+    A = unif(oracle, 5, M, K, np.float64); B = unif(oracle, 6, K, N, np.float64)
+    C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), mode=3).cpu().numpy()
+    check("f64", C, A, B, oracle)
+    ref = oracle.gemm_f64ref(A, B, nthreads=4)
+    assert np.all(np.abs(C - ref) <= 4 * K * 2.0 ** -53 * (np.abs(A) @ np.abs(B)))
+
+
+def test_gemm_8192_sampled_rows(chai, oracle):
+    """The headline shape, bf16x3 and FMA, checked on 8 sampled rows against FP64."""
+    n = 8192
+    A = unif(oracle, 42, n, n); B = unif(oracle, 43, n, n)
+    At, Bt = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    rows = np.random.default_rng(0).choice(n, 8, replace=False)
+    Bd = B.astype(np.float64)
+    C64 = A[rows].astype(np.float64) @ Bd
+    absprod = np.abs(A[rows].astype(np.float64)) @ np.abs(Bd)
+    for name, mode in (("fma", 1), ("bf16x3", 2)):
+        C = chai.gemm(At, Bt, mode=mode)[torch.from_numpy(rows).cuda()].cpu().numpy()
+        err = np.abs(C - C64)
+        assert np.all(err <= bound(name, n) * absprod), (name, float(err.max()))
+        print(name, "max abs err vs FP64", float(err.max()), "rms", float(np.sqrt((err ** 2).mean())))
+
+
+def test_gemm_rejects_bad_arguments(chai):
+    A = torch.ones(4, 4, device="cuda")
+    with pytest.raises(ValueError):
+        chai.gemm(A, torch.ones(5, 4, device="cuda"))  # inner dimension mismatch (binding check)
+    Ai = torch.ones(4, 4, dtype=torch.int32, device="cuda")
+    with pytest.raises(chai.ChaiError):
+        chai.gemm(Ai, Ai, out_dtype=torch.float32)
