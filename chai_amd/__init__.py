@@ -35,7 +35,7 @@ class GemmDesc(ctypes.Structure):
         ("general", ctypes.c_int), ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
         ("batch", ctypes.c_size_t), ("same_a", ctypes.c_int), ("same_b", ctypes.c_int),
         ("stride_a", ctypes.c_size_t), ("stride_b", ctypes.c_size_t), ("stride_c", ctypes.c_size_t),
-        ("mode", ctypes.c_int),
+        ("mode", ctypes.c_int), ("variant", ctypes.c_int),
     ]
 
 
@@ -66,6 +66,8 @@ def lib():
         L.chai_hip_rng_fill.argtypes = [i, i, i, ctypes.c_uint64, sz, sz, sz, i, i, d, d, vp, vp]
         L.chai_hip_reduce.argtypes = [i, i, vp, sz, sz, vp, vp, vp]
         L.chai_hip_gemm.argtypes = [ctypes.POINTER(GemmDesc), vp, vp, vp, vp]
+        L.chai_hip_gemm_num_variants.argtypes = [i, ctypes.POINTER(i)]
+        L.chai_hip_gemm_variant_params.argtypes = [i, i, ctypes.POINTER(i)]
         L.chai_hip_profile_enable.argtypes = [i]
         L.chai_hip_gemm_last_timing.argtypes = [ctypes.POINTER(ctypes.c_float),
                                                 ctypes.POINTER(ctypes.c_float)]
@@ -141,8 +143,20 @@ def reduce(x, op="sum", slots=1, prec=None):
 
 
 # --------------------------------------------------------------------- GEMM
+def gemm_variants(mode=GEMM_BF16X3):
+    """[(BM, BN, BK, waves_m, waves_n), ...] of a kernel family's tile variants."""
+    n = ctypes.c_int()
+    check(lib().chai_hip_gemm_num_variants(mode, ctypes.byref(n)), "gemm_num_variants")
+    out = []
+    for v in range(n.value):
+        p = (ctypes.c_int * 5)()
+        if lib().chai_hip_gemm_variant_params(mode, v, p) == 0:
+            out.append(tuple(p))
+    return out
+
+
 def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, beta=0.0,
-         general=False, out_dtype=None, prec_a=None, prec_b=None):
+         general=False, out_dtype=None, prec_a=None, prec_b=None, variant=-1):
     """C = op(A) op(B) (or alpha op(A) op(B) + beta C) on 2-D / 3-D (batched) tensors."""
     import torch
     init()
@@ -174,6 +188,7 @@ def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, 
     d.stride_b = B.stride(0) if B.dim() == 3 else 0
     d.stride_c = C.stride(0) if C.dim() == 3 else 0
     d.mode = mode
+    d.variant = variant
     check(lib().chai_hip_gemm(ctypes.byref(d), _ptr(A), _ptr(B), _ptr(C), _stream()), "chai_hip_gemm")
     return C
 

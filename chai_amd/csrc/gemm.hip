@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "gemm_device.hpp"
 #include "gemm_internal.hpp"
+#include "gemm_x3.hpp"
 
 #include <type_traits>
 
@@ -117,127 +118,33 @@ __global__ __launch_bounds__(256) void pack_kernel(const T* __restrict__ src, ui
 }
 
 // =====================================================================
-// split-bf16x3 MFMA GEMM
+// split-bf16x3 MFMA GEMM: tile variants of gemm_x3_kernel (gemm_x3.hpp)
 // =====================================================================
 
-constexpr int X3_BM = 256, X3_BN = 256, X3_BK = 32, X3_THREADS = 512;
-constexpr int X3_PANEL = X3_BM * X3_BK / 8;           // uint4 chunks per panel per stage
-constexpr int X3_STAGE = 4 * X3_PANEL;                 // Ah, Al, Bh, Bl
-constexpr size_t X3_LDS_BYTES = 2 * X3_STAGE * sizeof(uint4);
+struct X3Variant {
+    int BM, BN, BK, WGM, WGN;
+    size_t lds;
+    int threads;
+    void (*kernel)(GemmArgs);
+};
 
-__device__ __forceinline__ bf16x8 as_bf16x8(uint4 v)
-{
-    bf16x8 r;
-    __builtin_memcpy(&r, &v, 16);
-    return r;
-}
+#define CHAI_X3V(BM, BN, BK, WGM, WGN, MINB)                                              \
+    {BM, BN, BK, WGM, WGN, X3Shape<BM, BN, BK, WGM, WGN>::LDS_BYTES, WGM * WGN * 64,        \
+     gemm_x3_kernel<BM, BN, BK, WGM, WGN, MINB>}
 
-// Epilogue shared by the MFMA kernels: C = v, or alpha*v + beta*C in double.
-__device__ __forceinline__ void store_c(float* C, uint64_t idx, float v, const GemmArgs& a)
-{
-    if (a.general) C[idx] = (float)(a.alpha * (double)v + a.beta * (double)C[idx]);
-    else C[idx] = v;
-}
-
-__global__ __launch_bounds__(X3_THREADS, 1) void gemm_bf16x3(GemmArgs a)
-{
-    extern __shared__ uint4 lds[];
-    uint32_t tm, tn;
-    tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
-    const uint64_t bz = blockIdx.z;
-    const uint64_t Kp = a.Kp;
-    const uint16_t* src[4] = {
-        a.Ah + bz * a.pa_bstride + (uint64_t)tm * X3_BM * Kp,
-        a.Al + bz * a.pa_bstride + (uint64_t)tm * X3_BM * Kp,
-        a.Bh + bz * a.pb_bstride + (uint64_t)tn * X3_BN * Kp,
-        a.Bl + bz * a.pb_bstride + (uint64_t)tn * X3_BN * Kp};
-    float* C = a.C + bz * a.c_bstride;
-
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave >> 2) * 128;
-    const int wn = (wave & 3) * 64;
-
-    floatx16 acc[4][2];
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++)
-#pragma unroll
-            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
-
-    uint4 st[8];
-    // chunk j of this thread: panel j>>1, row (t + 512*(j&1)) >> 2, 16-byte column q = t & 3
-    const int q_ld = t & 3;
-    auto gload = [&](int kt) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int row = (t + 512 * (j & 1)) >> 2;
-            st[j] = *(reinterpret_cast<const uint4*>(src[j >> 1] + (uint64_t)row * Kp + (uint64_t)kt * X3_BK) + q_ld);
-        }
-    };
-    auto lstore = [&](int buf) {
-#pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const int row = (t + 512 * (j & 1)) >> 2;
-            const int p = q_ld ^ ((row >> 2) & 3);
-            lds[buf * X3_STAGE + (j >> 1) * X3_PANEL + row * 4 + p] = st[j];
-        }
-    };
-
-    const int nk = (int)(Kp / X3_BK);
-    gload(0);
-    lstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt++) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
-        const uint4* L = lds + buf * X3_STAGE;
-#pragma unroll
-        for (int kb = 0; kb < 2; kb++) {
-            const int q = kb * 2 + (lane >> 5);
-            bf16x8 ah[4], al[4], bh[2], bl[2];
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int row = wm + i * 32 + (lane & 31);
-                const int p = q ^ ((row >> 2) & 3);
-                ah[i] = as_bf16x8(L[0 * X3_PANEL + row * 4 + p]);
-                al[i] = as_bf16x8(L[1 * X3_PANEL + row * 4 + p]);
-            }
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const int row = wn + j * 32 + (lane & 31);
-                const int p = q ^ ((row >> 2) & 3);
-                bh[j] = as_bf16x8(L[2 * X3_PANEL + row * 4 + p]);
-                bl[j] = as_bf16x8(L[3 * X3_PANEL + row * 4 + p]);
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 2; j++) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-                }
-        }
-        if (kt + 1 < nk) lstore(buf ^ 1);
-        __syncthreads();
-    }
-
-    // 32x32 output layout: lane -> column lane%32; register r -> row 8*(r/4) + 4*(lane/32) + r%4
-    const uint64_t row_base = (uint64_t)tm * X3_BM + wm;
-    const uint64_t col_base = (uint64_t)tn * X3_BN + wn;
-#pragma unroll
-    for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const uint64_t col = col_base + j * 32 + (lane & 31);
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const uint64_t row = row_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
-            }
-        }
-}
+// The autotuner's search space for CHAI_GEMM_BF16X3 (index = variant id).
+static const X3Variant kX3Variants[] = {
+    CHAI_X3V(256, 256, 32, 2, 4, 1),   // 0: 8 waves of 128x64
+    CHAI_X3V(256, 256, 32, 2, 2, 1),   // 1: 4 waves of 128x128
+    CHAI_X3V(128, 128, 32, 2, 2, 2),   // 2: 4 waves of 64x64, 2 blocks / CU
+    CHAI_X3V(256, 128, 32, 2, 2, 1),   // 3: 4 waves of 128x64
+    CHAI_X3V(128, 256, 32, 2, 2, 1),   // 4: 4 waves of 64x128
+    CHAI_X3V(128, 128, 64, 2, 2, 1),   // 5: BK 64
+    CHAI_X3V(128, 256, 32, 2, 4, 1),   // 6: 8 waves of 64x64
+    CHAI_X3V(256, 128, 32, 4, 2, 1),   // 7: 8 waves of 64x64
+};
+static constexpr int kNumX3 = sizeof(kX3Variants) / sizeof(kX3Variants[0]);
+static int g_x3_default = 0;
 
 // =====================================================================
 // plain FP32 VALU FMA GEMM
@@ -268,15 +175,15 @@ __global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
 
     // A tile 128 x 16: thread loads rows (t>>2) and (t>>2)+64, 4 consecutive k at (t&3)*4
     // B tile 16 x 128: thread loads k rows (t>>5) and (t>>5)+8, 4 consecutive n at (t&31)*4
-    float4 ra[2], rb[2];
+    f32x4 ra[2], rb[2];
     auto gload = [&](int kt) {
         const uint64_t k0 = (uint64_t)kt * F_BK;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int row = (t >> 2) + h * 64;
-            ra[h] = *reinterpret_cast<const float4*>(A + (uint64_t)row * Kp + k0 + (t & 3) * 4);
+            ra[h] = *reinterpret_cast<const f32x4*>(A + (uint64_t)row * Kp + k0 + (t & 3) * 4);
             const int kr = (t >> 5) + h * 8;
-            rb[h] = *reinterpret_cast<const float4*>(B + (k0 + kr) * Np + (t & 31) * 4);
+            rb[h] = *reinterpret_cast<const f32x4*>(B + (k0 + kr) * Np + (t & 31) * 4);
         }
     };
     auto lstore = [&](int buf) {
@@ -288,7 +195,7 @@ __global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
             As[buf][kc + 2][row] = ra[h].z;
             As[buf][kc + 3][row] = ra[h].w;
             const int kr = (t >> 5) + h * 8;
-            *reinterpret_cast<float4*>(&Bs[buf][kr][(t & 31) * 4]) = rb[h];
+            *reinterpret_cast<f32x4*>(&Bs[buf][kr][(t & 31) * 4]) = rb[h];
         }
     };
     const int nk = (int)(Kp / F_BK);
@@ -300,10 +207,10 @@ __global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
         if (kt + 1 < nk) gload(kt + 1);
 #pragma unroll 2
         for (int k = 0; k < F_BK; k++) {
-            const float4 a0 = *reinterpret_cast<const float4*>(&As[buf][k][ty * 4]);
-            const float4 a1 = *reinterpret_cast<const float4*>(&As[buf][k][ty * 4 + 64]);
-            const float4 b0 = *reinterpret_cast<const float4*>(&Bs[buf][k][tx * 4]);
-            const float4 b1 = *reinterpret_cast<const float4*>(&Bs[buf][k][tx * 4 + 64]);
+            const f32x4 a0 = *reinterpret_cast<const f32x4*>(&As[buf][k][ty * 4]);
+            const f32x4 a1 = *reinterpret_cast<const f32x4*>(&As[buf][k][ty * 4 + 64]);
+            const f32x4 b0 = *reinterpret_cast<const f32x4*>(&Bs[buf][k][tx * 4]);
+            const f32x4 b1 = *reinterpret_cast<const f32x4*>(&Bs[buf][k][tx * 4 + 64]);
             const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
             const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
 #pragma unroll
@@ -355,11 +262,11 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
 
     // A tile 128 x 8 doubles: thread loads row t>>1, 4 consecutive k at (t&1)*4
     // B tile 8 x 128 doubles: thread loads k row t>>5, 4 consecutive n at (t&31)*4
-    double2 ra[2], rb[2];
+    f64x2 ra[2], rb[2];
     auto gload = [&](int kt) {
         const uint64_t k0 = (uint64_t)kt * D_BK;
-        const double2* pa = reinterpret_cast<const double2*>(A + (uint64_t)(t >> 1) * Kp + k0 + (t & 1) * 4);
-        const double2* pb = reinterpret_cast<const double2*>(B + (k0 + (t >> 5)) * Np + (t & 31) * 4);
+        const f64x2* pa = reinterpret_cast<const f64x2*>(A + (uint64_t)(t >> 1) * Kp + k0 + (t & 1) * 4);
+        const f64x2* pb = reinterpret_cast<const f64x2*>(B + (k0 + (t >> 5)) * Np + (t & 31) * 4);
 #pragma unroll
         for (int h = 0; h < 2; h++) { ra[h] = pa[h]; rb[h] = pb[h]; }
     };
@@ -397,7 +304,8 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
         if (kt + 1 < nk) lstore(buf ^ 1);
         __syncthreads();
     }
-    // 16x16 output layout: lane -> column lane%16; register r -> row 4*(lane/16) + r
+    // f64 16x16x4 output layout (measured on gfx950): lane -> column lane%16;
+    // register r -> row 4*r + lane/16 (A row i supplied by lane i + 16k).
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -405,7 +313,7 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
             const uint64_t col = (uint64_t)tn * D_BN + wn + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const uint64_t row = (uint64_t)tm * D_BM + wm + i * 16 + 4 * (lane >> 4) + r;
+                const uint64_t row = (uint64_t)tm * D_BM + wm + i * 16 + 4 * r + (lane >> 4);
                 if (row < a.M && col < a.N) {
                     const uint64_t idx = row * a.ldc + col;
                     const double v = acc[i][j][r];
@@ -490,14 +398,17 @@ static void pack_any(int prec, const void* src, uint64_t ld, bool trans, uint64_
 }
 
 int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
-                hipStream_t st)
+                int variant, hipStream_t st)
 {
     const uint64_t M = d.M, N = d.N, K = d.K, nb = d.batch ? d.batch : 1;
     const uint64_t a_rows = d.trans_a ? K : M;   // stored layout of A
     (void)a_rows;
     if (mode == CHAI_GEMM_BF16X3) {
         if (d.prec_c != CHAI_F32) { set_error("gemm: bf16x3 writes f32 C"); return CHAI_ERR_ARG; }
-        const uint64_t Mp = round_up(M, X3_BM), Np = round_up(N, X3_BN), Kp = round_up(K, 64);
+        const int vid = variant < 0 ? g_x3_default : variant;
+        if (vid >= kNumX3) { set_error("gemm: bf16x3 variant %d out of range", vid); return CHAI_ERR_ARG; }
+        const X3Variant& v = kX3Variants[vid];
+        const uint64_t Mp = round_up(M, v.BM), Np = round_up(N, v.BN), Kp = round_up(K, 64);
         const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
         const size_t bytes = 2 * sizeof(uint16_t) * (Mp * Kp * na + Np * Kp * nbb);
         uint16_t* ws = nullptr;
@@ -519,20 +430,20 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         g.pb_bstride = d.same_b ? 0 : Np * Kp;
         g.C = (float*)C; g.c_bstride = d.stride_c; g.ldc = d.ldc;
         g.M = M; g.N = N;
-        g.tiles_m = (uint32_t)(Mp / X3_BM); g.tiles_n = (uint32_t)(Np / X3_BN);
+        g.tiles_m = (uint32_t)(Mp / v.BM); g.tiles_n = (uint32_t)(Np / v.BN);
         g.general = d.general; g.alpha = d.alpha; g.beta = d.beta;
-        static bool attr_set = false;
-        if (!attr_set) {
-            CHAI_HIP_TRY(hipFuncSetAttribute((const void*)gemm_bf16x3,
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)X3_LDS_BYTES));
-            attr_set = true;
+        static bool attr_set[kNumX3] = {};
+        if (!attr_set[vid]) {
+            CHAI_HIP_TRY(hipFuncSetAttribute((const void*)v.kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds));
+            attr_set[vid] = true;
         }
         dim3 grid(g.tiles_m * g.tiles_n, 1, (unsigned)nb);
-        gemm_bf16x3<<<grid, X3_THREADS, X3_LDS_BYTES, st>>>(g);
+        hipLaunchKernelGGL(v.kernel, grid, dim3(v.threads), v.lds, st, g);
         hipError_t e = hipGetLastError();
         prof_mark(2, st);
         CHAI_HIP_TRY(hipFreeAsync(ws, st));
-        if (e != hipSuccess) return hip_fail(e, "gemm_bf16x3 launch");
+        if (e != hipSuccess) return hip_fail(e, "gemm_x3 launch");
         return CHAI_OK;
     }
     if (mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_F64) {
@@ -586,6 +497,25 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
 }  // namespace chai_hip
 
 using namespace chai_hip;
+
+namespace chai_hip {
+
+int gemm_num_variants(int mode) { return mode == CHAI_GEMM_BF16X3 ? kNumX3 : 1; }
+
+int gemm_variant_params(int mode, int vid, int* p)
+{
+    if (mode != CHAI_GEMM_BF16X3 || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;
+    const X3Variant& v = kX3Variants[vid];
+    p[0] = v.BM; p[1] = v.BN; p[2] = v.BK; p[3] = v.WGM; p[4] = v.WGN;
+    return CHAI_OK;
+}
+
+void gemm_set_default_variant(int mode, int vid)
+{
+    if (mode == CHAI_GEMM_BF16X3 && vid >= 0 && vid < kNumX3) g_x3_default = vid;
+}
+
+}  // namespace chai_hip
 
 extern "C" int chai_hip_profile_enable(int on)
 {

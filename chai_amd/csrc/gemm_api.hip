@@ -45,5 +45,24 @@ extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const vo
         set_error("gemm: K == 0");
         return CHAI_ERR_ARG;
     }
-    return gemm_launch(d, A, B, C, mode, as_stream(stream));
+    if (d.variant >= gemm_num_variants(mode)) {
+        set_error("gemm: variant %d out of range for mode %d", d.variant, mode);
+        return CHAI_ERR_ARG;
+    }
+    return gemm_launch(d, A, B, C, mode, d.variant, as_stream(stream));
+}
+
+extern "C" int chai_hip_gemm_num_variants(int mode, int* count)
+{
+    if (!count) { set_error("gemm_num_variants: null out"); return CHAI_ERR_ARG; }
+    *count = gemm_num_variants(mode);
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_gemm_variant_params(int mode, int variant, int* params)
+{
+    if (!params) { set_error("gemm_variant_params: null out"); return CHAI_ERR_ARG; }
+    const int rc = gemm_variant_params(mode, variant, params);
+    if (rc != CHAI_OK) set_error("gemm_variant_params: no variant %d for mode %d", variant, mode);
+    return rc;
 }

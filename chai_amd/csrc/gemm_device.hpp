@@ -9,6 +9,11 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef double doublex4 __attribute__((ext_vector_type(4)));
+// Native 16-byte vectors for staging through registers and LDS (HIP's uint4 /
+// float4 are structs with a union, which the compiler may keep in scratch).
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
 
 // Round-to-nearest-even f32 -> bf16 bits (finite inputs; NaN kept quiet).
 __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)

@@ -24,7 +24,11 @@ struct GemmArgs {
     double alpha, beta;
 };
 
+// variant < 0 selects the current default (set by the tuner) for the mode.
 int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
-                hipStream_t st);
+                int variant, hipStream_t st);
+int gemm_num_variants(int mode);
+int gemm_variant_params(int mode, int vid, int* params);   // BM, BN, BK, WGM, WGN
+void gemm_set_default_variant(int mode, int vid);
 
 }  // namespace chai_hip

@@ -131,7 +131,14 @@ typedef struct chai_gemm_desc {
     int same_a, same_b;             /* batch shares A / B (sameDataA/B)         */
     size_t stride_a, stride_b, stride_c; /* elements between instances          */
     int mode;                       /* CHAI_GEMM_*                              */
+    int variant;                    /* tile variant; -1 = tuned choice / default */
 } chai_gemm_desc;
+
+/* The tile-variant search space of a kernel family (autotune/KernelParams.hpp's
+ * endogenous parameters: work-group shape, inner blocking).  params receives
+ * {BM, BN, BK, waves_m, waves_n}. */
+int chai_hip_gemm_num_variants(int mode, int* count);
+int chai_hip_gemm_variant_params(int mode, int variant, int* params);
 
 /* Replaces Enqueue::visit(StmtMatmul/StmtMatmulAuto) and the Evergreen MM
  * kernels (enqueue/EnqueueMatmul.cpp:24-161, autotune/EvergreenMatmulMM.cpp:1077-1630). */
