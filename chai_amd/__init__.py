@@ -193,6 +193,26 @@ def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, 
     return C
 
 
+def gemv(A, x, y=None, trans_a=False, general=False, alpha=1.0, beta=0.0, prec_a=None, prec_x=None):
+    """y = op(A) x (or alpha op(A) x + beta y); A is M x N row-major (2-D tensor)."""
+    import torch
+    init()
+    M, N = A.shape
+    pa = prec_a if prec_a is not None else prec_of(A.dtype)
+    px = prec_x if prec_x is not None else prec_of(x.dtype)
+    py = F64 if F64 in (pa, px) else F32
+    if y is None:
+        y = torch.zeros(N if trans_a else M, dtype=dtype_of(py), device=A.device)
+    L = lib()
+    L.chai_hip_gemv.argtypes = [ctypes.c_int] * 4 + [ctypes.c_size_t] * 2 + [
+        ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+        ctypes.c_double, ctypes.c_double, ctypes.c_size_t, ctypes.c_int, ctypes.c_size_t,
+        ctypes.c_size_t, ctypes.c_size_t, ctypes.c_void_p]
+    check(L.chai_hip_gemv(pa, px, prec_of(y.dtype), int(trans_a), M, N, _ptr(A), A.stride(0), _ptr(x),
+                          _ptr(y), int(general), alpha, beta, 1, 0, 0, 0, 0, _stream()), "chai_hip_gemv")
+    return y
+
+
 def profile(on=True):
     """Enable per-kernel hipEvent timing inside chai_hip_gemm."""
     check(lib().chai_hip_profile_enable(int(on)), "profile_enable")

@@ -1,0 +1,1103 @@
+// trace.hip -- fused-trace compiler (host) and VM kernel (device).
+//
+// Replaces the reference's JIT middle and back end for elementwise /
+// reduction / matmul statements: MakeStmt splitting (jitstmt/MakeStmt.cpp:
+// 579-726), kernelization (jitkrnl/WorkTrace.cpp:62-370,
+// jitkrnl/GroupTogether.cpp:178-201), OpenCL text emission
+// (enqueue/EnqueueIdSpace.cpp:19-320, EnqueueReduce.cpp:21-222,
+// EnqueueAST.cpp:95-740) and the per-trace JustInTimeMemo.
+//
+// Grouping rules (kept from the reference): matmul is its own kernel;
+// elementwise statements over one index space fuse; a reduction is fused into
+// the kernel that computes its operand (the reference instead emits a separate
+// serial reduction kernel); a statement reading a reduction result of the
+// current group starts a new group.  Within a group, TEMP variables live in
+// VM registers; a variable crossing groups is stored.
+#include "common.hpp"
+#include "gemm_internal.hpp"
+#include "trace_vm.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+namespace chai_hip {
+
+// ===================================================================== device
+
+#define VM_SLOTS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11)
+constexpr int VM_NSLOTS = 12;
+#define VM_RD_CASE(n) case n: v = R##n; break;
+#define VM_WR_CASE(n) case n: R##n = v; break;
+#define VM_DECL(n) V4 R##n = {{0, 0, 0, 0}};
+
+__device__ __forceinline__ uint64_t vm_load_elem(const void* p, uint32_t prec, uint64_t idx)
+{
+    if (prec == CHAI_F64) return ((const uint64_t*)p)[idx];
+    return (uint64_t)((const uint32_t*)p)[idx];
+}
+
+// 32-bit stream element from the words of its vector (component comp)
+__device__ __forceinline__ uint64_t rng_elem32(const uint32_t w[4], int comp, int prec, int normal,
+                                               int has_lim, double lo, double hi)
+{
+    // select instead of indexing with a runtime component (keeps w in registers)
+    const uint32_t wc = comp == 0 ? w[0] : comp == 1 ? w[1] : comp == 2 ? w[2] : w[3];
+    if (prec == CHAI_U32 || prec == CHAI_I32) return (uint64_t)wc;
+    if (normal) {
+        const uint32_t wa = comp < 2 ? w[0] : w[2], wb = comp < 2 ? w[1] : w[3];
+        double n0, n1;
+        box_muller((double)unit_f32(wa), (double)unit_f32(wb), n0, n1);
+        return bf((float)((comp & 1) ? n1 : n0));
+    }
+    float u = unit_f32(wc);
+    if (has_lim) {
+        u = u * (float)(hi - lo);
+        if (lo != 0.0) u = u + (float)lo;
+    }
+    return bf(u);
+}
+
+__device__ __forceinline__ uint64_t rng_elem64(const uint64_t w[2], int comp, int normal, int has_lim,
+                                               double lo, double hi)
+{
+    const double u0 = unit_f64(w[0]), u1 = unit_f64(w[1]);
+    if (normal) {
+        double n0, n1;
+        box_muller(u0, u1, n0, n1);
+        return bd(comp ? n1 : n0);
+    }
+    double u = comp ? u1 : u0;
+    if (has_lim) {
+        u = u * (hi - lo);
+        if (lo != 0.0) u = u + lo;
+    }
+    return bd(u);
+}
+
+__device__ __forceinline__ void words32(int variant, int vl, uint32_t vidx, uint32_t seed, uint32_t w[4])
+{
+    if (variant == CHAI_RNG_THREEFRY) {
+        if (vl == 4) rng_words32<1231, 4>(vidx, seed, w);
+        else rng_words32<1231, 2>(vidx, seed, w);
+    } else {
+        if (vl == 4) rng_words32<1230, 4>(vidx, seed, w);
+        else rng_words32<1230, 2>(vidx, seed, w);
+    }
+}
+
+__device__ __forceinline__ void words64(int variant, uint64_t vidx, uint64_t seed, uint64_t w[2])
+{
+    if (variant == CHAI_RNG_THREEFRY) rng_words64<1231>(vidx, seed, w);
+    else rng_words64<1230>(vidx, seed, w);
+}
+
+__device__ V4 vm_rng(const VmIns& I, const VmRngDims& R, uint64_t slot, uint64_t y, uint64_t x0, int nval)
+{
+    V4 r = {{0, 0, 0, 0}};
+    const int variant = (int)I.a, vl = (int)I.b, normal = I.c & 1, lim = (I.c >> 1) & 1;
+    const int prec = I.p;
+    if (prec != CHAI_F64) {
+        if (R.same && vl == 4) {   // one stream vector covers the thread's 4 lanes
+            uint32_t w[4];
+            words32(variant, 4, (uint32_t)(slot * R.H * R.vpr + y * R.vpr + x0 / 4), (uint32_t)I.u, w);
+            if (normal && prec == CHAI_F32) {
+                double n0, n1, n2, n3;
+                box_muller((double)unit_f32(w[0]), (double)unit_f32(w[1]), n0, n1);
+                box_muller((double)unit_f32(w[2]), (double)unit_f32(w[3]), n2, n3);
+                r.l[0] = bf((float)n0); r.l[1] = bf((float)n1); r.l[2] = bf((float)n2); r.l[3] = bf((float)n3);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) r.l[i] = rng_elem32(w, i, prec, 0, lim, I.d0, I.d1);
+            }
+            return r;
+        }
+        for (int i = 0; i < 4; i++) if (i < nval) {
+            const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
+            uint32_t w[4];
+            words32(variant, vl, (uint32_t)(sr * R.H * R.vpr + yr * R.vpr + xr / vl), (uint32_t)I.u, w);
+            r.l[i] = rng_elem32(w, (int)(xr % vl), prec, normal, lim, I.d0, I.d1);
+        }
+        return r;
+    }
+    for (int i = 0; i < 4; i++) if (i < nval) {
+        const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
+        uint64_t w[2];
+        words64(variant, sr * R.H * R.vpr + yr * R.vpr + xr / vl, I.u, w);
+        r.l[i] = rng_elem64(w, (int)(xr % vl), normal, lim, I.d0, I.d1);
+    }
+    return r;
+}
+
+__device__ __forceinline__ V4 vm_load(const VmVar& V, uint64_t slot, uint64_t y, uint64_t x0, int nval,
+                                      uint64_t W, uint64_t H)
+{
+    V4 r = {{0, 0, 0, 0}};
+    if (V.same) {
+        const uint64_t base = (slot * H + y) * W + x0;
+        if (nval == 4 && V.prec != CHAI_F64 && ((W & 3) == 0) && (((uintptr_t)V.in & 15) == 0)) {
+            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+            const u4 q = *reinterpret_cast<const u4*>((const uint32_t*)V.in + base);
+            r.l[0] = q[0]; r.l[1] = q[1]; r.l[2] = q[2]; r.l[3] = q[3];
+            return r;
+        }
+        for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, base + i);
+        return r;
+    }
+    if (V.W == 1 && V.H == 1 && V.S == 1) {
+        const uint64_t e = vm_load_elem(V.in, V.prec, 0);
+        r.l[0] = r.l[1] = r.l[2] = r.l[3] = e;
+        return r;
+    }
+    const uint64_t rowoff = ((slot % V.S) * V.H + (y % V.H)) * V.W;
+    for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, rowoff + (x0 + i) % V.W);
+    return r;
+}
+
+__device__ __forceinline__ void vm_store(const VmVar& V, const V4& v, uint64_t slot, uint64_t y,
+                                         uint64_t x0, int nval, uint64_t W, uint64_t H)
+{
+    const uint64_t base = (slot * H + y) * W + x0;
+    if (V.prec == CHAI_F64) {
+        for (int i = 0; i < 4; i++) if (i < nval) ((uint64_t*)V.out)[base + i] = v.l[i];
+    } else if (nval == 4 && (W & 3) == 0 && (((uintptr_t)V.out & 15) == 0)) {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        u4 q = {(uint32_t)v.l[0], (uint32_t)v.l[1], (uint32_t)v.l[2], (uint32_t)v.l[3]};
+        *reinterpret_cast<u4*>((uint32_t*)V.out + base) = q;
+    } else {
+        for (int i = 0; i < 4; i++) if (i < nval) ((uint32_t*)V.out)[base + i] = (uint32_t)v.l[i];
+    }
+}
+
+__device__ __forceinline__ V4 vm_bin(uint32_t op, int prec, const V4& a, const V4& b)
+{
+    V4 r;
+    switch (prec) {
+        case CHAI_F32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bf(fp_bin<float>(op, f_(a.l[i]), f_(b.l[i])));
+            break;
+        case CHAI_F64:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bd(fp_bin<double>(op, d_(a.l[i]), d_(b.l[i])));
+            break;
+        case CHAI_I32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bi(i_bin(op, i_(a.l[i]), i_(b.l[i])));
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bu(u_bin(op, u_(a.l[i]), u_(b.l[i])));
+            break;
+    }
+    return r;
+}
+
+__device__ __forceinline__ V4 vm_un(uint32_t op, int prec, const V4& a)
+{
+    V4 r;
+    switch (prec) {
+        case CHAI_F32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bf(fp_un<float>(op, f_(a.l[i])));
+            break;
+        case CHAI_F64:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bd(fp_un<double>(op, d_(a.l[i])));
+            break;
+        case CHAI_I32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bi(i_un(op, i_(a.l[i])));
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bu(u_un(op, u_(a.l[i])));
+            break;
+    }
+    return r;
+}
+
+__device__ __forceinline__ V4 vm_ter(uint32_t op, int prec, const V4& a, const V4& b, const V4& c)
+{
+    V4 r;
+    switch (prec) {
+        case CHAI_F32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bf(ter<float>(op, f_(a.l[i]), f_(b.l[i]), f_(c.l[i])));
+            break;
+        case CHAI_F64:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bd(ter<double>(op, d_(a.l[i]), d_(b.l[i]), d_(c.l[i])));
+            break;
+        case CHAI_I32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bi(ter<int32_t>(op, i_(a.l[i]), i_(b.l[i]), i_(c.l[i])));
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bu(ter<uint32_t>(op, u_(a.l[i]), u_(b.l[i]), u_(c.l[i])));
+            break;
+    }
+    return r;
+}
+
+struct VmLaunchArgs {
+    const VmIns* prog;
+    uint32_t nins, nacc;
+    uint64_t W, H, S, vpr, vps;
+    double* partial;
+    VmVar vars[24];
+    VmRngDims rng[8];
+};
+
+__global__ __launch_bounds__(VM_THREADS) void trace_vm_kernel(VmLaunchArgs L)
+{
+    const uint64_t slot = blockIdx.y;
+    VM_SLOTS(VM_DECL)
+    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;
+    // table entries are picked with constant indices (a runtime index into the
+    // by-value argument block would make the compiler copy it to scratch)
+#define VM_VAR_CASE(n) case n: return L.vars[n];
+#define VM_RNG_CASE(n) case n: return L.rng[n];
+    auto var_at = [&](uint32_t k) -> VmVar {
+        switch (k) { VM_VAR_CASE(0) VM_VAR_CASE(1) VM_VAR_CASE(2) VM_VAR_CASE(3) VM_VAR_CASE(4)
+                     VM_VAR_CASE(5) VM_VAR_CASE(6) VM_VAR_CASE(7) VM_VAR_CASE(8) VM_VAR_CASE(9)
+                     VM_VAR_CASE(10) VM_VAR_CASE(11) VM_VAR_CASE(12) VM_VAR_CASE(13) VM_VAR_CASE(14)
+                     VM_VAR_CASE(15) VM_VAR_CASE(16) VM_VAR_CASE(17) VM_VAR_CASE(18) VM_VAR_CASE(19)
+                     VM_VAR_CASE(20) VM_VAR_CASE(21) VM_VAR_CASE(22)
+                     default: return L.vars[23]; }
+    };
+    auto rng_at = [&](uint32_t k) -> VmRngDims {
+        switch (k) { VM_RNG_CASE(0) VM_RNG_CASE(1) VM_RNG_CASE(2) VM_RNG_CASE(3) VM_RNG_CASE(4)
+                     VM_RNG_CASE(5) VM_RNG_CASE(6) default: return L.rng[7]; }
+    };
+    const uint64_t stride = (uint64_t)gridDim.x * VM_THREADS;
+    for (uint64_t v = (uint64_t)blockIdx.x * VM_THREADS + threadIdx.x; v < L.vps; v += stride) {
+        const uint64_t y = v / L.vpr;
+        const uint64_t x0 = (v - y * L.vpr) * 4;
+        const int nval = (L.W - x0) < 4 ? (int)(L.W - x0) : 4;
+        for (uint32_t pc = 0; pc < L.nins; pc++) {
+            const VmIns I = L.prog[pc];
+            V4 a, b, c, v;
+            bool wr = true;
+            switch (I.s1) { VM_SLOTS(VM_RD_CASE) }
+            a = v;
+            switch (I.op) {
+                case VM_LOAD: v = vm_load(var_at(I.a), slot, y, x0, nval, L.W, L.H); break;
+                case VM_IMM: v.l[0] = v.l[1] = v.l[2] = v.l[3] = I.u; break;
+                case VM_RNG: v = vm_rng(I, rng_at(I.e), slot, y, x0, nval); break;
+                case VM_INDEX:
+                    for (int i = 0; i < 4; i++)
+                        v.l[i] = from_double(I.p, (double)(I.a == 0 ? x0 + i : y));
+                    break;
+                case VM_CVT:
+                    for (int i = 0; i < 4; i++) v.l[i] = from_double(I.p, as_double(I.p2, a.l[i]));
+                    break;
+                case VM_UN: v = vm_un(I.a, I.p, a); break;
+                case VM_BIN:
+                    switch (I.s2) { VM_SLOTS(VM_RD_CASE) }
+                    v = vm_bin(I.a, I.p, a, v);
+                    break;
+                case VM_TER:
+                    switch (I.s2) { VM_SLOTS(VM_RD_CASE) }
+                    b = v;
+                    switch (I.s3) { VM_SLOTS(VM_RD_CASE) }
+                    c = v;
+                    v = vm_ter(I.a, I.p, a, b, c);
+                    break;
+                case VM_SEL:
+                    switch (I.s2) { VM_SLOTS(VM_RD_CASE) }
+                    b = v;
+                    switch (I.s3) { VM_SLOTS(VM_RD_CASE) }
+                    c = v;
+                    for (int i = 0; i < 4; i++) v.l[i] = as_double(I.p2, a.l[i]) != 0.0 ? b.l[i] : c.l[i];
+                    break;
+                case VM_STORE:
+                    vm_store(var_at(I.a), a, slot, y, x0, nval, L.W, L.H);
+                    wr = false;
+                    break;
+                case VM_ACC: {
+                    double s = 0.0;
+                    for (int i = 0; i < 4; i++) if (i < nval) s += as_double(I.p, a.l[i]);
+                    if (I.a == 0) acc0 += s;
+                    else if (I.a == 1) acc1 += s;
+                    else if (I.a == 2) acc2 += s;
+                    else acc3 += s;
+                    wr = false;
+                    break;
+                }
+                default: v = a; break;   // VM_MOV
+            }
+            if (wr) switch (I.dst) { VM_SLOTS(VM_WR_CASE) }
+        }
+    }
+    if (L.nacc) {
+        const uint64_t base = ((uint64_t)slot * gridDim.x + blockIdx.x) * L.nacc;
+        double r = block_sum_f64<VM_THREADS>(acc0);
+        if (threadIdx.x == 0) L.partial[base + 0] = r;
+        if (L.nacc > 1) { r = block_sum_f64<VM_THREADS>(acc1); if (threadIdx.x == 0) L.partial[base + 1] = r; }
+        if (L.nacc > 2) { r = block_sum_f64<VM_THREADS>(acc2); if (threadIdx.x == 0) L.partial[base + 2] = r; }
+        if (L.nacc > 3) { r = block_sum_f64<VM_THREADS>(acc3); if (threadIdx.x == 0) L.partial[base + 3] = r; }
+    }
+}
+
+struct RedOut {
+    void* out;
+    uint32_t prec;
+    double divisor;
+};
+
+// fixed-order combine of the per-block partials: one block per (slot, accumulator)
+__global__ __launch_bounds__(VM_THREADS) void trace_reduce_final(const double* __restrict__ partial,
+                                                                 uint32_t blocks, uint32_t nacc,
+                                                                 RedOut o0, RedOut o1, RedOut o2, RedOut o3)
+{
+    const uint32_t slot = blockIdx.x, k = blockIdx.y;
+    double s = 0.0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += VM_THREADS)
+        s += partial[((uint64_t)slot * blocks + b) * nacc + k];
+    s = block_sum_f64<VM_THREADS>(s);
+    if (threadIdx.x == 0) {
+        const RedOut o = k == 0 ? o0 : k == 1 ? o1 : k == 2 ? o2 : o3;
+        const double r = o.divisor != 1.0 ? s / o.divisor : s;
+        const uint64_t bits = from_double(o.prec, r);
+        if (o.prec == CHAI_F64) ((uint64_t*)o.out)[slot] = bits;
+        else ((uint32_t*)o.out)[slot] = (uint32_t)bits;
+    }
+}
+
+// ======================================================================= host
+
+namespace {
+
+bool is_fp(int p) { return p == CHAI_F32 || p == CHAI_F64; }
+
+// opcode classes (bytecode/ByteCodes.hpp)
+bool is_scalar(uint32_t c) { return c >= 5 && c <= 8; }
+bool is_convert(uint32_t c) { return c >= 1 && c <= 4; }
+bool is_rng(uint32_t c) { return c >= 110 && c <= 115; }
+bool is_index(uint32_t c) { return c >= 42 && c <= 49; }
+bool is_red(uint32_t c) { return c == 130 || c == 131 || c == 132; }
+bool is_mm(uint32_t c) { return c == 120 || c == 121; }
+
+int prec_of_suffix(int k) { return k == 0 ? CHAI_U32 : k == 1 ? CHAI_I32 : k == 2 ? CHAI_F32 : CHAI_F64; }
+
+bool is_unary(uint32_t c, bool& fp_only)
+{
+    static const uint32_t fp1[] = {223, 261, 256, 219, 220, 221, 222, 237, 238, 239, 240, 257, 214, 262,
+                                   203, 200, 206, 259, 215, 263, 212, 253, 255, 266, 211, 218, 217};
+    fp_only = false;
+    if (c == 10 || c == 15 || c == 16 || c == 400 || c == 404 || c == 225) return true;
+    for (uint32_t f : fp1)
+        if (c == f) { fp_only = true; return true; }
+    return false;
+}
+
+bool is_binary(uint32_t c, int& kind)   // kind 0: any, 1: fp only, 2: int only
+{
+    static const uint32_t any[] = {11, 12, 13, 14, 32, 33, 34, 35, 36, 37, 302, 303};
+    static const uint32_t fp[] = {227, 228, 248, 207, 229, 232, 213, 224};
+    static const uint32_t in[] = {17, 18, 19, 20, 21, 22, 30, 31, 401, 402, 405, 411, 412, 413, 414, 415};
+    for (uint32_t x : any) if (c == x) { kind = 0; return true; }
+    for (uint32_t x : fp) if (c == x) { kind = 1; return true; }
+    for (uint32_t x : in) if (c == x) { kind = 2; return true; }
+    return false;
+}
+
+bool is_ternary(uint32_t c) { return c == 23 || c == 300 || c == 226 || c == 242; }
+
+struct Dims {
+    uint64_t W = 1, H = 1, S = 1;
+    bool operator==(const Dims& o) const { return W == o.W && H == o.H && S == o.S; }
+    bool operator!=(const Dims& o) const { return !(*this == o); }
+};
+
+struct ENode {
+    uint32_t code;
+    const chai_trace_node* src;
+    std::vector<int> kids;
+    int prec = 0;
+    Dims d;
+};
+
+struct Stmt {
+    uint32_t lhs;
+    std::vector<ENode> nodes;   // nodes[root] last
+    int root;
+    int kind;                   // 0 elementwise, 1 reduction, 2 matmul
+    Dims space;                 // index space of the kernel computing it
+};
+
+struct Group {
+    int kind;                   // 0 VM, 2 matmul
+    Dims space;
+    std::vector<int> stmts;
+    // VM
+    std::vector<VmIns> prog;
+    std::vector<uint32_t> var_of_slot;   // VmVar table: trace variable per entry
+    std::vector<VmRngDims> rng;
+    std::vector<uint32_t> red_vars;      // output variable of each accumulator
+    std::vector<double> red_div;
+    VmIns* dprog = nullptr;
+};
+
+}  // namespace
+
+struct TraceProgram {
+    std::vector<chai_trace_var> vars;
+    std::vector<Stmt> stmts;
+    std::vector<Group> groups;
+    std::vector<bool> needs_mem;      // TEMP variables that must be stored
+    int ninstr = 0;
+    ~TraceProgram()
+    {
+        for (auto& g : groups)
+            if (g.dprog) (void)hipFree(g.dprog);
+    }
+};
+
+namespace {
+
+int fail(const char* fmt, ...)
+{
+    char buf[300];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    set_error("trace_compile: %s", buf);
+    return CHAI_ERR_ARG;
+}
+
+Dims dmax(const Dims& a, const Dims& b)
+{
+    Dims r;
+    r.W = a.W > b.W ? a.W : b.W;
+    r.H = a.H > b.H ? a.H : b.H;
+    r.S = a.S > b.S ? a.S : b.S;
+    return r;
+}
+
+int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
+{
+    s.lhs = cs.lhs;
+    if (cs.lhs >= t.num_vars) return fail("statement assigns variable %u of %u", cs.lhs, t.num_vars);
+    if ((uint64_t)cs.first + cs.count > t.num_nodes || cs.count == 0) return fail("bad node range");
+    std::vector<int> stack;
+    for (uint32_t i = 0; i < cs.count; i++) {
+        const chai_trace_node& n = t.nodes[cs.first + i];
+        ENode e;
+        e.code = n.code;
+        e.src = &n;
+        int arity = 0;
+        bool fp_only = false;
+        int bkind = 0;
+        if (n.code == CHAI_TRACE_VAR) {
+            if (n.arg >= t.num_vars) return fail("read of variable %u of %u", n.arg, t.num_vars);
+            const chai_trace_var& v = t.vars[n.arg];
+            e.prec = (int)v.prec;
+            e.d.W = v.W; e.d.H = v.H; e.d.S = v.slots;
+        } else if (is_scalar(n.code)) {
+            e.prec = prec_of_suffix((int)n.code - 5);
+        } else if (is_rng(n.code)) {
+            static const int rp[] = {CHAI_F32, CHAI_F64, CHAI_U32, CHAI_I32, CHAI_F32, CHAI_F64};
+            e.prec = rp[n.code - 110];
+            e.d.W = n.W; e.d.H = n.H ? n.H : 1; e.d.S = n.slots ? n.slots : 1;
+            if (n.arg != CHAI_RNG_PHILOX && n.arg != CHAI_RNG_THREEFRY) return fail("rng variant %u", n.arg);
+        } else if (is_index(n.code)) {
+            e.prec = prec_of_suffix((int)((n.code - 42) % 4));
+            e.d.W = n.W; e.d.H = n.H ? n.H : 1; e.d.S = n.slots ? n.slots : 1;
+        } else if (is_convert(n.code)) {
+            arity = 1;
+            e.prec = prec_of_suffix((int)n.code - 1);
+        } else if (is_unary(n.code, fp_only)) {
+            arity = 1;
+        } else if (is_binary(n.code, bkind)) {
+            arity = 2;
+        } else if (is_ternary(n.code)) {
+            arity = 3;
+        } else if (n.code == 131 || n.code == 132) {
+            arity = 1;
+        } else if (n.code == 130) {
+            arity = 2;
+        } else if (n.code == 120) {
+            arity = 2;
+        } else if (n.code == 121) {
+            arity = 3;
+        } else {
+            return fail("bytecode %u is not supported by the GPU back end", n.code);
+        }
+        if ((int)stack.size() < arity) return fail("stack underflow at bytecode %u", n.code);
+        e.kids.assign(stack.end() - arity, stack.end());
+        stack.resize(stack.size() - arity);
+        // typing
+        if (arity == 1 && !is_convert(n.code)) {
+            const ENode& k = s.nodes[e.kids[0]];
+            e.prec = k.prec;
+            e.d = k.d;
+            if (fp_only && !is_fp(k.prec)) return fail("bytecode %u needs a floating point operand", n.code);
+            if (n.code == 15 || n.code == 16 || n.code == 404) {
+                if (is_fp(k.prec)) return fail("bytecode %u needs an integer operand", n.code);
+            }
+            if (n.code == 131 || n.code == 132) { e.d.W = 1; e.d.H = 1; }
+        } else if (is_convert(n.code)) {
+            e.d = s.nodes[e.kids[0]].d;
+        } else if (arity == 2) {
+            const ENode& a = s.nodes[e.kids[0]];
+            const ENode& b = s.nodes[e.kids[1]];
+            e.prec = a.prec > b.prec ? a.prec : b.prec;
+            if (n.code == 120) {
+                e.d = Dims();   // fixed below
+            } else {
+                e.d = dmax(a.d, b.d);
+                if (bkind == 1 && !is_fp(e.prec)) return fail("bytecode %u needs floating point", n.code);
+                if (bkind == 2 && is_fp(e.prec)) return fail("bytecode %u needs integers", n.code);
+                if (n.code == 130) { e.d.W = 1; e.d.H = 1; }
+            }
+        } else if (arity == 3) {
+            const ENode& a = s.nodes[e.kids[0]];
+            const ENode& b = s.nodes[e.kids[1]];
+            const ENode& c = s.nodes[e.kids[2]];
+            if (n.code == 23) {        // cond: type and dims from the two branches (InterpCond.cpp)
+                e.prec = b.prec > c.prec ? b.prec : c.prec;
+                e.d = dmax(b.d, c.d);
+            } else if (n.code == 121) {
+                e.prec = c.prec;
+                e.d = c.d;
+            } else {
+                e.prec = a.prec > b.prec ? a.prec : b.prec;
+                e.prec = e.prec > c.prec ? e.prec : c.prec;
+                e.d = dmax(dmax(a.d, b.d), c.d);
+                if ((n.code == 226 || n.code == 242) && !is_fp(e.prec)) return fail("fma/mad need floating point");
+            }
+        }
+        s.nodes.push_back(e);
+        stack.push_back((int)s.nodes.size() - 1);
+    }
+    if (stack.size() != 1) return fail("statement leaves %zu values", stack.size());
+    s.root = stack[0];
+    const ENode& r = s.nodes[s.root];
+    s.kind = is_red(r.code) ? 1 : is_mm(r.code) ? 2 : 0;
+    // reductions / matmul only at the root
+    for (size_t i = 0; i + 1 < s.nodes.size(); i++)
+        if (is_red(s.nodes[i].code) || is_mm(s.nodes[i].code))
+            return fail("reduction or matmul below the root of a statement (split it first)");
+    const chai_trace_var& lv = t.vars[s.lhs];
+    if (s.kind == 0) {
+        s.space = r.d;
+        if (r.d.W != lv.W || r.d.H != lv.H || r.d.S != lv.slots)
+            return fail("statement shape %llux%llux%llu differs from its variable",
+                        (unsigned long long)r.d.W, (unsigned long long)r.d.H, (unsigned long long)r.d.S);
+    } else if (s.kind == 1) {
+        Dims in = s.nodes[r.kids[0]].d;
+        if (r.code == 130) in = dmax(in, s.nodes[r.kids[1]].d);
+        s.space = in;
+        if (lv.W != 1 || lv.H != 1 || lv.slots != in.S) return fail("reduction result must be 1x1xslots");
+    } else {
+        for (int k : r.kids)
+            if (s.nodes[k].code != CHAI_TRACE_VAR) return fail("matmul operands must be variables");
+        if (lv.slots != 1) return fail("matmul of slotted arrays is not supported");
+    }
+    return CHAI_OK;
+}
+
+struct Emitter {
+    const chai_trace& t;
+    TraceProgram& P;
+    Group& G;
+    std::map<uint32_t, int> reg_of_var;   // temp variable -> register slot
+    std::map<uint32_t, uint32_t> vtab;    // trace variable -> VmVar table index
+    bool temp_used[VM_NSLOTS] = {};      // register slots holding live temporaries
+    int max_stack = 0;
+
+    int min_temp() const
+    {
+        for (int s = 0; s < VM_NSLOTS; s++)
+            if (temp_used[s]) return s;
+        return VM_NSLOTS;
+    }
+    // highest free slot above `above`, or -1
+    int alloc_temp(int above)
+    {
+        for (int s = VM_NSLOTS - 1; s > above; s--)
+            if (!temp_used[s]) { temp_used[s] = true; return s; }
+        return -1;
+    }
+
+    uint32_t var_index(uint32_t v)
+    {
+        auto it = vtab.find(v);
+        if (it != vtab.end()) return it->second;
+        const uint32_t k = (uint32_t)G.var_of_slot.size();
+        G.var_of_slot.push_back(v);
+        vtab[v] = k;
+        return k;
+    }
+
+    VmIns ins(uint16_t op, int p, int dst)
+    {
+        VmIns I;
+        std::memset(&I, 0, sizeof I);
+        I.op = op;
+        I.p = (uint8_t)p;
+        I.dst = (uint8_t)dst;
+        return I;
+    }
+
+    // value of node n in slot >= tgt (or in a temp slot); returns the slot
+    int emit(const Stmt& s, int n, int tgt, int& err)
+    {
+        if (tgt > max_stack) max_stack = tgt;
+        if (tgt >= min_temp()) { err = fail("expression too deep for the VM registers"); return 0; }
+        const ENode& e = s.nodes[n];
+        const chai_trace_node& src = *e.src;
+        if (e.code == CHAI_TRACE_VAR) {
+            auto it = reg_of_var.find(src.arg);
+            if (it != reg_of_var.end()) return it->second;
+            VmIns I = ins(VM_LOAD, e.prec, tgt);
+            I.a = var_index(src.arg);
+            G.prog.push_back(I);
+            return tgt;
+        }
+        if (is_scalar(e.code)) {
+            VmIns I = ins(VM_IMM, e.prec, tgt);
+            switch (e.prec) {
+                case CHAI_U32: I.u = (uint32_t)(int64_t)src.s0; break;
+                case CHAI_I32: I.u = (uint32_t)(int32_t)(int64_t)src.s0; break;
+                case CHAI_F32: { float f = (float)src.s0; uint32_t b; std::memcpy(&b, &f, 4); I.u = b; break; }
+                default: std::memcpy(&I.u, &src.s0, 8); break;
+            }
+            G.prog.push_back(I);
+            return tgt;
+        }
+        if (is_rng(e.code)) {
+            VmIns I = ins(VM_RNG, e.prec, tgt);
+            I.a = src.arg;
+            const int svl = e.prec == CHAI_F64 ? 2 : 4;
+            const int vl = src.vec_len ? (int)src.vec_len : svl;
+            if (e.prec == CHAI_F64 ? (vl != 1 && vl != 2) : (vl != 1 && vl != 2 && vl != 4)) {
+                err = fail("rng vector length %d", vl);
+                return 0;
+            }
+            I.b = (uint32_t)vl;
+            const bool normal = e.code == 110 || e.code == 111;
+            const bool lim = !normal && (e.code == 114 || e.code == 115) && (src.s0 != 0.0 || src.s1 != 0.0);
+            I.c = (normal ? 1u : 0u) | (lim ? 2u : 0u);
+            I.u = src.seed;
+            I.d0 = src.s0;
+            I.d1 = src.s1;
+            VmRngDims R;
+            std::memset(&R, 0, sizeof R);
+            R.W = e.d.W; R.H = e.d.H; R.S = e.d.S;
+            R.vpr = pad_width(e.prec, e.d.W) / (uint64_t)vl;
+            R.same = (e.d == G.space) ? 1u : 0u;
+            if (G.rng.size() >= 8) { err = fail("more than 8 rng calls in one kernel"); return 0; }
+            I.e = (uint32_t)G.rng.size();
+            G.rng.push_back(R);
+            G.prog.push_back(I);
+            return tgt;
+        }
+        if (is_index(e.code)) {
+            VmIns I = ins(VM_INDEX, e.prec, tgt);
+            I.a = src.arg ? 1 : 0;
+            G.prog.push_back(I);
+            return tgt;
+        }
+        // operators: evaluate the operands into consecutive slots, converting to
+        // the operation's precision where it differs
+        int want[3] = {e.prec, e.prec, e.prec};
+        if (is_convert(e.code)) want[0] = s.nodes[e.kids[0]].prec;
+        if (e.code == 23) want[0] = s.nodes[e.kids[0]].prec;   // condition keeps its type
+        int slots[3] = {0, 0, 0};
+        int next = tgt;
+        for (size_t k = 0; k < e.kids.size(); k++) {
+            int sl = emit(s, e.kids[k], next, err);
+            if (err) return 0;
+            const int kp = s.nodes[e.kids[k]].prec;
+            if (kp != want[k]) {
+                VmIns C = ins(VM_CVT, want[k], next);
+                C.p2 = (uint8_t)kp;
+                C.s1 = (uint8_t)sl;
+                G.prog.push_back(C);
+                sl = next;
+            }
+            slots[k] = sl;
+            if (sl == next) next++;
+        }
+        VmIns I = ins(VM_MOV, e.prec, tgt);
+        I.s1 = (uint8_t)slots[0];
+        I.s2 = (uint8_t)slots[1];
+        I.s3 = (uint8_t)slots[2];
+        I.a = e.code;
+        if (is_convert(e.code)) {
+            I.op = VM_CVT;
+            I.p2 = (uint8_t)s.nodes[e.kids[0]].prec;
+        } else if (e.code == 23) {
+            I.op = VM_SEL;
+            I.p2 = (uint8_t)s.nodes[e.kids[0]].prec;
+        } else if (e.kids.size() == 1) {
+            I.op = VM_UN;
+        } else if (e.kids.size() == 2) {
+            I.op = VM_BIN;
+        } else {
+            I.op = VM_TER;
+        }
+        G.prog.push_back(I);
+        return tgt;
+    }
+};
+
+}  // namespace
+
+// Statement-to-kernel grouping and VM code generation.
+static int compile_trace(const chai_trace& t, TraceProgram& P)
+{
+    P.vars.assign(t.vars, t.vars + t.num_vars);
+    for (const auto& v : P.vars) {
+        if (!valid_prec((int)v.prec)) return fail("variable precision %u", v.prec);
+        if (v.W == 0 || v.H == 0 || v.slots == 0) return fail("empty variable");
+        if (v.slots > 65535) return fail("more than 65535 slots");
+    }
+    std::vector<int> writes(t.num_vars, 0);
+    P.stmts.resize(t.num_stmts);
+    for (uint32_t i = 0; i < t.num_stmts; i++) {
+        int rc = parse_stmt(t, t.stmts[i], P.stmts[i]);
+        if (rc) return rc;
+        if (P.vars[P.stmts[i].lhs].role == CHAI_VAR_INPUT) return fail("statement assigns an input");
+        if (writes[P.stmts[i].lhs]++) return fail("variable %u assigned twice", P.stmts[i].lhs);
+    }
+    // group statements
+    std::vector<int> group_of_stmt(t.num_stmts, -1);
+    std::vector<int> red_group_of_var(t.num_vars, -1);
+    for (uint32_t i = 0; i < t.num_stmts; i++) {
+        const Stmt& s = P.stmts[i];
+        bool reads_red = false;
+        for (const auto& e : s.nodes)
+            if (e.code == CHAI_TRACE_VAR && !P.groups.empty() &&
+                red_group_of_var[e.src->arg] == (int)P.groups.size() - 1)
+                reads_red = true;
+        const bool join = s.kind != 2 && !P.groups.empty() && P.groups.back().kind == 0 &&
+                          P.groups.back().space == s.space && !reads_red &&
+                          (s.kind == 0 || std::count_if(P.groups.back().stmts.begin(), P.groups.back().stmts.end(),
+                                                        [&](int q) { return P.stmts[q].kind == 1; }) < VM_MAX_ACC);
+        if (!join) {
+            Group g;
+            g.kind = s.kind == 2 ? 2 : 0;
+            g.space = s.space;
+            P.groups.push_back(g);
+        }
+        P.groups.back().stmts.push_back((int)i);
+        group_of_stmt[i] = (int)P.groups.size() - 1;
+        if (s.kind == 1) red_group_of_var[s.lhs] = (int)P.groups.size() - 1;
+    }
+    // which TEMP variables cross groups (or feed matmul) and need memory
+    P.needs_mem.assign(t.num_vars, false);
+    std::vector<int> def_group(t.num_vars, -1);
+    for (uint32_t i = 0; i < t.num_stmts; i++) def_group[P.stmts[i].lhs] = group_of_stmt[i];
+    for (uint32_t i = 0; i < t.num_stmts; i++) {
+        const Stmt& s = P.stmts[i];
+        for (const auto& e : s.nodes) {
+            if (e.code != CHAI_TRACE_VAR) continue;
+            const uint32_t v = e.src->arg;
+            if (P.vars[v].role != CHAI_VAR_TEMP) continue;
+            if (def_group[v] < 0) return fail("temp variable %u read before it is assigned", v);
+            if (def_group[v] != group_of_stmt[i] || s.kind == 2) P.needs_mem[v] = true;
+        }
+        if (s.kind == 1 && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
+        if (s.kind == 2 && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
+    }
+    // VM code per group
+    for (auto& G : P.groups) {
+        if (G.kind == 2) continue;
+        Emitter E{t, P, G};
+        // last statement of the group reading each variable: its register is
+        // released after that statement
+        std::map<uint32_t, int> last_use;
+        for (int si : G.stmts)
+            for (const auto& e : P.stmts[si].nodes)
+                if (e.code == CHAI_TRACE_VAR) last_use[e.src->arg] = si;
+        auto free_dead = [&](int si) {
+            for (auto it = E.reg_of_var.begin(); it != E.reg_of_var.end();) {
+                auto lu = last_use.find(it->first);
+                if (lu != last_use.end() && lu->second <= si) {
+                    E.temp_used[it->second] = false;
+                    it = E.reg_of_var.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+        };
+        for (int si : G.stmts) {
+            const Stmt& s = P.stmts[si];
+            int err = 0;
+            const ENode& r = s.nodes[s.root];
+            if (s.kind == 1) {
+                int sl;
+                int p = r.prec;
+                if (r.code == 130) {      // dot = sum of the promoted products
+                    ENode mul = r;
+                    (void)mul;
+                    int a = E.emit(s, r.kids[0], 0, err);
+                    if (err) return err;
+                    const int ap = s.nodes[r.kids[0]].prec;
+                    if (ap != p) {
+                        VmIns C = E.ins(VM_CVT, p, 0); C.p2 = (uint8_t)ap; C.s1 = (uint8_t)a;
+                        G.prog.push_back(C); a = 0;
+                    }
+                    const int nb = a == 0 ? 1 : 0;
+                    int b = E.emit(s, r.kids[1], nb, err);
+                    if (err) return err;
+                    const int bp = s.nodes[r.kids[1]].prec;
+                    if (bp != p) {
+                        VmIns C = E.ins(VM_CVT, p, nb); C.p2 = (uint8_t)bp; C.s1 = (uint8_t)b;
+                        G.prog.push_back(C); b = nb;
+                    }
+                    VmIns M = E.ins(VM_BIN, p, 0); M.a = 13; M.s1 = (uint8_t)a; M.s2 = (uint8_t)b;
+                    G.prog.push_back(M);
+                    sl = 0;
+                } else {
+                    sl = E.emit(s, r.kids[0], 0, err);
+                    if (err) return err;
+                }
+                VmIns A = E.ins(VM_ACC, p, 0);
+                A.a = (uint32_t)G.red_vars.size();
+                A.s1 = (uint8_t)sl;
+                G.prog.push_back(A);
+                G.red_vars.push_back(s.lhs);
+                G.red_div.push_back(r.code == 131 ? (double)(s.space.W * s.space.H) : 1.0);
+                free_dead(si);
+                continue;
+            }
+            int sl = E.emit(s, s.root, 0, err);
+            if (err) return err;
+            const chai_trace_var& lv = P.vars[s.lhs];
+            if ((int)lv.prec != r.prec) {
+                VmIns C = E.ins(VM_CVT, (int)lv.prec, 0);
+                C.p2 = (uint8_t)r.prec;
+                C.s1 = (uint8_t)sl;
+                G.prog.push_back(C);
+                sl = 0;
+            }
+            if (lv.role == CHAI_VAR_OUTPUT || P.needs_mem[s.lhs]) {
+                VmIns S = E.ins(VM_STORE, (int)lv.prec, 0);
+                S.a = E.var_index(s.lhs);
+                S.s1 = (uint8_t)sl;
+                G.prog.push_back(S);
+            }
+            // later statements of this group read it from a register
+            bool used_later = false;
+            for (int sj : G.stmts)
+                if (sj > si)
+                    for (const auto& e : P.stmts[sj].nodes)
+                        if (e.code == CHAI_TRACE_VAR && e.src->arg == s.lhs) used_later = true;
+            if (used_later) {
+                const int ts = E.alloc_temp(0);
+                if (ts < 0) return fail("too many live temporaries for the VM registers");
+                VmIns Mv = E.ins(VM_MOV, (int)lv.prec, ts);
+                Mv.s1 = (uint8_t)sl;
+                G.prog.push_back(Mv);
+                E.reg_of_var[s.lhs] = ts;
+            }
+            free_dead(si);
+        }
+        if (G.var_of_slot.size() > 24) return fail("more than 24 arrays in one fused kernel");
+        P.ninstr += (int)G.prog.size();
+        CHAI_HIP_TRY(hipMalloc((void**)&G.dprog, sizeof(VmIns) * G.prog.size()));
+        CHAI_HIP_TRY(hipMemcpy(G.dprog, G.prog.data(), sizeof(VmIns) * G.prog.size(), hipMemcpyHostToDevice));
+    }
+    return CHAI_OK;
+}
+
+static int run_matmul(const TraceProgram& P, const Stmt& s, void* const* bufs, hipStream_t st)
+{
+    const ENode& r = s.nodes[s.root];
+    const uint32_t va = s.nodes[r.kids[0]].src->arg, vb = s.nodes[r.kids[1]].src->arg;
+    const bool ta = s.nodes[r.kids[0]].src->flags & 1, tb = s.nodes[r.kids[1]].src->flags & 1;
+    const chai_trace_var &A = P.vars[va], &B = P.vars[vb], &Cv = P.vars[s.lhs];
+    const bool general = r.code == 121;
+    if (general && s.nodes[r.kids[2]].src->arg != s.lhs) {
+        // C operand is a different variable: copy it into the output first
+        const uint32_t vc = s.nodes[r.kids[2]].src->arg;
+        const size_t bytes = prec_size((int)Cv.prec) * Cv.W * Cv.H;
+        if (P.vars[vc].prec != Cv.prec) return fail("matmulG C operand precision differs from its result");
+        CHAI_HIP_TRY(hipMemcpyAsync(bufs[s.lhs], bufs[vc], bytes, hipMemcpyDeviceToDevice, st));
+    }
+    // row-major arrays: W = columns, H = rows.  A vector has H == 1.
+    const uint64_t am = ta ? A.W : A.H, ak = ta ? A.H : A.W;
+    const uint64_t bk = tb ? B.W : B.H, bn = tb ? B.H : B.W;
+    const int pc = (int)Cv.prec;
+    if (B.H == 1 && !tb && A.H > 1) {                 // matrix * vector
+        if (ak != B.W) return fail("matrix-vector inner dimension mismatch");
+        return chai_hip_gemv((int)A.prec, (int)B.prec, pc, ta, ta ? A.H : A.H, A.W, bufs[va], A.W,
+                             bufs[vb], bufs[s.lhs], general, r.src->s0, r.src->s1, 1, 0, 0, 0, 0, st);
+    }
+    if (A.H == 1 && !ta && B.H > 1) {                 // vector * matrix = B^T x
+        if (A.W != bk) return fail("vector-matrix inner dimension mismatch");
+        return chai_hip_gemv((int)B.prec, (int)A.prec, pc, !tb, B.H, B.W, bufs[vb], B.W, bufs[va],
+                             bufs[s.lhs], general, r.src->s0, r.src->s1, 1, 0, 0, 0, 0, st);
+    }
+    if (ak != bk) return fail("matmul inner dimension mismatch");
+    if (Cv.H != am || Cv.W != bn) return fail("matmul result has the wrong shape");
+    chai_gemm_desc d;
+    std::memset(&d, 0, sizeof d);
+    d.prec_a = (int)A.prec; d.prec_b = (int)B.prec; d.prec_c = pc;
+    d.trans_a = ta; d.trans_b = tb;
+    d.M = am; d.N = bn; d.K = ak;
+    d.lda = A.W; d.ldb = B.W; d.ldc = Cv.W;
+    d.general = general; d.alpha = r.src->s0; d.beta = r.src->s1;
+    d.batch = 1;
+    d.mode = CHAI_GEMM_AUTO;
+    d.variant = -1;
+    return chai_hip_gemm(&d, bufs[va], bufs[vb], bufs[s.lhs], st);
+}
+
+static int launch_program(const TraceProgram& P, void* const* user, hipStream_t st)
+{
+    const uint32_t nv = (uint32_t)P.vars.size();
+    std::vector<void*> bufs(nv, nullptr);
+    std::vector<void*> owned;
+    for (uint32_t v = 0; v < nv; v++) {
+        bufs[v] = user ? user[v] : nullptr;
+        if (P.vars[v].role != CHAI_VAR_TEMP && !bufs[v]) return fail("variable %u has no buffer", v);
+        if (P.vars[v].role == CHAI_VAR_TEMP && P.needs_mem[v] && !bufs[v]) {
+            const size_t bytes = prec_size((int)P.vars[v].prec) * P.vars[v].W * P.vars[v].H * P.vars[v].slots;
+            CHAI_HIP_TRY(hipMallocAsync(&bufs[v], bytes, st));
+            owned.push_back(bufs[v]);
+        }
+    }
+    int rc = CHAI_OK;
+    for (const auto& G : P.groups) {
+        if (G.kind == 2) {
+            rc = run_matmul(P, P.stmts[G.stmts[0]], bufs.data(), st);
+            if (rc) break;
+            continue;
+        }
+        VmLaunchArgs L;
+        std::memset(&L, 0, sizeof L);
+        L.prog = G.dprog;
+        L.nins = (uint32_t)G.prog.size();
+        L.nacc = (uint32_t)G.red_vars.size();
+        L.W = G.space.W; L.H = G.space.H; L.S = G.space.S;
+        L.vpr = (G.space.W + 3) / 4;
+        L.vps = L.vpr * G.space.H;
+        for (size_t k = 0; k < G.var_of_slot.size(); k++) {
+            const uint32_t v = G.var_of_slot[k];
+            VmVar& V = L.vars[k];
+            V.in = bufs[v];
+            V.out = bufs[v];
+            V.W = P.vars[v].W; V.H = P.vars[v].H; V.S = P.vars[v].slots;
+            V.prec = P.vars[v].prec;
+            V.same = (V.W == L.W && V.H == L.H && V.S == L.S) ? 1u : 0u;
+        }
+        for (size_t k = 0; k < G.rng.size(); k++) L.rng[k] = G.rng[k];
+        uint64_t bps = (L.vps + VM_THREADS - 1) / VM_THREADS;
+        const uint64_t cap = L.nacc ? 1024 : 65535;
+        if (bps > cap) bps = cap;
+        if (bps < 1) bps = 1;
+        double* partial = nullptr;
+        if (L.nacc) {
+            CHAI_HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * L.S * bps * L.nacc, st));
+            L.partial = partial;
+        }
+        trace_vm_kernel<<<dim3((unsigned)bps, (unsigned)L.S), VM_THREADS, 0, st>>>(L);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess && L.nacc) {
+            RedOut o[4];
+            std::memset(o, 0, sizeof o);
+            for (uint32_t k = 0; k < L.nacc; k++) {
+                o[k].out = bufs[G.red_vars[k]];
+                o[k].prec = P.vars[G.red_vars[k]].prec;
+                o[k].divisor = G.red_div[k];
+            }
+            trace_reduce_final<<<dim3((unsigned)L.S, L.nacc), VM_THREADS, 0, st>>>(partial, (uint32_t)bps, L.nacc,
+                                                                                   o[0], o[1], o[2], o[3]);
+            e = hipGetLastError();
+        }
+        if (partial) (void)hipFreeAsync(partial, st);
+        if (e != hipSuccess) { rc = hip_fail(e, "trace_vm launch"); break; }
+    }
+    for (void* p : owned) (void)hipFreeAsync(p, st);
+    return rc;
+}
+
+// ------------------------------------------------ compiled-trace cache (JIT memo)
+static uint64_t fnv_hash(const void* p, size_t n, uint64_t h)
+{
+    const unsigned char* b = (const unsigned char*)p;
+    for (size_t i = 0; i < n; i++) { h ^= b[i]; h *= 1099511628211ull; }
+    return h;
+}
+
+struct CacheEntry {
+    std::shared_ptr<TraceProgram> prog;
+    std::string key;
+};
+static std::mutex g_cache_mu;
+static std::map<uint64_t, std::vector<CacheEntry>> g_cache;
+
+struct Handle {
+    std::shared_ptr<TraceProgram> prog;
+};
+
+}  // namespace chai_hip
+
+using namespace chai_hip;
+
+extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
+{
+    if (!trace || !handle) { set_error("trace_compile: null argument"); return CHAI_ERR_ARG; }
+    *handle = nullptr;
+    if (!trace->num_stmts || !trace->vars || !trace->stmts || !trace->nodes) {
+        set_error("trace_compile: empty trace");
+        return CHAI_ERR_ARG;
+    }
+    std::string key;
+    key.append((const char*)trace->vars, sizeof(chai_trace_var) * trace->num_vars);
+    key.append((const char*)trace->stmts, sizeof(chai_trace_stmt) * trace->num_stmts);
+    key.append((const char*)trace->nodes, sizeof(chai_trace_node) * trace->num_nodes);
+    const uint64_t h = fnv_hash(key.data(), key.size(), 1469598103934665603ull);
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        auto it = g_cache.find(h);
+        if (it != g_cache.end())
+            for (auto& e : it->second)
+                if (e.key == key) {
+                    *handle = new Handle{e.prog};
+                    return CHAI_OK;
+                }
+    }
+    auto prog = std::make_shared<TraceProgram>();
+    const int rc = compile_trace(*trace, *prog);
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_cache_mu);
+        g_cache[h].push_back(CacheEntry{prog, key});
+    }
+    *handle = new Handle{prog};
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_trace_launch(void* handle, void* const* buffers, void* stream)
+{
+    if (!handle) { set_error("trace_launch: null handle"); return CHAI_ERR_STATE; }
+    return launch_program(*static_cast<Handle*>(handle)->prog, buffers, as_stream(stream));
+}
+
+extern "C" int chai_hip_trace_info(void* handle, int* num_kernels, int* num_instructions)
+{
+    if (!handle) { set_error("trace_info: null handle"); return CHAI_ERR_STATE; }
+    const TraceProgram& P = *static_cast<Handle*>(handle)->prog;
+    if (num_kernels) *num_kernels = (int)P.groups.size();
+    if (num_instructions) *num_instructions = P.ninstr;
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_trace_release(void* handle)
+{
+    delete static_cast<Handle*>(handle);
+    return CHAI_OK;
+}

@@ -1,0 +1,291 @@
+// trace_vm.hpp -- the fused-trace virtual machine (device side).
+//
+// One pre-compiled kernel executes any elementwise / reduction trace the host
+// compiler (trace.hip) lowers into 3-address VM instructions.  Each thread
+// owns a vector of 4 consecutive row elements (the reference JIT's float4
+// vectors, enqueue/EnqueueIdSpace.cpp:19-320); all lanes of a wavefront run the
+// same instruction, so dispatch is a uniform scalar branch and the instruction
+// stream is read through the scalar cache.  Values live in 16 register slots
+// R0..R15 (4 lanes x 64 bits each) addressed through uniform switches, so no
+// per-thread array is ever indexed dynamically (nothing goes to scratch).
+//
+// Scalar semantics follow Chai's interpreter (misc/SimpleFun{1,2,3}.cpp):
+// comparisons yield 0/1 in the operand type, integer shifts use the low five
+// bits of the count (x86 behaviour of the reference build), saturating
+// RHADD, non-rotating signed ROTATE, max/min as a<b?b:a.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "rng_device.hpp"
+#include "reduce_device.hpp"
+
+namespace chai_hip {
+
+enum VmOp : uint16_t {
+    VM_LOAD = 1,   // R[dst] <- var a (broadcast by modulo), element prec p
+    VM_IMM,        // R[dst] <- immediate bits u
+    VM_RNG,        // R[dst] <- rng stream: a = variant, b = vec_len, c = normal|limits<<1, rng dims index e
+    VM_INDEX,      // R[dst] <- index along dim a, as prec p
+    VM_CVT,        // R[dst] <- convert R[s1] from prec p2 to prec p (through double)
+    VM_UN,         // R[dst] <- op a (R[s1])
+    VM_BIN,        // R[dst] <- op a (R[s1], R[s2])
+    VM_TER,        // R[dst] <- op a (R[s1], R[s2], R[s3])
+    VM_SEL,        // R[dst] <- R[s1] != 0 ? R[s2] : R[s3]   (s1 in prec p2, values in p)
+    VM_STORE,      // var a <- R[s1]
+    VM_ACC,        // acc[a] += (double) R[s1]
+    VM_MOV,        // R[dst] <- R[s1]
+};
+
+struct VmIns {
+    uint16_t op;
+    uint8_t p, p2;
+    uint8_t dst, s1, s2, s3;
+    uint32_t a, b, c, e;
+    uint64_t u;
+    double d0, d1;
+};
+
+struct VmVar {
+    const void* in;
+    void* out;
+    uint64_t W, H, S;
+    uint32_t prec, same;       // same: dims equal the index space
+};
+
+struct VmRngDims {
+    uint64_t W, H, S, vpr;     // vpr = padded width / vec_len
+    uint32_t same, pad;
+};
+
+constexpr int VM_MAX_ACC = 4;
+constexpr int VM_THREADS = 256;
+
+struct VmLaunch {
+    const VmIns* prog;
+    uint32_t nins, nacc;
+    const VmVar* vars;
+    const VmRngDims* rng;
+    uint64_t W, H, S, vpr, vps;         // index space; vectors per row / per slot
+    double* partial;                    // [S][gridDim.x][nacc]
+};
+
+struct V4 {
+    uint64_t l[4];
+};
+
+// ------------------------------------------------------------ bit helpers
+__device__ __forceinline__ float f_(uint64_t b) { return __uint_as_float((uint32_t)b); }
+__device__ __forceinline__ double d_(uint64_t b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ int32_t i_(uint64_t b) { return (int32_t)(uint32_t)b; }
+__device__ __forceinline__ uint32_t u_(uint64_t b) { return (uint32_t)b; }
+__device__ __forceinline__ uint64_t bf(float v) { return (uint64_t)__float_as_uint(v); }
+__device__ __forceinline__ uint64_t bd(double v) { return (uint64_t)__double_as_longlong(v); }
+__device__ __forceinline__ uint64_t bi(int32_t v) { return (uint64_t)(uint32_t)v; }
+__device__ __forceinline__ uint64_t bu(uint32_t v) { return (uint64_t)v; }
+
+__device__ __forceinline__ double as_double(int prec, uint64_t b)
+{
+    switch (prec) {
+        case 0: return (double)u_(b);
+        case 1: return (double)i_(b);
+        case 4: return (double)f_(b);
+        default: return d_(b);
+    }
+}
+
+// double -> element bits, C conversion semantics of the interpreter
+// (InterpConvert.cpp: values pass through double)
+__device__ __forceinline__ uint64_t from_double(int prec, double v)
+{
+    switch (prec) {
+        case 0: return bu((uint32_t)(long long)v);
+        case 1: return bi((int32_t)(long long)v);
+        case 4: return bf((float)v);
+        default: return bd(v);
+    }
+}
+
+// ---------------------------------------------------------- scalar ops
+__device__ __forceinline__ uint32_t shl32(uint32_t a, uint32_t n) { return a << (n & 31u); }
+__device__ __forceinline__ uint32_t shr32u(uint32_t a, uint32_t n) { return a >> (n & 31u); }
+__device__ __forceinline__ int32_t shr32s(int32_t a, uint32_t n) { return a >> (n & 31u); }
+
+template <typename T>
+__device__ __forceinline__ T fp_bin(uint32_t op, T a, T b)
+{
+    switch (op) {
+        case 11: return a + b;
+        case 12: return a / b;
+        case 13: return a * b;
+        case 14: return a - b;
+        case 32: return (T)(a == b);
+        case 33: return (T)(a != b);
+        case 34: return (T)(a >= b);
+        case 35: return (T)(a > b);
+        case 36: return (T)(a <= b);
+        case 37: return (T)(a < b);
+        case 302: return a < b ? b : a;
+        case 303: return b < a ? b : a;
+        case 227: return sizeof(T) == 4 ? (T)fmaxf((float)a, (float)b) : (T)fmax((double)a, (double)b);
+        case 228: return sizeof(T) == 4 ? (T)fminf((float)a, (float)b) : (T)fmin((double)a, (double)b);
+        case 248: return sizeof(T) == 4 ? (T)powf((float)a, (float)b) : (T)pow((double)a, (double)b);
+        case 207: return sizeof(T) == 4 ? (T)atan2f((float)a, (float)b) : (T)atan2((double)a, (double)b);
+        case 229: return sizeof(T) == 4 ? (T)fmodf((float)a, (float)b) : (T)fmod((double)a, (double)b);
+        case 232: return sizeof(T) == 4 ? (T)hypotf((float)a, (float)b) : (T)hypot((double)a, (double)b);
+        case 213: return sizeof(T) == 4 ? (T)copysignf((float)a, (float)b) : (T)copysign((double)a, (double)b);
+        case 224: return sizeof(T) == 4 ? (T)fdimf((float)a, (float)b) : (T)fdim((double)a, (double)b);
+        default: return (T)0;
+    }
+}
+
+__device__ __forceinline__ uint32_t u_bin(uint32_t op, uint32_t a, uint32_t b)
+{
+    switch (op) {
+        case 11: return a + b;
+        case 12: return b ? a / b : 0u;
+        case 13: return a * b;
+        case 14: return a - b;
+        case 17: return b ? a % b : 0u;
+        case 18: return shl32(a, b);
+        case 19: return shr32u(a, b);
+        case 20: return a & b;
+        case 21: return a | b;
+        case 22: return a ^ b;
+        case 30: return (uint32_t)(a && b);
+        case 31: return (uint32_t)(a || b);
+        case 32: return (uint32_t)(a == b);
+        case 33: return (uint32_t)(a != b);
+        case 34: return (uint32_t)(a >= b);
+        case 35: return (uint32_t)(a > b);
+        case 36: return (uint32_t)(a <= b);
+        case 37: return (uint32_t)(a < b);
+        case 302: return a < b ? b : a;
+        case 303: return b < a ? b : a;
+        case 401: return a > b ? a - b : b - a;
+        case 402: { const uint64_t c = (uint64_t)a + b; return c > 0xffffffffull ? 0xffffffffu : (uint32_t)c; }
+        case 405: { const uint64_t c = (uint64_t)a + b; return (c <= 0xffffffffull ? (uint32_t)c : 0xffffffffu) >> 1; }
+        case 411: return (a & 0x00ffffffu) * (b & 0x00ffffffu);
+        case 412: return (uint32_t)(((uint64_t)a * b) >> 32);
+        case 413: { const uint64_t c = (uint64_t)a + b + 1; return (uint32_t)((c <= 0xffffffffull ? c : 0xffffffffull) >> 1); }
+        case 414: return shl32(a, b) | shr32u(a, 32u - b);
+        case 415: return a < b ? 0u : b - a;   // reference returns b - a (SimpleFun2.cpp:521-527)
+        default: return 0u;
+    }
+}
+
+__device__ __forceinline__ int32_t i_bin(uint32_t op, int32_t a, int32_t b)
+{
+    const uint32_t ua = (uint32_t)a, ub = (uint32_t)b;
+    switch (op) {
+        case 11: return (int32_t)(ua + ub);
+        case 12: return (b == 0 || (a == INT32_MIN && b == -1)) ? 0 : a / b;
+        case 13: return (int32_t)(ua * ub);
+        case 14: return (int32_t)(ua - ub);
+        case 17: return (b == 0 || (a == INT32_MIN && b == -1)) ? 0 : a % b;
+        case 18: return (int32_t)shl32(ua, ub);
+        case 19: return shr32s(a, ub);
+        case 20: return a & b;
+        case 21: return a | b;
+        case 22: return a ^ b;
+        case 30: return (int32_t)(a && b);
+        case 31: return (int32_t)(a || b);
+        case 32: return (int32_t)(a == b);
+        case 33: return (int32_t)(a != b);
+        case 34: return (int32_t)(a >= b);
+        case 35: return (int32_t)(a > b);
+        case 36: return (int32_t)(a <= b);
+        case 37: return (int32_t)(a < b);
+        case 302: return a < b ? b : a;
+        case 303: return b < a ? b : a;
+        case 401: return (int32_t)(a > b ? ua - ub : ub - ua);
+        // upper saturation only; below INT32_MIN the 32-bit value wraps
+        case 402: { const long long c = (long long)a + b; return c <= INT32_MAX ? (int32_t)(uint32_t)c : INT32_MAX; }
+        case 405: { const long long c = (long long)a + b; const int32_t d = c <= INT32_MAX ? (int32_t)(uint32_t)c : INT32_MAX; return d >> 1; }
+        case 411: return (int32_t)((uint32_t)(a & 0x00ffffff) * (uint32_t)(b & 0x00ffffff));
+        case 412: return (int32_t)(((long long)a * b) >> 32);
+        case 413: { const long long c = (long long)a + b + 1; const int32_t d = c <= INT32_MAX ? (int32_t)(uint32_t)c : INT32_MAX; return d >> 1; }
+        case 414: return (int32_t)shl32(ua, ub) | shr32s(a, 32u - ub);
+        case 415: { const long long c = (long long)a - b; return c > INT32_MAX ? INT32_MAX : c < INT32_MIN ? INT32_MIN : (int32_t)c; }
+        default: return 0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T fp_un(uint32_t op, T a)
+{
+    const bool f = sizeof(T) == 4;
+    switch (op) {
+        case 10: return -a;
+        case 223: return f ? (T)fabsf((float)a) : (T)fabs((double)a);
+        case 261: return f ? (T)sqrtf((float)a) : (T)sqrt((double)a);
+        case 256: return f ? (T)(1.0f / sqrtf((float)a)) : (T)(1.0 / sqrt((double)a));
+        case 219: return f ? (T)expf((float)a) : (T)exp((double)a);
+        case 220: return f ? (T)exp2f((float)a) : (T)exp2((double)a);
+        case 221: return f ? (T)exp10f((float)a) : (T)exp10((double)a);
+        case 222: return f ? (T)expm1f((float)a) : (T)expm1((double)a);
+        case 237: return f ? (T)logf((float)a) : (T)log((double)a);
+        case 238: return f ? (T)log2f((float)a) : (T)log2((double)a);
+        case 239: return f ? (T)log10f((float)a) : (T)log10((double)a);
+        case 240: return f ? (T)log1pf((float)a) : (T)log1p((double)a);
+        case 257: return f ? (T)sinf((float)a) : (T)sin((double)a);
+        case 214: return f ? (T)cosf((float)a) : (T)cos((double)a);
+        case 262: return f ? (T)tanf((float)a) : (T)tan((double)a);
+        case 203: return f ? (T)asinf((float)a) : (T)asin((double)a);
+        case 200: return f ? (T)acosf((float)a) : (T)acos((double)a);
+        case 206: return f ? (T)atanf((float)a) : (T)atan((double)a);
+        case 259: return f ? (T)sinhf((float)a) : (T)sinh((double)a);
+        case 215: return f ? (T)coshf((float)a) : (T)cosh((double)a);
+        case 263: return f ? (T)tanhf((float)a) : (T)tanh((double)a);
+        case 225: return f ? (T)floorf((float)a) : (T)floor((double)a);
+        case 212: return f ? (T)ceilf((float)a) : (T)ceil((double)a);
+        case 253: return f ? (T)rintf((float)a) : (T)rint((double)a);
+        case 255: return f ? (T)roundf((float)a) : (T)round((double)a);
+        case 266: return f ? (T)truncf((float)a) : (T)trunc((double)a);
+        case 211: return f ? (T)cbrtf((float)a) : (T)cbrt((double)a);
+        case 218: return f ? (T)erff((float)a) : (T)erf((double)a);
+        case 217: return f ? (T)erfcf((float)a) : (T)erfc((double)a);
+        default: return (T)0;
+    }
+}
+
+__device__ __forceinline__ uint32_t u_un(uint32_t op, uint32_t a)
+{
+    switch (op) {
+        case 15: return ~a;
+        case 16: return (uint32_t)(!a);
+        case 400: return a;
+        case 404: return a ? (uint32_t)__clz((int)a) : 32u;
+        case 225: return a;
+        default: return 0u;
+    }
+}
+
+__device__ __forceinline__ int32_t i_un(uint32_t op, int32_t a)
+{
+    switch (op) {
+        case 10: return (int32_t)(0u - (uint32_t)a);
+        case 15: return ~a;
+        case 16: return (int32_t)(!a);
+        case 400: return a < 0 ? (int32_t)(0u - (uint32_t)a) : a;
+        case 404: return a ? __clz(a) : 32;
+        case 225: return a;
+        default: return 0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T ter(uint32_t op, T a, T b, T c)
+{
+    switch (op) {
+        case 300: { const T m = a < b ? b : a; return m < c ? m : c; }
+        case 226:
+            if constexpr (sizeof(T) == 4 && (T)0.5 != 0) return (T)__builtin_fmaf((float)a, (float)b, (float)c);
+            else if constexpr ((T)0.5 != 0) return (T)__builtin_fma((double)a, (double)b, (double)c);
+            else return (T)0;
+        case 242: return a * b + c;
+        default: return (T)0;
+    }
+}
+
+}  // namespace chai_hip

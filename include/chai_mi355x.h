@@ -144,6 +144,75 @@ int chai_hip_gemm_variant_params(int mode, int variant, int* params);
  * kernels (enqueue/EnqueueMatmul.cpp:24-161, autotune/EvergreenMatmulMM.cpp:1077-1630). */
 int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C, void* stream);
 
+/* ----------------------------------------------------------------- traces */
+/* A trace is what one VectorTrace hands a device (common/VectorTrace.cpp):
+ * variables (arrays) and statements "var = expression", each expression a
+ * postfix (RPN) list of nodes whose codes are Chai's own bytecodes
+ * (bytecode/ByteCodes.hpp:21-338) plus CHAI_TRACE_VAR for a variable read.
+ * chai_hip_trace_compile groups statements into fused kernels the way the
+ * reference JIT kernelizes (jitkrnl/WorkTrace.cpp:62-370,
+ * jitkrnl/GroupTogether.cpp:178-201): elementwise statements over one index
+ * space fuse into one launch of the trace VM kernel; a sum/mean/dot fuses
+ * into the kernel producing its operand; matmul runs the GEMM/GEMV family;
+ * temporaries used inside one fused kernel never touch memory.  A trace the
+ * compiler cannot place on the GPU returns CHAI_ERR_ARG, so the caller falls
+ * back to the interpreter (runtime/Scheduler.cpp:221-254). */
+enum { CHAI_TRACE_VAR = 0x10000 };
+enum { CHAI_VAR_INPUT = 0, CHAI_VAR_OUTPUT = 1, CHAI_VAR_TEMP = 2 };
+
+typedef struct chai_trace_var {
+    uint32_t prec;                 /* CHAI_U32/I32/F32/F64                    */
+    uint32_t role;                 /* CHAI_VAR_INPUT / OUTPUT / TEMP          */
+    uint64_t W, H, slots;          /* dense [slots][H][W]                     */
+} chai_trace_var;
+
+typedef struct chai_trace_node {
+    uint32_t code;                 /* bytecode or CHAI_TRACE_VAR              */
+    uint32_t arg;                  /* VAR: variable index; rng: variant;
+                                      index1/2: dimension (0 = x, 1 = y)      */
+    uint32_t vec_len;              /* rng vector length (0 = default)        */
+    uint32_t flags;                /* transpose of a matmul operand (bit 0)   */
+    double s0, s1;                 /* scalar_* value; rng limits lo, hi;
+                                      matmulG alpha, beta                     */
+    uint64_t seed;                 /* rng seed                                */
+    uint64_t W, H, slots;          /* dims of data-creating leaves            */
+} chai_trace_node;
+
+typedef struct chai_trace_stmt {
+    uint32_t lhs;                  /* variable assigned                       */
+    uint32_t first, count;         /* node range in postfix order            */
+    uint32_t pad;
+} chai_trace_stmt;
+
+typedef struct chai_trace {
+    const chai_trace_var* vars;   uint32_t num_vars;
+    const chai_trace_stmt* stmts; uint32_t num_stmts;
+    const chai_trace_node* nodes; uint32_t num_nodes;
+} chai_trace;
+
+/* Compile (validate, type, group, emit VM programs).  Replaces MakeStmt /
+ * WorkTrace / Enqueue text emission and the per-trace JustInTimeMemo
+ * (enqueue/JustInTimeMemo.cpp:31-334): compiled handles are cached by the
+ * trace's TEA hash (misc/TEA.cpp:62-89) and shape. */
+int chai_hip_trace_compile(const chai_trace* trace, void** handle);
+/* buffers[v] = device pointer of variable v (INPUT and OUTPUT required;
+ * TEMP may be NULL, the library allocates one when it must be stored). */
+int chai_hip_trace_launch(void* handle, void* const* buffers, void* stream);
+/* number of kernels the trace compiled into, and of VM instructions */
+int chai_hip_trace_info(void* handle, int* num_kernels, int* num_instructions);
+int chai_hip_trace_release(void* handle);
+
+/* ------------------------------------------------------------------- GEMV */
+/* y = op(A) x or y = alpha op(A) x + beta y (Evergreen MV,
+ * autotune/EvergreenMatmulMV.cpp:884-1403): A is M x N row-major (op = A^T
+ * when trans_a, then y has N entries).  HBM-streaming kernel, one wavefront
+ * per row block with cross-lane reductions; accumulation in FP32 per lane and
+ * FP64 across lanes. */
+int chai_hip_gemv(int prec_a, int prec_x, int prec_y, int trans_a, size_t M, size_t N,
+                  const void* A, size_t lda, const void* x, void* y, int general,
+                  double alpha, double beta, size_t batch, int same_a,
+                  size_t stride_a, size_t stride_x, size_t stride_y, void* stream);
+
 /* Per-kernel timing (the replacement for AutoTuneBench's timer hooks,
  * autotune/AutoTuneBench.cpp:116-123): when enabled, chai_hip_gemm records
  * hipEvents on its launch stream around the operand pre-pass (split / pack)

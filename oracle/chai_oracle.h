@@ -80,6 +80,11 @@ int chai_or_dot(int prec_a, int prec_b, const void* a, const void* b, size_t n,
 /* ---- trace hashing (misc/TEA.cpp:13-89) */
 uint64_t chai_or_tea_hash(const uint64_t* data, size_t n);
 
+/* ---- trace evaluation with interpreter semantics (trace.c).  bufs[v] is a
+ *      host buffer for every variable (inputs filled, others written). */
+struct chai_trace;
+int chai_or_trace_eval(const struct chai_trace* trace, void* const* bufs);
+
 #ifdef __cplusplus
 }
 #endif

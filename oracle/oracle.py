@@ -45,8 +45,7 @@ def lib():
         L.chai_or_dot.argtypes = [i, i, vp, vp, sz, sz, vp]
         L.chai_or_tea_hash.restype = ctypes.c_uint64
         L.chai_or_tea_hash.argtypes = [vp, sz]
-        if hasattr(L, "chai_or_trace_eval"):
-            L.chai_or_trace_eval.argtypes = [vp, vp, vp]
+        L.chai_or_trace_eval.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -187,6 +186,26 @@ def reduce(op, a):
                               a2.shape[1], a2.shape[0], _p(out))
     assert rc == 0
     return out
+
+
+def trace_eval(trace, inputs):
+    """Evaluate a chai_amd.trace.Trace with interpreter semantics on the host.
+    inputs: {var index: numpy array}; returns {var index: numpy array} for all variables."""
+    c = trace.to_c()
+    bufs = []
+    arrs = {}
+    for i, (p, role, W, H, S) in enumerate(trace.vars):
+        if i in inputs:
+            a = np.ascontiguousarray(inputs[i], dtype=PREC_DTYPE[p]).reshape(S, H, W).copy()
+        else:
+            a = np.zeros((S, H, W), dtype=PREC_DTYPE[p])
+        arrs[i] = a
+        bufs.append(a.ctypes.data_as(ctypes.c_void_p))
+    arr = (ctypes.c_void_p * len(bufs))(*[b.value for b in bufs])
+    rc = lib().chai_or_trace_eval(ctypes.byref(c), arr)
+    if rc != 0:
+        raise ValueError("oracle trace evaluation failed")
+    return arrs
 
 
 def tea_hash(words):

@@ -1,0 +1,173 @@
+"""GPU parity of fused traces (chai_hip_trace_*) against the oracle's
+interpreter-semantics evaluation.
+
+Bar: integer traces bit-exact; floating point traces built from IEEE
+operations (+ - * / comparisons max min, conversions) bit-exact; reductions
+(FP64 accumulation in a different order) within n * 2^-52 * sum|x|;
+transcendentals (device math library vs glibc) within 4 ulp of the result
+per element (float: 4 * 2^-23 relative, plus 1e-30 absolute).
+"""
+import numpy as np
+import pytest
+import torch
+
+from chai_amd import F32, F64, I32, U32
+from chai_amd.trace import (Trace, exp, f32, log, maxe, matmul, mean, sin, sqrt, sum_, tanh,
+                            u32, cos, Expr)
+from tests.trace_cases import (MSGS, md5_expected, md5_hex, md5_inputs, md5_trace, mc_trace,
+                               random_trace)
+
+pytestmark = pytest.mark.gpu
+
+TDT = {U32: torch.int32, I32: torch.int32, F32: torch.float32, F64: torch.float64}
+NDT = {U32: np.uint32, I32: np.int32, F32: np.float32, F64: np.float64}
+
+
+def run_gpu(t, inputs):
+    """inputs {index: numpy}; returns {index: numpy} of every INPUT/OUTPUT var."""
+    prog = t.compile()
+    bufs = {}
+    for i, (p, role, W, H, S) in enumerate(t.vars):
+        if role == 0:
+            a = np.ascontiguousarray(inputs[i], dtype=NDT[p]).reshape(-1)
+            bufs[i] = torch.from_numpy(a.view(np.int32) if p == U32 else a).cuda()
+        elif role == 1:
+            bufs[i] = torch.zeros(S * H * W, dtype=TDT[p], device="cuda")
+    prog.run(bufs)
+    torch.cuda.synchronize()
+    out = {}
+    for i, b in bufs.items():
+        p, _, W, H, S = t.vars[i]
+        a = b.cpu().numpy()
+        out[i] = (a.view(np.uint32) if p == U32 else a).reshape(S, H, W)
+    return out, prog
+
+
+def test_md5_known_messages(chai, oracle):
+    t, M, outs = md5_trace(len(MSGS))
+    ins = md5_inputs(MSGS)
+    r, prog = run_gpu(t, {m.index: ins[k] for k, m in enumerate(M)})
+    assert md5_hex([r[o.index].ravel() for o in outs]) == md5_expected(MSGS)
+    nk, ni = prog.info()
+    assert nk == 1, "the whole MD5 trace fuses into one kernel"
+    print("md5 trace: kernels", nk, "VM instructions", ni)
+
+
+def test_md5_many_messages(chai, oracle):
+    rng = np.random.default_rng(0)
+    msgs = [bytes(rng.integers(0, 256, size=int(rng.integers(0, 56)), dtype=np.uint8)) for _ in range(3001)]
+    t, M, outs = md5_trace(len(msgs))
+    ins = md5_inputs(msgs)
+    r, _ = run_gpu(t, {m.index: ins[k] for k, m in enumerate(M)})
+    assert md5_hex([r[o.index].ravel() for o in outs]) == md5_expected(msgs)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_random_int_traces_bit_exact(chai, oracle, seed):
+    t, data, outs, red = random_trace(seed, "int")
+    print("seed", seed, flush=True)
+    o = oracle.trace_eval(t, data)
+    g, _ = run_gpu(t, data)
+    for v in outs + [red]:
+        assert np.array_equal(g[v.index], o[v.index]), v.index
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_fp_traces(chai, oracle, seed):
+    t, data, outs, red = random_trace(seed, "fp")
+    o = oracle.trace_eval(t, data)
+    g, _ = run_gpu(t, data)
+    for v in outs:
+        assert np.array_equal(g[v.index], o[v.index]), v.index
+    x = o[outs[-1].index].reshape(2, -1)
+    tol = x.shape[1] * 2.0 ** -52 * np.abs(x).sum(1) + 1e-300
+    assert np.all(np.abs(g[red.index].ravel() - o[red.index].ravel()) <= tol)
+
+
+def test_transcendentals_within_ulp(chai, oracle):
+    t = Trace()
+    x = t.input(F32, 4096)
+    outs = [t.assign(t.output(F32, 4096), f(x)) for f in (exp, log, sin, cos, sqrt, tanh)]
+    X = (np.random.default_rng(1).random(4096) * 8 + 1e-3).astype(np.float32)
+    o = oracle.trace_eval(t, {x.index: X})
+    g, _ = run_gpu(t, {x.index: X})
+    for v in outs:
+        a, b = g[v.index].astype(np.float64), o[v.index].astype(np.float64)
+        assert np.all(np.abs(a - b) <= 4 * 2.0 ** -23 * np.abs(b) + 1e-30), v.index
+
+
+def test_mc_trace_full_size(chai, oracle):
+    """2^24 Philox4x32-10 normals -> S = 100 exp(0.03 + 0.2 z) -> max(S-105,0) -> mean, one kernel."""
+    n = 1 << 24
+    t, m = mc_trace(n)
+    g, prog = run_gpu(t, {})
+    assert prog.info()[0] == 1
+    o = oracle.trace_eval(t, {})
+    gm, om = float(g[m.index].ravel()[0]), float(o[m.index].ravel()[0])
+    # the FP64 sums differ only in order and expf by <= 2 ulp per element: relative 1e-6
+    assert abs(gm - om) <= 1e-6 * abs(om), (gm, om)
+    # closed form: E[max(S-K,0)] for lognormal S (Black-Scholes, undiscounted)
+    from math import erf, exp as mexp, log as mlog, sqrt as msqrt
+    N = lambda v: 0.5 * (1 + erf(v / msqrt(2)))
+    mu, sig = mlog(100.0) + 0.03, 0.2
+    d2 = (mu - mlog(105.0)) / sig
+    bs = mexp(mu + sig * sig / 2) * N(d2 + sig) - 105.0 * N(d2)
+    assert abs(gm - bs) < 0.02, (gm, bs)
+    print("mc mean gpu %.7f oracle %.7f closed form %.7f" % (gm, om, bs))
+
+
+def test_matmul_with_fused_epilogue(chai, oracle):
+    """C = matmul(A, B); y = max(C * 0.5f + 1, 0): GEMM kernel + one VM kernel."""
+    n = 96
+    t = Trace()
+    A = t.input(F32, n, n); B = t.input(F32, n, n)
+    C = t.assign(t.temp(F32, n, n), matmul(A, B))
+    y = t.assign(t.output(F32, n, n), maxe(C * f32(0.5) + f32(1.0), f32(0.0)))
+    rng = np.random.default_rng(2)
+    a = (rng.random((n, n)) * 2 - 1).astype(np.float32); b = (rng.random((n, n)) * 2 - 1).astype(np.float32)
+    g, prog = run_gpu(t, {A.index: a, B.index: b})
+    assert prog.info()[0] == 2
+    ref = np.maximum((a.astype(np.float64) @ b.astype(np.float64)) * 0.5 + 1, 0)
+    assert np.abs(g[y.index].reshape(n, n) - ref).max() < 1e-4
+
+
+def test_gemv_in_trace(chai, oracle):
+    t = Trace()
+    A = t.input(F32, 500, 300); x = t.input(F32, 500)
+    y = t.assign(t.output(F32, 300), matmul(A, x))
+    rng = np.random.default_rng(3)
+    a = (rng.random((300, 500)) * 2 - 1).astype(np.float32); xv = (rng.random(500) * 2 - 1).astype(np.float32)
+    g, _ = run_gpu(t, {A.index: a, x.index: xv})
+    o = oracle.trace_eval(t, {A.index: a, x.index: xv})
+    tol = 500 * 2.0 ** -24 * (np.abs(a) @ np.abs(xv)) * 2
+    assert np.all(np.abs(g[y.index].ravel() - o[y.index].ravel()) <= tol)
+
+
+def test_slots_and_broadcast(chai, oracle):
+    t = Trace()
+    a = t.input(F32, 33, 7, 3)
+    row = t.input(F32, 33, 1, 1)
+    s = t.input(I32, 1, 1, 3)
+    o = t.assign(t.output(F64, 33, 7, 3), a * row + s + 0.25)
+    m = t.assign(t.output(F64, 1, 1, 3), mean(o))
+    rng = np.random.default_rng(4)
+    d = {a.index: rng.random((3, 7, 33), dtype=np.float32), row.index: rng.random((1, 1, 33), dtype=np.float32),
+         s.index: rng.integers(-9, 9, size=(3, 1, 1)).astype(np.int32)}
+    g, _ = run_gpu(t, d)
+    r = oracle.trace_eval(t, d)
+    assert np.array_equal(g[o.index], r[o.index])
+    assert np.allclose(g[m.index], r[m.index], rtol=1e-14)
+
+
+def test_unsupported_trace_is_rejected(chai):
+    t = Trace()
+    x = t.input(F32, 16)
+    t.assign(t.output(F32, 16), x - mean(x))      # reduction below the root: caller must split
+    with pytest.raises(chai.ChaiError):
+        t.compile()
+
+
+def test_compile_cache_returns_same_program(chai):
+    t, m = mc_trace(1 << 10)
+    p1 = t.compile(); p2 = t.compile()
+    assert p1.info() == p2.info()

@@ -1,0 +1,137 @@
+"""Pins the oracle's trace evaluator (oracle/trace.c).
+
+- MD5 of known messages as a pure uint32 trace equals hashlib's digests
+  (the reference's sample/md5.cpp workload): pins +, &, |, ^, ~, rotate.
+- Integer scalar ops against the reference's own misc/SimpleFun{1,2}.cpp
+  compiled into oracle/_ref.
+- Broadcast, promotion and reductions against numpy closed forms.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from chai_amd import F32, F64, I32, U32
+from chai_amd.trace import Trace, f32, i32, u32, mean, sum_, cond, maxe, exp, dot
+from tests.trace_cases import (MSGS, md5_expected, md5_hex, md5_inputs, md5_trace, mc_trace,
+                               random_trace)
+
+
+def test_md5_trace_known_answers(oracle):
+    t, M, outs = md5_trace(len(MSGS))
+    ins = md5_inputs(MSGS)
+    r = oracle.trace_eval(t, {m.index: ins[k] for k, m in enumerate(M)})
+    got = md5_hex([r[o.index].ravel() for o in outs])
+    assert got == md5_expected(MSGS)
+
+
+INT_CODES = [11, 12, 13, 14, 17, 18, 19, 20, 21, 22, 30, 31, 32, 33, 34, 35, 36, 37, 302, 303,
+             401, 402, 405, 411, 412, 413, 414, 415]
+
+
+@pytest.mark.parametrize("prec", [U32, I32])
+def test_int_binary_ops_match_reference_simplefun(oracle, prec):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built (no reference tree here)")
+    rng = np.random.default_rng(1)
+    n = 512
+    a = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    b = rng.integers(0, 2 ** 32, size=n, dtype=np.uint64).astype(np.uint32)
+    b[:64] = rng.integers(1, 32, size=64)            # small shift / rotate counts
+    a[64:80] = 0x80000000; b[64:80] = 0xffffffff     # INT_MIN / -1 edge
+    for code in INT_CODES:
+        bb = b.copy()
+        if code in (12, 17):
+            bb[bb == 0] = 1                          # the reference traps on x / 0
+            if prec == I32:
+                bb[(a == 0x80000000) & (bb == 0xffffffff)] = 1
+        if code in (18, 19, 414):
+            bb = bb % 32 if code != 414 else bb % 31 + 1   # defined shift counts only
+        expect = np.empty(n, np.uint32)
+        rc = ref.chai_ref_fun2_int(code, prec, a.ctypes.data_as(ctypes.c_void_p),
+                                   bb.ctypes.data_as(ctypes.c_void_p),
+                                   expect.ctypes.data_as(ctypes.c_void_p), n)
+        assert rc == 0, code
+        t = Trace()
+        va = t.input(prec, n); vb = t.input(prec, n)
+        from chai_amd.trace import Expr
+        o = t.assign(t.output(prec, n), Expr(code, [va, vb]))
+        dt = np.uint32 if prec == U32 else np.int32
+        r = oracle.trace_eval(t, {va.index: a.view(dt), vb.index: bb.view(dt)})
+        assert np.array_equal(r[o.index].ravel().view(np.uint32), expect), code
+
+
+def test_int_unary_ops_match_reference_simplefun(oracle):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("oracle/_ref not built")
+    from chai_amd.trace import Expr
+    rng = np.random.default_rng(2)
+    a = rng.integers(0, 2 ** 32, size=256, dtype=np.uint64).astype(np.uint32)
+    a[0] = 0
+    for prec in (U32, I32):
+        for code in (15, 16, 400, 10):
+            if code == 10 and prec == U32:
+                continue
+            expect = np.empty(256, np.uint32)
+            assert ref.chai_ref_fun1_int(code, prec, a.ctypes.data_as(ctypes.c_void_p),
+                                         expect.ctypes.data_as(ctypes.c_void_p), 256) == 0
+            t = Trace()
+            va = t.input(prec, 256)
+            o = t.assign(t.output(prec, 256), Expr(code, [va]))
+            dt = np.uint32 if prec == U32 else np.int32
+            r = oracle.trace_eval(t, {va.index: a.view(dt)})
+            assert np.array_equal(r[o.index].ravel().view(np.uint32), expect), (prec, code)
+
+
+def test_broadcast_promotion_reduction(oracle):
+    """(x % W, y % H, s % slots) broadcast; f32 + f64 literal -> f64; mean / W*H."""
+    t = Trace()
+    a = t.input(F32, 6, 4, 2)
+    row = t.input(F32, 6, 1, 1)           # broadcast down the height and the slots
+    col = t.input(I32, 1, 4, 2)           # broadcast across the width
+    o = t.assign(t.output(F64, 6, 4, 2), a * row + col + 0.5)      # 0.5 is an f64 literal
+    m = t.assign(t.output(F64, 1, 1, 2), mean(o))
+    s = t.assign(t.output(F32, 1, 1, 2), sum_(a))
+    rng = np.random.default_rng(0)
+    A = rng.random((2, 4, 6), dtype=np.float32)
+    R = rng.random((1, 1, 6), dtype=np.float32)
+    Cl = rng.integers(-5, 5, size=(2, 4, 1)).astype(np.int32)
+    r = oracle.trace_eval(t, {a.index: A, row.index: R, col.index: Cl})
+    # f32 * f32 -> f32; f32 + i32 -> f32 (int converted to float); + f64 literal -> f64
+    expect = ((A * R).astype(np.float32) + Cl.astype(np.float32)).astype(np.float32).astype(np.float64) + 0.5
+    assert np.array_equal(r[o.index], expect)
+    assert np.allclose(r[m.index].ravel(), expect.reshape(2, -1).mean(1), rtol=1e-15)
+    assert np.array_equal(r[s.index].ravel(), A.astype(np.float64).reshape(2, -1).sum(1).astype(np.float32))
+
+
+def test_cond_and_dot(oracle):
+    t = Trace()
+    x = t.input(F32, 50)
+    y = t.input(I32, 50)
+    c = t.assign(t.output(F32, 50), cond(x > f32(0.5), x, y))
+    d = t.assign(t.output(F32, 1), dot(x, y))
+    rng = np.random.default_rng(3)
+    X = rng.random(50, dtype=np.float32)
+    Y = rng.integers(-9, 9, size=50).astype(np.int32)
+    r = oracle.trace_eval(t, {x.index: X, y.index: Y})
+    assert np.array_equal(r[c.index].ravel(), np.where(X > 0.5, X, Y.astype(np.float32)))
+    assert r[d.index].ravel()[0] == np.float32((X * Y.astype(np.float32)).astype(np.float64).sum())
+
+
+def test_mc_trace_small(oracle):
+    t, m = mc_trace(1 << 14)
+    r = oracle.trace_eval(t, {})
+    z = oracle.rng_fill(1230, 4, 4, 7, 1 << 14, normal=True).ravel()
+    S = np.float32(100) * np.exp(np.float32(0.03) + np.float32(0.2) * z)
+    # numpy's f32 exp and glibc's expf may differ in the last bit; the mean agrees closely
+    assert abs(r[m.index].ravel()[0] - np.maximum(S - np.float32(105), 0).astype(np.float64).mean()) < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_traces_run_on_oracle(oracle, seed):
+    for kind in ("int", "fp"):
+        t, data, outs, red = random_trace(seed, kind)
+        r = oracle.trace_eval(t, data)
+        assert r[red.index].size == 2

@@ -1,0 +1,139 @@
+"""Trace workloads shared by the CPU oracle tests and the GPU parity tests."""
+import hashlib
+import math
+import struct
+
+import numpy as np
+
+from chai_amd import F32, F64, I32, U32
+from chai_amd.trace import (Trace, cond, exp, f32, fabs, fma, hadd, log, lor, maxe, mean, mine,
+                            mul_hi, rotate, sqrt, sum_, u32, i32, add_sat, sub_sat, rhadd,
+                            abs_diff, mul24, clamp, dot)
+
+# ---------------------------------------------------------------- MD5 (RFC 1321)
+MD5_S = [7, 12, 17, 22] * 4 + [5, 9, 14, 20] * 4 + [4, 11, 16, 23] * 4 + [6, 10, 15, 21] * 4
+MD5_K = [int(abs(math.sin(i + 1)) * 2 ** 32) & 0xffffffff for i in range(64)]
+
+
+def md5_words(msg):
+    """one padded 64-byte block as 16 little-endian u32 words (len(msg) <= 55)"""
+    assert len(msg) <= 55
+    b = msg + b"\x80" + b"\x00" * (55 - len(msg)) + struct.pack("<Q", 8 * len(msg))
+    return list(struct.unpack("<16I", b))
+
+
+def md5_trace(n):
+    """MD5 of n single-block messages as a pure uint32 trace (the reference's
+    sample/md5.cpp workload): 16 input word arrays, 4 digest outputs."""
+    t = Trace()
+    M = [t.input(U32, n) for _ in range(16)]
+    A, B, C, D = u32(0x67452301), u32(0xefcdab89), u32(0x98badcfe), u32(0x10325476)
+    a0, b0, c0, d0 = A, B, C, D
+    for i in range(64):
+        if i < 16:
+            F, g = (B & C) | (~B & D), i
+        elif i < 32:
+            F, g = (D & B) | (~D & C), (5 * i + 1) % 16
+        elif i < 48:
+            F, g = B ^ C ^ D, (3 * i + 5) % 16
+        else:
+            F, g = C ^ (B | ~D), (7 * i) % 16
+        nb = t.assign(t.temp(U32, n), B + rotate(F + A + u32(MD5_K[i]) + M[g], u32(MD5_S[i])))
+        A, D, C, B = D, C, B, nb
+    outs = [t.assign(t.output(U32, n), x + y) for x, y in ((a0, A), (b0, B), (c0, C), (d0, D))]
+    return t, M, outs
+
+
+def md5_inputs(msgs):
+    words = np.array([md5_words(m) for m in msgs], dtype=np.uint32)   # [n][16]
+    return [np.ascontiguousarray(words[:, k]) for k in range(16)]
+
+
+def md5_expected(msgs):
+    return [hashlib.md5(m).hexdigest() for m in msgs]
+
+
+def md5_hex(outs):
+    n = len(outs[0])
+    return [b"".join(struct.pack("<I", int(o[i])) for o in outs).hex() for i in range(n)]
+
+
+MSGS = [("chai md5 message %d %s" % (i, "x" * (i % 23))).encode() for i in range(28)]
+
+# ------------------------------------------------------------ Monte Carlo trace
+def mc_trace(W, H=1, seed=7):
+    """S = 100*exp((0.05-0.02)*1 + 0.2*z), payoff = max(S-105, 0), mean(payoff);
+    z = 2^24 Philox4x32-10 normals (Box-Muller).  Float literals keep it f32."""
+    t = Trace()
+    z = t.rng_normal(F32, W, H, seed=seed)
+    drift = float(np.float32(np.float32(0.05) - np.float32(0.02)) * np.float32(1))
+    S = f32(100.0) * exp(f32(drift) + f32(0.2) * z)
+    m = t.assign(t.output(F32, 1), mean(maxe(S - f32(105.0), f32(0.0))))
+    return t, m
+
+
+# ------------------------------------------------------------- random traces
+INT_BIN = ["add", "sub", "mul", "and", "or", "xor", "shl", "shr", "max", "min", "lt", "eq",
+           "rotate", "add_sat", "sub_sat", "hadd", "rhadd", "abs_diff", "mul24", "mul_hi", "div", "mod"]
+FP_BIN = ["add", "sub", "mul", "div", "max", "min", "lt", "ge", "eq"]
+
+
+def _bin(name, a, b):
+    import operator as op
+    table = {"add": op.add, "sub": op.sub, "mul": op.mul, "div": op.truediv, "mod": op.mod,
+             "and": op.and_, "or": op.or_, "xor": op.xor, "shl": op.lshift, "shr": op.rshift,
+             "max": maxe, "min": mine, "lt": op.lt, "ge": op.ge, "eq": op.eq, "rotate": rotate,
+             "add_sat": add_sat, "sub_sat": sub_sat, "hadd": hadd, "rhadd": rhadd,
+             "abs_diff": abs_diff, "mul24": mul24, "mul_hi": mul_hi}
+    return table[name](a, b)
+
+
+def random_trace(seed, kind="int", n_in=3, W=37, H=5, slots=2, depth=4, n_stmt=3):
+    """A random multi-statement trace with broadcast operands (smaller inputs),
+    temps, an output per statement and a sum reduction.  kind: int | fp."""
+    rng = np.random.default_rng(seed)
+    t = Trace()
+    precs = [U32, I32] if kind == "int" else [F32, F64, I32]
+    ins, data = [], {}
+    for k in range(n_in):
+        p = precs[rng.integers(len(precs))]
+        w = W if rng.random() < 0.7 else 1 + int(rng.integers(W))
+        h = H if rng.random() < 0.7 else 1
+        s = slots if rng.random() < 0.7 else 1
+        v = t.input(p, w, h, s)
+        ins.append(v)
+        if p == U32:
+            data[v.index] = rng.integers(0, 2 ** 32, size=(s, h, w), dtype=np.uint64).astype(np.uint32)
+        elif p == I32:
+            lo, hi = (-2 ** 31, 2 ** 31) if kind == "int" else (-50, 50)
+            data[v.index] = rng.integers(lo, hi, size=(s, h, w), dtype=np.int64).astype(np.int32)
+        else:
+            data[v.index] = (rng.random((s, h, w)) * 4 - 2).astype(np.float32 if p == F32 else np.float64)
+    ops = INT_BIN if kind == "int" else FP_BIN
+
+    def gen(d, pool):
+        if d == 0 or rng.random() < 0.25:
+            return pool[rng.integers(len(pool))]
+        name = ops[rng.integers(len(ops))]
+        a, b = gen(d - 1, pool), gen(d - 1, pool)
+        if kind == "fp" and name == "div":
+            b = fabs(b + f32(0.0)) + f32(0.5)      # (+ 0.0f lifts an int operand to f32)
+        return _bin(name, a, b)
+
+    pool = list(ins)
+    outs = []
+    for i in range(n_stmt):
+        e = gen(depth, pool)
+        tv = t.temp(F64 if kind == "fp" else U32, W, H, slots) if i < n_stmt - 1 else None
+        ov = t.output(F64 if kind == "fp" else U32, W, H, slots)
+        # full-shape anchor so the statement has the index space of its variable
+        anchor = u32(0) if kind == "int" else f32(0.0)
+        full = t.rng_uniform(U32 if kind == "int" else F32, W, H, slots, seed=seed + i)
+        e = e + (full & u32(0)) if kind == "int" else e + full * f32(0.0)
+        t.assign(ov, e)
+        outs.append(ov)
+        if tv is not None:
+            t.assign(tv, e * anchor + e)
+            pool.append(tv)
+    red = t.assign(t.output(F64 if kind == "fp" else U32, 1, 1, slots), sum_(outs[-1]))
+    return t, data, outs, red
