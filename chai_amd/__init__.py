@@ -213,6 +213,90 @@ def gemv(A, x, y=None, trans_a=False, general=False, alpha=1.0, beta=0.0, prec_a
     return y
 
 
+# ------------------------------------------------------------------ tuning
+def memo_open(path, device=0):
+    """Load + journal to `path` (autotune/AutoTuneMemo semantics)."""
+    L = lib()
+    L.chai_hip_memo_open.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    check(L.chai_hip_memo_open(str(path).encode(), device), "memo_open")
+
+
+def memo_close():
+    check(lib().chai_hip_memo_close(), "memo_close")
+
+
+def memo_save(path):
+    L = lib()
+    L.chai_hip_memo_save.argtypes = [ctypes.c_char_p]
+    check(L.chai_hip_memo_save(str(path).encode()), "memo_save")
+
+
+def memo_record(kernel, value):
+    L = lib()
+    L.chai_hip_memo_record.argtypes = [ctypes.c_char_p, ctypes.c_long]
+    check(L.chai_hip_memo_record(kernel.encode(), value), "memo_record")
+
+
+def memo_status(kernel):
+    """(status, us): 1 good / 0 unknown / -1 blacklisted."""
+    L = lib()
+    L.chai_hip_memo_kernel_status.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_long)]
+    s, us = ctypes.c_int(), ctypes.c_long()
+    check(L.chai_hip_memo_kernel_status(kernel.encode(), ctypes.byref(s), ctypes.byref(us)), "memo_status")
+    return s.value, us.value
+
+
+def _desc(A, B, C, trans_a=False, trans_b=False, mode=GEMM_AUTO, variant=-1):
+    M, K = (A.shape[1], A.shape[0]) if trans_a else (A.shape[0], A.shape[1])
+    N = B.shape[0] if trans_b else B.shape[1]
+    d = GemmDesc()
+    d.prec_a, d.prec_b, d.prec_c = prec_of(A.dtype), prec_of(B.dtype), prec_of(C.dtype)
+    d.trans_a, d.trans_b = int(trans_a), int(trans_b)
+    d.M, d.N, d.K = M, N, K
+    d.lda, d.ldb, d.ldc = A.stride(0), B.stride(0), C.stride(0)
+    d.batch = 1
+    d.mode = mode
+    d.variant = variant
+    return d
+
+
+def gemm_time_variant(A, B, C, variant, mode=GEMM_BF16X3, trials=3):
+    """core-kernel ms of one tile variant (journaled); -1 if blacklisted."""
+    init()
+    d = _desc(A, B, C, mode=mode)
+    L = lib()
+    L.chai_hip_gemm_time_variant.argtypes = [ctypes.POINTER(GemmDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]
+    ms = ctypes.c_float()
+    check(L.chai_hip_gemm_time_variant(ctypes.byref(d), _ptr(A), _ptr(B), _ptr(C), variant, trials,
+                                       ctypes.byref(ms), _stream()), "gemm_time_variant")
+    return ms.value
+
+
+def gemm_tune(A, B, C, mode=GEMM_BF16X3, trials=3):
+    """exhaustive variant search for this shape; returns (variant, core ms)."""
+    init()
+    d = _desc(A, B, C, mode=mode)
+    L = lib()
+    L.chai_hip_gemm_tune.argtypes = [ctypes.POINTER(GemmDesc), ctypes.c_void_p, ctypes.c_void_p,
+                                     ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]
+    v, ms = ctypes.c_int(), ctypes.c_float()
+    check(L.chai_hip_gemm_tune(ctypes.byref(d), _ptr(A), _ptr(B), _ptr(C), trials, ctypes.byref(v),
+                               ctypes.byref(ms), _stream()), "gemm_tune")
+    return v.value, ms.value
+
+
+def gemm_set_variant(A, B, C, variant, mode=GEMM_BF16X3):
+    """record `variant` for this shape (used when gemm() gets variant=-1)."""
+    d = _desc(A, B, C, mode=mode)
+    L = lib()
+    L.chai_hip_gemm_set_variant.argtypes = [ctypes.POINTER(GemmDesc), ctypes.c_int]
+    check(L.chai_hip_gemm_set_variant(ctypes.byref(d), variant), "gemm_set_variant")
+
+
 def profile(on=True):
     """Enable per-kernel hipEvent timing inside chai_hip_gemm."""
     check(lib().chai_hip_profile_enable(int(on)), "profile_enable")

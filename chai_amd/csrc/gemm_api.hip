@@ -49,7 +49,8 @@ extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const vo
         set_error("gemm: variant %d out of range for mode %d", d.variant, mode);
         return CHAI_ERR_ARG;
     }
-    return gemm_launch(d, A, B, C, mode, d.variant, as_stream(stream));
+    const int variant = d.variant >= 0 ? d.variant : tuned_variant(d, mode);
+    return gemm_launch(d, A, B, C, mode, variant, as_stream(stream));
 }
 
 extern "C" int chai_hip_gemm_num_variants(int mode, int* count)

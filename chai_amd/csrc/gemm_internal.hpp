@@ -30,5 +30,7 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
 int gemm_num_variants(int mode);
 int gemm_variant_params(int mode, int vid, int* params);   // BM, BN, BK, WGM, WGN
 void gemm_set_default_variant(int mode, int vid);
+// best variant recorded by the tuner for this exogenous key, or -1
+int tuned_variant(const chai_gemm_desc& d, int mode);
 
 }  // namespace chai_hip

@@ -1,7 +1,8 @@
 // reduce_device.hpp -- wavefront (64-lane) and workgroup reductions.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
-#include <type_traits>
+#endif
 
 namespace chai_hip {
 

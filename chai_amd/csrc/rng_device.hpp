@@ -15,8 +15,10 @@
 //     sin/cos(2 pi u) series (no contraction, explicit fma), rounded once.
 // All code here is compiled with -ffp-contract=off.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#endif
 
 namespace chai_hip {
 

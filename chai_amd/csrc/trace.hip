@@ -16,6 +16,9 @@
 #include "common.hpp"
 #include "gemm_internal.hpp"
 #include "trace_vm.hpp"
+#include "jit_headers.inc"
+
+#include <hip/hiprtc.h>
 
 #include <algorithm>
 #include <cstring>
@@ -34,225 +37,6 @@ constexpr int VM_NSLOTS = 12;
 #define VM_RD_CASE(n) case n: v = R##n; break;
 #define VM_WR_CASE(n) case n: R##n = v; break;
 #define VM_DECL(n) V4 R##n = {{0, 0, 0, 0}};
-
-__device__ __forceinline__ uint64_t vm_load_elem(const void* p, uint32_t prec, uint64_t idx)
-{
-    if (prec == CHAI_F64) return ((const uint64_t*)p)[idx];
-    return (uint64_t)((const uint32_t*)p)[idx];
-}
-
-// 32-bit stream element from the words of its vector (component comp)
-__device__ __forceinline__ uint64_t rng_elem32(const uint32_t w[4], int comp, int prec, int normal,
-                                               int has_lim, double lo, double hi)
-{
-    // select instead of indexing with a runtime component (keeps w in registers)
-    const uint32_t wc = comp == 0 ? w[0] : comp == 1 ? w[1] : comp == 2 ? w[2] : w[3];
-    if (prec == CHAI_U32 || prec == CHAI_I32) return (uint64_t)wc;
-    if (normal) {
-        const uint32_t wa = comp < 2 ? w[0] : w[2], wb = comp < 2 ? w[1] : w[3];
-        double n0, n1;
-        box_muller((double)unit_f32(wa), (double)unit_f32(wb), n0, n1);
-        return bf((float)((comp & 1) ? n1 : n0));
-    }
-    float u = unit_f32(wc);
-    if (has_lim) {
-        u = u * (float)(hi - lo);
-        if (lo != 0.0) u = u + (float)lo;
-    }
-    return bf(u);
-}
-
-__device__ __forceinline__ uint64_t rng_elem64(const uint64_t w[2], int comp, int normal, int has_lim,
-                                               double lo, double hi)
-{
-    const double u0 = unit_f64(w[0]), u1 = unit_f64(w[1]);
-    if (normal) {
-        double n0, n1;
-        box_muller(u0, u1, n0, n1);
-        return bd(comp ? n1 : n0);
-    }
-    double u = comp ? u1 : u0;
-    if (has_lim) {
-        u = u * (hi - lo);
-        if (lo != 0.0) u = u + lo;
-    }
-    return bd(u);
-}
-
-__device__ __forceinline__ void words32(int variant, int vl, uint32_t vidx, uint32_t seed, uint32_t w[4])
-{
-    if (variant == CHAI_RNG_THREEFRY) {
-        if (vl == 4) rng_words32<1231, 4>(vidx, seed, w);
-        else rng_words32<1231, 2>(vidx, seed, w);
-    } else {
-        if (vl == 4) rng_words32<1230, 4>(vidx, seed, w);
-        else rng_words32<1230, 2>(vidx, seed, w);
-    }
-}
-
-__device__ __forceinline__ void words64(int variant, uint64_t vidx, uint64_t seed, uint64_t w[2])
-{
-    if (variant == CHAI_RNG_THREEFRY) rng_words64<1231>(vidx, seed, w);
-    else rng_words64<1230>(vidx, seed, w);
-}
-
-__device__ V4 vm_rng(const VmIns& I, const VmRngDims& R, uint64_t slot, uint64_t y, uint64_t x0, int nval)
-{
-    V4 r = {{0, 0, 0, 0}};
-    const int variant = (int)I.a, vl = (int)I.b, normal = I.c & 1, lim = (I.c >> 1) & 1;
-    const int prec = I.p;
-    if (prec != CHAI_F64) {
-        if (R.same && vl == 4) {   // one stream vector covers the thread's 4 lanes
-            uint32_t w[4];
-            words32(variant, 4, (uint32_t)(slot * R.H * R.vpr + y * R.vpr + x0 / 4), (uint32_t)I.u, w);
-            if (normal && prec == CHAI_F32) {
-                double n0, n1, n2, n3;
-                box_muller((double)unit_f32(w[0]), (double)unit_f32(w[1]), n0, n1);
-                box_muller((double)unit_f32(w[2]), (double)unit_f32(w[3]), n2, n3);
-                r.l[0] = bf((float)n0); r.l[1] = bf((float)n1); r.l[2] = bf((float)n2); r.l[3] = bf((float)n3);
-            } else {
-#pragma unroll
-                for (int i = 0; i < 4; i++) r.l[i] = rng_elem32(w, i, prec, 0, lim, I.d0, I.d1);
-            }
-            return r;
-        }
-        for (int i = 0; i < 4; i++) if (i < nval) {
-            const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
-            uint32_t w[4];
-            words32(variant, vl, (uint32_t)(sr * R.H * R.vpr + yr * R.vpr + xr / vl), (uint32_t)I.u, w);
-            r.l[i] = rng_elem32(w, (int)(xr % vl), prec, normal, lim, I.d0, I.d1);
-        }
-        return r;
-    }
-    for (int i = 0; i < 4; i++) if (i < nval) {
-        const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
-        uint64_t w[2];
-        words64(variant, sr * R.H * R.vpr + yr * R.vpr + xr / vl, I.u, w);
-        r.l[i] = rng_elem64(w, (int)(xr % vl), normal, lim, I.d0, I.d1);
-    }
-    return r;
-}
-
-__device__ __forceinline__ V4 vm_load(const VmVar& V, uint64_t slot, uint64_t y, uint64_t x0, int nval,
-                                      uint64_t W, uint64_t H)
-{
-    V4 r = {{0, 0, 0, 0}};
-    if (V.same) {
-        const uint64_t base = (slot * H + y) * W + x0;
-        if (nval == 4 && V.prec != CHAI_F64 && ((W & 3) == 0) && (((uintptr_t)V.in & 15) == 0)) {
-            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-            const u4 q = *reinterpret_cast<const u4*>((const uint32_t*)V.in + base);
-            r.l[0] = q[0]; r.l[1] = q[1]; r.l[2] = q[2]; r.l[3] = q[3];
-            return r;
-        }
-        for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, base + i);
-        return r;
-    }
-    if (V.W == 1 && V.H == 1 && V.S == 1) {
-        const uint64_t e = vm_load_elem(V.in, V.prec, 0);
-        r.l[0] = r.l[1] = r.l[2] = r.l[3] = e;
-        return r;
-    }
-    const uint64_t rowoff = ((slot % V.S) * V.H + (y % V.H)) * V.W;
-    for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, rowoff + (x0 + i) % V.W);
-    return r;
-}
-
-__device__ __forceinline__ void vm_store(const VmVar& V, const V4& v, uint64_t slot, uint64_t y,
-                                         uint64_t x0, int nval, uint64_t W, uint64_t H)
-{
-    const uint64_t base = (slot * H + y) * W + x0;
-    if (V.prec == CHAI_F64) {
-        for (int i = 0; i < 4; i++) if (i < nval) ((uint64_t*)V.out)[base + i] = v.l[i];
-    } else if (nval == 4 && (W & 3) == 0 && (((uintptr_t)V.out & 15) == 0)) {
-        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-        u4 q = {(uint32_t)v.l[0], (uint32_t)v.l[1], (uint32_t)v.l[2], (uint32_t)v.l[3]};
-        *reinterpret_cast<u4*>((uint32_t*)V.out + base) = q;
-    } else {
-        for (int i = 0; i < 4; i++) if (i < nval) ((uint32_t*)V.out)[base + i] = (uint32_t)v.l[i];
-    }
-}
-
-__device__ __forceinline__ V4 vm_bin(uint32_t op, int prec, const V4& a, const V4& b)
-{
-    V4 r;
-    switch (prec) {
-        case CHAI_F32:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bf(fp_bin<float>(op, f_(a.l[i]), f_(b.l[i])));
-            break;
-        case CHAI_F64:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bd(fp_bin<double>(op, d_(a.l[i]), d_(b.l[i])));
-            break;
-        case CHAI_I32:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bi(i_bin(op, i_(a.l[i]), i_(b.l[i])));
-            break;
-        default:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bu(u_bin(op, u_(a.l[i]), u_(b.l[i])));
-            break;
-    }
-    return r;
-}
-
-__device__ __forceinline__ V4 vm_un(uint32_t op, int prec, const V4& a)
-{
-    V4 r;
-    switch (prec) {
-        case CHAI_F32:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bf(fp_un<float>(op, f_(a.l[i])));
-            break;
-        case CHAI_F64:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bd(fp_un<double>(op, d_(a.l[i])));
-            break;
-        case CHAI_I32:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bi(i_un(op, i_(a.l[i])));
-            break;
-        default:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bu(u_un(op, u_(a.l[i])));
-            break;
-    }
-    return r;
-}
-
-__device__ __forceinline__ V4 vm_ter(uint32_t op, int prec, const V4& a, const V4& b, const V4& c)
-{
-    V4 r;
-    switch (prec) {
-        case CHAI_F32:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bf(ter<float>(op, f_(a.l[i]), f_(b.l[i]), f_(c.l[i])));
-            break;
-        case CHAI_F64:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bd(ter<double>(op, d_(a.l[i]), d_(b.l[i]), d_(c.l[i])));
-            break;
-        case CHAI_I32:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bi(ter<int32_t>(op, i_(a.l[i]), i_(b.l[i]), i_(c.l[i])));
-            break;
-        default:
-#pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bu(ter<uint32_t>(op, u_(a.l[i]), u_(b.l[i]), u_(c.l[i])));
-            break;
-    }
-    return r;
-}
-
-struct VmLaunchArgs {
-    const VmIns* prog;
-    uint32_t nins, nacc;
-    uint64_t W, H, S, vpr, vps;
-    double* partial;
-    VmVar vars[24];
-    VmRngDims rng[8];
-};
 
 __global__ __launch_bounds__(VM_THREADS) void trace_vm_kernel(VmLaunchArgs L)
 {
@@ -289,7 +73,11 @@ __global__ __launch_bounds__(VM_THREADS) void trace_vm_kernel(VmLaunchArgs L)
             switch (I.op) {
                 case VM_LOAD: v = vm_load(var_at(I.a), slot, y, x0, nval, L.W, L.H); break;
                 case VM_IMM: v.l[0] = v.l[1] = v.l[2] = v.l[3] = I.u; break;
-                case VM_RNG: v = vm_rng(I, rng_at(I.e), slot, y, x0, nval); break;
+                case VM_RNG: {
+                    const RngArgs ra = {I.a, I.b, I.c, I.p, I.u, I.d0, I.d1};
+                    v = vm_rng(ra, rng_at(I.e), slot, y, x0, nval);
+                    break;
+                }
                 case VM_INDEX:
                     for (int i = 0; i < 4; i++)
                         v.l[i] = from_double(I.p, (double)(I.a == 0 ? x0 + i : y));
@@ -443,6 +231,10 @@ struct Group {
     std::vector<uint32_t> red_vars;      // output variable of each accumulator
     std::vector<double> red_div;
     VmIns* dprog = nullptr;
+    // JIT: the same program as straight-line HIP, compiled with hipRTC
+    std::string jit_src, jit_log;
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
 };
 
 }  // namespace
@@ -455,8 +247,10 @@ struct TraceProgram {
     int ninstr = 0;
     ~TraceProgram()
     {
-        for (auto& g : groups)
+        for (auto& g : groups) {
             if (g.dprog) (void)hipFree(g.dprog);
+            if (g.mod) (void)hipModuleUnload(g.mod);
+        }
     }
 };
 
@@ -906,8 +700,21 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
         }
         if (G.var_of_slot.size() > 24) return fail("more than 24 arrays in one fused kernel");
         P.ninstr += (int)G.prog.size();
+    }
+    return CHAI_OK;
+}
+
+static bool jit_enabled();
+static void jit_build(const TraceProgram& P, Group& G);
+
+// device side of compilation: upload VM programs, JIT-compile when enabled
+static int materialize(TraceProgram& P)
+{
+    for (auto& G : P.groups) {
+        if (G.kind == 2) continue;
         CHAI_HIP_TRY(hipMalloc((void**)&G.dprog, sizeof(VmIns) * G.prog.size()));
         CHAI_HIP_TRY(hipMemcpy(G.dprog, G.prog.data(), sizeof(VmIns) * G.prog.size(), hipMemcpyHostToDevice));
+        if (jit_enabled()) jit_build(P, G);
     }
     return CHAI_OK;
 }
@@ -953,6 +760,181 @@ static int run_matmul(const TraceProgram& P, const Stmt& s, void* const* bufs, h
     d.mode = CHAI_GEMM_AUTO;
     d.variant = -1;
     return chai_hip_gemm(&d, bufs[va], bufs[vb], bufs[s.lhs], st);
+}
+
+// ------------------------------------------------------------ trace JIT
+// The typed VM program of a group, emitted as straight-line HIP: every VM
+// instruction becomes one statement on register variables R0..R11, array
+// shapes and instruction fields become literals, so the compiler specialises
+// every load, op and store (the role the reference's OpenCL text emission
+// plays, enqueue/EnqueueIdSpace.cpp:19-320).  Semantics are the VM's own
+// device functions (trace_vm.hpp), so JIT and VM agree bit for bit.
+static int g_jit_mode = -1;   // -1: read CHAI_TRACE_JIT once; 0 off; 1 on
+
+static bool jit_enabled()
+{
+    if (g_jit_mode < 0) {
+        const char* e = getenv("CHAI_TRACE_JIT");
+        g_jit_mode = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_jit_mode == 1;
+}
+
+static std::string hexd(double d)
+{
+    char b[64];
+    snprintf(b, sizeof b, "%a", d);
+    if (d != d) return "__builtin_nan(\"\")";
+    if (d == __builtin_inf()) return "__builtin_inf()";
+    if (d == -__builtin_inf()) return "(-__builtin_inf())";
+    return b;
+}
+
+static std::string jit_source(const TraceProgram& P, const Group& G)
+{
+    const uint64_t W = G.space.W, H = G.space.H, S = G.space.S;
+    const uint64_t vpr = (W + 3) / 4, vps = vpr * H;
+    std::string s;
+    char b[512];
+    s += "#include \"trace_vm.hpp\"\n";
+    s += "extern \"C\" __global__ __launch_bounds__(256) void chai_trace_kernel(chai_hip::VmLaunchArgs L)\n{\n";
+    s += "    using namespace chai_hip;\n    const uint64_t slot = blockIdx.y;\n";
+    for (int r = 0; r < VM_NSLOTS; r++) { snprintf(b, sizeof b, "    V4 R%d = {{0, 0, 0, 0}};\n", r); s += b; }
+    s += "    double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0;\n";
+    snprintf(b, sizeof b,
+             "    for (uint64_t v = (uint64_t)blockIdx.x * 256 + threadIdx.x; v < %lluull; v += (uint64_t)gridDim.x * 256) {\n"
+             "        const uint64_t y = v / %lluull;\n        const uint64_t x0 = (v - y * %lluull) * 4;\n"
+             "        const int nval = (%lluull - x0) < 4 ? (int)(%lluull - x0) : 4;\n",
+             (unsigned long long)vps, (unsigned long long)vpr, (unsigned long long)vpr,
+             (unsigned long long)W, (unsigned long long)W);
+    s += b;
+    for (const VmIns& I : G.prog) {
+        const auto var_lit = [&](uint32_t k, bool in) {
+            const uint32_t v = G.var_of_slot[k];
+            const chai_trace_var& tv = P.vars[v];
+            const int same = (tv.W == W && tv.H == H && tv.slots == S) ? 1 : 0;
+            char t[256];
+            snprintf(t, sizeof t, "VmVar{%s, %s, %lluull, %lluull, %lluull, %uu, %uu}",
+                     in ? ("L.vars[" + std::to_string(k) + "].in").c_str() : "nullptr",
+                     in ? "nullptr" : ("L.vars[" + std::to_string(k) + "].out").c_str(),
+                     (unsigned long long)tv.W, (unsigned long long)tv.H, (unsigned long long)tv.slots,
+                     tv.prec, same);
+            return std::string(t);
+        };
+        switch (I.op) {
+            case VM_LOAD:
+                snprintf(b, sizeof b, "        R%d = vm_load(%s, slot, y, x0, nval, %lluull, %lluull);\n", I.dst,
+                         var_lit(I.a, true).c_str(), (unsigned long long)W, (unsigned long long)H);
+                break;
+            case VM_IMM:
+                snprintf(b, sizeof b, "        R%d.l[0] = R%d.l[1] = R%d.l[2] = R%d.l[3] = %lluull;\n", I.dst, I.dst,
+                         I.dst, I.dst, (unsigned long long)I.u);
+                break;
+            case VM_RNG: {
+                const VmRngDims& R = G.rng[I.e];
+                snprintf(b, sizeof b,
+                         "        R%d = vm_rng(RngArgs{%uu, %uu, %uu, %uu, %lluull, %s, %s}, "
+                         "VmRngDims{%lluull, %lluull, %lluull, %lluull, %uu, 0u}, slot, y, x0, nval);\n",
+                         I.dst, I.a, I.b, I.c, (unsigned)I.p, (unsigned long long)I.u, hexd(I.d0).c_str(),
+                         hexd(I.d1).c_str(), (unsigned long long)R.W, (unsigned long long)R.H,
+                         (unsigned long long)R.S, (unsigned long long)R.vpr, R.same);
+                break;
+            }
+            case VM_INDEX:
+                snprintf(b, sizeof b,
+                         "        for (int i = 0; i < 4; i++) R%d.l[i] = from_double(%u, (double)(%s));\n",
+                         I.dst, (unsigned)I.p, I.a == 0 ? "x0 + i" : "y");
+                break;
+            case VM_CVT:
+                snprintf(b, sizeof b,
+                         "        { V4 t = R%d; for (int i = 0; i < 4; i++) R%d.l[i] = from_double(%u, as_double(%u, t.l[i])); }\n",
+                         I.s1, I.dst, (unsigned)I.p, (unsigned)I.p2);
+                break;
+            case VM_UN:
+                snprintf(b, sizeof b, "        R%d = vm_un(%uu, %u, R%d);\n", I.dst, I.a, (unsigned)I.p, I.s1);
+                break;
+            case VM_BIN:
+                snprintf(b, sizeof b, "        R%d = vm_bin(%uu, %u, R%d, R%d);\n", I.dst, I.a, (unsigned)I.p, I.s1, I.s2);
+                break;
+            case VM_TER:
+                snprintf(b, sizeof b, "        R%d = vm_ter(%uu, %u, R%d, R%d, R%d);\n", I.dst, I.a, (unsigned)I.p, I.s1,
+                         I.s2, I.s3);
+                break;
+            case VM_SEL:
+                snprintf(b, sizeof b,
+                         "        { V4 c = R%d, p = R%d, q = R%d; for (int i = 0; i < 4; i++) R%d.l[i] = as_double(%u, c.l[i]) != 0.0 ? p.l[i] : q.l[i]; }\n",
+                         I.s1, I.s2, I.s3, I.dst, (unsigned)I.p2);
+                break;
+            case VM_STORE:
+                snprintf(b, sizeof b, "        vm_store(%s, R%d, slot, y, x0, nval, %lluull, %lluull);\n",
+                         var_lit(I.a, false).c_str(), I.s1, (unsigned long long)W, (unsigned long long)H);
+                break;
+            case VM_ACC:
+                snprintf(b, sizeof b,
+                         "        { double t = 0.0; for (int i = 0; i < 4; i++) if (i < nval) t += as_double(%u, R%d.l[i]); acc%u += t; }\n",
+                         (unsigned)I.p, I.s1, I.a);
+                break;
+            default:
+                snprintf(b, sizeof b, "        R%d = R%d;\n", I.dst, I.s1);
+                break;
+        }
+        s += b;
+    }
+    s += "    }\n";
+    if (!G.red_vars.empty()) {
+        snprintf(b, sizeof b, "    const uint64_t base = ((uint64_t)slot * gridDim.x + blockIdx.x) * %zu;\n", G.red_vars.size());
+        s += b;
+        for (size_t k = 0; k < G.red_vars.size(); k++) {
+            snprintf(b, sizeof b,
+                     "    { const double r = block_sum_f64<256>(acc%zu); if (threadIdx.x == 0) L.partial[base + %zu] = r; }\n", k, k);
+            s += b;
+        }
+    }
+    s += "}\n";
+    return s;
+}
+
+static const char* kJitPrelude =
+    "typedef __hip_internal::uint8_t uint8_t;\ntypedef __hip_internal::uint16_t uint16_t;\n"
+    "typedef __hip_internal::uint32_t uint32_t;\ntypedef __hip_internal::uint64_t uint64_t;\n"
+    "typedef __hip_internal::int32_t int32_t;\ntypedef __hip_internal::int64_t int64_t;\n"
+    "typedef unsigned long uintptr_t;\n#define INT32_MAX 2147483647\n#define INT32_MIN (-2147483647 - 1)\n"
+    "enum { CHAI_U32 = 0, CHAI_I32 = 1, CHAI_F32 = 4, CHAI_F64 = 8, CHAI_RNG_PHILOX = 1230, CHAI_RNG_THREEFRY = 1231 };\n";
+
+// Compile one group's source; on failure the group keeps running on the VM.
+static void jit_build(const TraceProgram& P, Group& G)
+{
+    G.jit_src = jit_source(P, G);
+    const std::string full = std::string(kJitPrelude) + G.jit_src;
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, full.c_str(), "chai_trace.hip", kJitNumHeaders, kJitHeaders, kJitHeaderNames) !=
+        HIPRTC_SUCCESS) {
+        G.jit_log = "hiprtcCreateProgram failed";
+        return;
+    }
+    const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off",
+                          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+    const hiprtcResult rc = hiprtcCompileProgram(prog, 8, opts);
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    if (ls > 1) {
+        G.jit_log.resize(ls);
+        hiprtcGetProgramLog(prog, &G.jit_log[0]);
+    }
+    if (rc == HIPRTC_SUCCESS) {
+        size_t n = 0;
+        hiprtcGetCodeSize(prog, &n);
+        std::vector<char> code(n);
+        hiprtcGetCode(prog, code.data());
+        if (hipModuleLoadData(&G.mod, code.data()) == hipSuccess &&
+            hipModuleGetFunction(&G.fn, G.mod, "chai_trace_kernel") == hipSuccess) {
+            // compiled and loaded
+        } else {
+            G.fn = nullptr;
+            G.jit_log += "\nhipModuleLoadData / GetFunction failed";
+        }
+    }
+    hiprtcDestroyProgram(&prog);
 }
 
 static int launch_program(const TraceProgram& P, void* const* user, hipStream_t st)
@@ -1003,8 +985,14 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
             CHAI_HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * L.S * bps * L.nacc, st));
             L.partial = partial;
         }
-        trace_vm_kernel<<<dim3((unsigned)bps, (unsigned)L.S), VM_THREADS, 0, st>>>(L);
-        hipError_t e = hipGetLastError();
+        hipError_t e;
+        if (G.fn) {
+            void* args[] = {&L};
+            e = hipModuleLaunchKernel(G.fn, (unsigned)bps, (unsigned)L.S, 1, VM_THREADS, 1, 1, 0, st, args, nullptr);
+        } else {
+            trace_vm_kernel<<<dim3((unsigned)bps, (unsigned)L.S), VM_THREADS, 0, st>>>(L);
+            e = hipGetLastError();
+        }
         if (e == hipSuccess && L.nacc) {
             RedOut o[4];
             std::memset(o, 0, sizeof o);
@@ -1059,6 +1047,7 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
     key.append((const char*)trace->vars, sizeof(chai_trace_var) * trace->num_vars);
     key.append((const char*)trace->stmts, sizeof(chai_trace_stmt) * trace->num_stmts);
     key.append((const char*)trace->nodes, sizeof(chai_trace_node) * trace->num_nodes);
+    key.push_back(jit_enabled() ? 'J' : 'V');
     const uint64_t h = fnv_hash(key.data(), key.size(), 1469598103934665603ull);
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -1071,7 +1060,8 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
                 }
     }
     auto prog = std::make_shared<TraceProgram>();
-    const int rc = compile_trace(*trace, *prog);
+    int rc = compile_trace(*trace, *prog);
+    if (rc == CHAI_OK) rc = materialize(*prog);
     if (rc) return rc;
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
@@ -1099,5 +1089,45 @@ extern "C" int chai_hip_trace_info(void* handle, int* num_kernels, int* num_inst
 extern "C" int chai_hip_trace_release(void* handle)
 {
     delete static_cast<Handle*>(handle);
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_trace_set_jit(int on)
+{
+    g_jit_mode = on ? 1 : 0;
+    return CHAI_OK;
+}
+
+static int copy_out(const std::string& s, char* buf, size_t len)
+{
+    if (buf && len) {
+        const size_t n = s.size() < len - 1 ? s.size() : len - 1;
+        std::memcpy(buf, s.data(), n);
+        buf[n] = 0;
+    }
+    return (int)s.size();
+}
+
+extern "C" int chai_hip_trace_kernel_source(void* handle, int kernel, char* buf, size_t len, int* jitted)
+{
+    if (!handle) { set_error("trace_kernel_source: null handle"); return CHAI_ERR_STATE; }
+    const TraceProgram& P = *static_cast<Handle*>(handle)->prog;
+    if (kernel < 0 || kernel >= (int)P.groups.size()) { set_error("trace_kernel_source: no kernel %d", kernel); return CHAI_ERR_ARG; }
+    const Group& G = P.groups[kernel];
+    if (jitted) *jitted = G.fn ? 1 : 0;
+    return copy_out(G.fn ? G.jit_src : G.jit_log, buf, len) >= 0 ? CHAI_OK : CHAI_ERR_ARG;
+}
+
+extern "C" int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, char* buf, size_t len)
+{
+    if (!trace) { set_error("trace_emit_source: null trace"); return CHAI_ERR_ARG; }
+    TraceProgram P;
+    const int rc = compile_trace(*trace, P);
+    if (rc) return rc;
+    if (kernel < 0 || kernel >= (int)P.groups.size() || P.groups[kernel].kind == 2) {
+        set_error("trace_emit_source: kernel %d is not a fused VM kernel", kernel);
+        return CHAI_ERR_ARG;
+    }
+    copy_out(std::string(kJitPrelude) + jit_source(P, P.groups[kernel]), buf, len);
     return CHAI_OK;
 }

@@ -14,8 +14,11 @@
 // bits of the count (x86 behaviour of the reference build), saturating
 // RHADD, non-rotating signed ROTATE, max/min as a<b?b:a.
 #pragma once
+#ifndef __HIPCC_RTC__
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include "../../include/chai_mi355x.h"
+#endif
 
 #include "rng_device.hpp"
 #include "reduce_device.hpp"
@@ -287,5 +290,236 @@ __device__ __forceinline__ T ter(uint32_t op, T a, T b, T c)
         default: return (T)0;
     }
 }
+
+// ------------------------------------------------ loads, stores, rng, ops
+__device__ __forceinline__ uint64_t vm_load_elem(const void* p, uint32_t prec, uint64_t idx)
+{
+    if (prec == CHAI_F64) return ((const uint64_t*)p)[idx];
+    return (uint64_t)((const uint32_t*)p)[idx];
+}
+
+// 32-bit stream element from the words of its vector (component comp)
+__device__ __forceinline__ uint64_t rng_elem32(const uint32_t w[4], int comp, int prec, int normal,
+                                               int has_lim, double lo, double hi)
+{
+    // select instead of indexing with a runtime component (keeps w in registers)
+    const uint32_t wc = comp == 0 ? w[0] : comp == 1 ? w[1] : comp == 2 ? w[2] : w[3];
+    if (prec == CHAI_U32 || prec == CHAI_I32) return (uint64_t)wc;
+    if (normal) {
+        const uint32_t wa = comp < 2 ? w[0] : w[2], wb = comp < 2 ? w[1] : w[3];
+        double n0, n1;
+        box_muller((double)unit_f32(wa), (double)unit_f32(wb), n0, n1);
+        return bf((float)((comp & 1) ? n1 : n0));
+    }
+    float u = unit_f32(wc);
+    if (has_lim) {
+        u = u * (float)(hi - lo);
+        if (lo != 0.0) u = u + (float)lo;
+    }
+    return bf(u);
+}
+
+__device__ __forceinline__ uint64_t rng_elem64(const uint64_t w[2], int comp, int normal, int has_lim,
+                                               double lo, double hi)
+{
+    const double u0 = unit_f64(w[0]), u1 = unit_f64(w[1]);
+    if (normal) {
+        double n0, n1;
+        box_muller(u0, u1, n0, n1);
+        return bd(comp ? n1 : n0);
+    }
+    double u = comp ? u1 : u0;
+    if (has_lim) {
+        u = u * (hi - lo);
+        if (lo != 0.0) u = u + lo;
+    }
+    return bd(u);
+}
+
+__device__ __forceinline__ void words32(int variant, int vl, uint32_t vidx, uint32_t seed, uint32_t w[4])
+{
+    if (variant == CHAI_RNG_THREEFRY) {
+        if (vl == 4) rng_words32<1231, 4>(vidx, seed, w);
+        else rng_words32<1231, 2>(vidx, seed, w);
+    } else {
+        if (vl == 4) rng_words32<1230, 4>(vidx, seed, w);
+        else rng_words32<1230, 2>(vidx, seed, w);
+    }
+}
+
+__device__ __forceinline__ void words64(int variant, uint64_t vidx, uint64_t seed, uint64_t w[2])
+{
+    if (variant == CHAI_RNG_THREEFRY) rng_words64<1231>(vidx, seed, w);
+    else rng_words64<1230>(vidx, seed, w);
+}
+
+// Instruction fields arrive by value so the instruction never needs an
+// address (an addressed copy would live in scratch and lose its uniformity).
+struct RngArgs {
+    uint32_t variant, vl, flags, prec;
+    uint64_t seed;
+    double lo, hi;
+};
+
+__device__ __forceinline__ V4 vm_rng(const RngArgs A, const VmRngDims R, uint64_t slot, uint64_t y,
+                                     uint64_t x0, int nval)
+{
+    V4 r = {{0, 0, 0, 0}};
+    const int variant = (int)A.variant, vl = (int)A.vl, normal = A.flags & 1, lim = (A.flags >> 1) & 1;
+    const int prec = (int)A.prec;
+    struct { uint64_t u; double d0, d1; } I = {A.seed, A.lo, A.hi};
+    if (prec != CHAI_F64) {
+        if (R.same && vl == 4) {   // one stream vector covers the thread's 4 lanes
+            uint32_t w[4];
+            words32(variant, 4, (uint32_t)(slot * R.H * R.vpr + y * R.vpr + x0 / 4), (uint32_t)I.u, w);
+            if (normal && prec == CHAI_F32) {
+                double n0, n1, n2, n3;
+                box_muller((double)unit_f32(w[0]), (double)unit_f32(w[1]), n0, n1);
+                box_muller((double)unit_f32(w[2]), (double)unit_f32(w[3]), n2, n3);
+                r.l[0] = bf((float)n0); r.l[1] = bf((float)n1); r.l[2] = bf((float)n2); r.l[3] = bf((float)n3);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++) r.l[i] = rng_elem32(w, i, prec, 0, lim, I.d0, I.d1);
+            }
+            return r;
+        }
+        for (int i = 0; i < 4; i++) if (i < nval) {
+            const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
+            uint32_t w[4];
+            words32(variant, vl, (uint32_t)(sr * R.H * R.vpr + yr * R.vpr + xr / vl), (uint32_t)I.u, w);
+            r.l[i] = rng_elem32(w, (int)(xr % vl), prec, normal, lim, I.d0, I.d1);
+        }
+        return r;
+    }
+    for (int i = 0; i < 4; i++) if (i < nval) {
+        const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
+        uint64_t w[2];
+        words64(variant, sr * R.H * R.vpr + yr * R.vpr + xr / vl, I.u, w);
+        r.l[i] = rng_elem64(w, (int)(xr % vl), normal, lim, I.d0, I.d1);
+    }
+    return r;
+}
+
+__device__ __forceinline__ V4 vm_load(const VmVar& V, uint64_t slot, uint64_t y, uint64_t x0, int nval,
+                                      uint64_t W, uint64_t H)
+{
+    V4 r = {{0, 0, 0, 0}};
+    if (V.same) {
+        const uint64_t base = (slot * H + y) * W + x0;
+        if (nval == 4 && V.prec != CHAI_F64 && ((W & 3) == 0) && (((uintptr_t)V.in & 15) == 0)) {
+            typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+            const u4 q = *reinterpret_cast<const u4*>((const uint32_t*)V.in + base);
+            r.l[0] = q[0]; r.l[1] = q[1]; r.l[2] = q[2]; r.l[3] = q[3];
+            return r;
+        }
+        for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, base + i);
+        return r;
+    }
+    if (V.W == 1 && V.H == 1 && V.S == 1) {
+        const uint64_t e = vm_load_elem(V.in, V.prec, 0);
+        r.l[0] = r.l[1] = r.l[2] = r.l[3] = e;
+        return r;
+    }
+    const uint64_t rowoff = ((slot % V.S) * V.H + (y % V.H)) * V.W;
+    for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, rowoff + (x0 + i) % V.W);
+    return r;
+}
+
+__device__ __forceinline__ void vm_store(const VmVar& V, const V4& v, uint64_t slot, uint64_t y,
+                                         uint64_t x0, int nval, uint64_t W, uint64_t H)
+{
+    const uint64_t base = (slot * H + y) * W + x0;
+    if (V.prec == CHAI_F64) {
+        for (int i = 0; i < 4; i++) if (i < nval) ((uint64_t*)V.out)[base + i] = v.l[i];
+    } else if (nval == 4 && (W & 3) == 0 && (((uintptr_t)V.out & 15) == 0)) {
+        typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+        u4 q = {(uint32_t)v.l[0], (uint32_t)v.l[1], (uint32_t)v.l[2], (uint32_t)v.l[3]};
+        *reinterpret_cast<u4*>((uint32_t*)V.out + base) = q;
+    } else {
+        for (int i = 0; i < 4; i++) if (i < nval) ((uint32_t*)V.out)[base + i] = (uint32_t)v.l[i];
+    }
+}
+
+__device__ __forceinline__ V4 vm_bin(uint32_t op, int prec, const V4& a, const V4& b)
+{
+    V4 r;
+    switch (prec) {
+        case CHAI_F32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bf(fp_bin<float>(op, f_(a.l[i]), f_(b.l[i])));
+            break;
+        case CHAI_F64:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bd(fp_bin<double>(op, d_(a.l[i]), d_(b.l[i])));
+            break;
+        case CHAI_I32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bi(i_bin(op, i_(a.l[i]), i_(b.l[i])));
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bu(u_bin(op, u_(a.l[i]), u_(b.l[i])));
+            break;
+    }
+    return r;
+}
+
+__device__ __forceinline__ V4 vm_un(uint32_t op, int prec, const V4& a)
+{
+    V4 r;
+    switch (prec) {
+        case CHAI_F32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bf(fp_un<float>(op, f_(a.l[i])));
+            break;
+        case CHAI_F64:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bd(fp_un<double>(op, d_(a.l[i])));
+            break;
+        case CHAI_I32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bi(i_un(op, i_(a.l[i])));
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bu(u_un(op, u_(a.l[i])));
+            break;
+    }
+    return r;
+}
+
+__device__ __forceinline__ V4 vm_ter(uint32_t op, int prec, const V4& a, const V4& b, const V4& c)
+{
+    V4 r;
+    switch (prec) {
+        case CHAI_F32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bf(ter<float>(op, f_(a.l[i]), f_(b.l[i]), f_(c.l[i])));
+            break;
+        case CHAI_F64:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bd(ter<double>(op, d_(a.l[i]), d_(b.l[i]), d_(c.l[i])));
+            break;
+        case CHAI_I32:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bi(ter<int32_t>(op, i_(a.l[i]), i_(b.l[i]), i_(c.l[i])));
+            break;
+        default:
+#pragma unroll
+            for (int i = 0; i < 4; i++) r.l[i] = bu(ter<uint32_t>(op, u_(a.l[i]), u_(b.l[i]), u_(c.l[i])));
+            break;
+    }
+    return r;
+}
+
+struct VmLaunchArgs {
+    const VmIns* prog;
+    uint32_t nins, nacc;
+    uint64_t W, H, S, vpr, vps;
+    double* partial;
+    VmVar vars[24];
+    VmRngDims rng[8];
+};
+
 
 }  // namespace chai_hip

@@ -1,0 +1,271 @@
+// tune.hip -- autotuning: parameter space walk, hipEvent timing, the
+// crash-safe journal memo and the per-shape best-variant memo.
+//
+// Replaces (behaviour kept, code new):
+//   autotune/AutoTuneMemo.cpp:16-318  journal state machine: per kernel name
+//       PRINT_SOURCE -> COMPILE_KERNEL -> SET_KERNEL_ARGS -> START_TIMER ->
+//       STOP_TIMER -> GOOD_OUTPUT (+ microsecond lines); any other transition
+//       marks the kernel bad; after a reload, kernels whose last state is
+//       COMPILE_KERNEL, START_TIMER or BEFORE_CHECK crashed or hung and are
+//       blacklisted.  Journal lines are "<device> <kernelName> <value>".
+//   autotune/AutoTuneBench.cpp:17-251 timing of one candidate (memo hit ->
+//       cached time; else run, time, check output).
+//   autotune/AutoTuneEM.cpp:48-131 / StandardEM.cpp:69-148 search: here the
+//       space is the tile-variant table of the kernel family, searched
+//       exhaustively (it is small); best = minimum total time over trials.
+//   enqueue/JustInTimeMemo.cpp:31-114 exogenous key -> best parameters,
+//       consulted by chai_hip_gemm when no variant is given.
+#include "common.hpp"
+#include "gemm_internal.hpp"
+
+#include <cmath>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <set>
+#include <sstream>
+#include <string>
+#include <vector>
+
+namespace chai_hip {
+
+namespace {
+
+// journal states (values as in autotune/AutoTuneMemo.hpp:19-27)
+constexpr long PRINT_SOURCE = -1, COMPILE_KERNEL = -2, SET_KERNEL_ARGS = -3, START_TIMER = -4,
+               STOP_TIMER = -5, GOOD_OUTPUT = -6, BAD_OUTPUT = -7, BEFORE_CHECK = -100,
+               AFTER_CHECK = -101;
+
+struct Memo {
+    int device = 0;
+    std::set<std::string> journal_files;
+    std::map<std::string, long> state;
+    std::set<std::string> bad;
+    std::map<std::string, std::vector<long>> bench;
+
+    void update(const std::string& k, long v)
+    {
+        if (v < 0) {
+            bool badk = v == BAD_OUTPUT;
+            auto it = state.find(k);
+            if (it == state.end()) {
+                if (v != PRINT_SOURCE) badk = true;
+            } else {
+                switch (it->second) {
+                    case PRINT_SOURCE: badk |= v != COMPILE_KERNEL; break;
+                    case COMPILE_KERNEL: badk |= v != SET_KERNEL_ARGS; break;
+                    case SET_KERNEL_ARGS: badk |= v != START_TIMER; break;
+                    case START_TIMER: badk |= v != STOP_TIMER; break;
+                    case STOP_TIMER: badk |= v != GOOD_OUTPUT; break;
+                    case GOOD_OUTPUT: badk |= v != PRINT_SOURCE && v != BEFORE_CHECK; break;
+                    case BAD_OUTPUT: break;
+                    case BEFORE_CHECK: badk |= v != AFTER_CHECK; break;
+                    case AFTER_CHECK: badk |= v != PRINT_SOURCE && v != BEFORE_CHECK; break;
+                    default: break;
+                }
+            }
+            if (badk) bad.insert(k);
+            state[k] = v;
+        } else {
+            bench[k].push_back(v);
+        }
+    }
+
+    bool add(const std::string& k, long v)
+    {
+        update(k, v);
+        bool ok = true;
+        for (const auto& f : journal_files) {
+            std::ofstream o(f, std::ios::app);
+            if (!o) { ok = false; continue; }
+            o << device << " " << k << " " << v << "\n";
+            o.flush();
+        }
+        return ok;
+    }
+
+    bool load(const std::string& f)
+    {
+        std::ifstream in(f);
+        if (!in) return false;
+        long dev;
+        std::string k;
+        long v;
+        while (in >> dev >> k >> v)
+            if (dev == device) update(k, v);
+        for (const auto& kv : state)
+            if (kv.second == COMPILE_KERNEL || kv.second == START_TIMER || kv.second == BEFORE_CHECK)
+                add(kv.first, BAD_OUTPUT);
+        return true;
+    }
+
+    bool good(const std::string& k) const { return !bad.count(k) && bench.count(k); }
+};
+
+std::mutex g_mu;
+Memo g_memo;
+std::map<std::string, int> g_best;            // exogenous key -> variant (JustInTimeMemo)
+
+std::string exo_key(const chai_gemm_desc& d, int mode)
+{
+    std::ostringstream s;
+    s << "m" << mode << "_p" << d.prec_a << d.prec_b << d.prec_c << "_t" << d.trans_a << d.trans_b << "_"
+      << d.M << "x" << d.N << "x" << d.K << "_g" << d.general << "_b" << (d.batch ? d.batch : 1);
+    return s.str();
+}
+
+// the kernel name a variant gets in the journal (the reference names kernels
+// "evergreenmatmul_<params>", EvergreenMatmulBase.cpp:113-132)
+std::string kernel_name(const chai_gemm_desc& d, int mode, int v)
+{
+    return "mi355xgemm_" + exo_key(d, mode) + "_v" + std::to_string(v);
+}
+
+int mode_of(const chai_gemm_desc& d)
+{
+    if (d.mode != CHAI_GEMM_AUTO) return d.mode;
+    return (d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64) ? CHAI_GEMM_F64 : CHAI_GEMM_BF16X3;
+}
+
+}  // namespace
+
+int tuned_variant(const chai_gemm_desc& d, int mode)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_best.find(exo_key(d, mode));
+    return it == g_best.end() ? -1 : it->second;
+}
+
+}  // namespace chai_hip
+
+using namespace chai_hip;
+
+extern "C" {
+
+int chai_hip_memo_open(const char* journal_path, int device)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_memo.device = device;
+    if (!journal_path) { set_error("memo_open: null path"); return CHAI_ERR_ARG; }
+    g_memo.load(journal_path);                 // a missing file is an empty journal
+    g_memo.journal_files.insert(journal_path);
+    return CHAI_OK;
+}
+
+int chai_hip_memo_close(void)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_memo = Memo();
+    g_best.clear();
+    return CHAI_OK;
+}
+
+int chai_hip_memo_save(const char* path)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    std::ofstream o(path);
+    if (!o) { set_error("memo_save: cannot write %s", path); return CHAI_ERR_IO; }
+    for (const auto& kv : g_memo.state) o << g_memo.device << " " << kv.first << " " << kv.second << "\n";
+    for (const auto& kv : g_memo.bench)
+        for (long us : kv.second) o << g_memo.device << " " << kv.first << " " << us << "\n";
+    return CHAI_OK;
+}
+
+/* 1 = good (has times, not blacklisted), 0 = unknown, -1 = bad; *us = first time */
+int chai_hip_memo_kernel_status(const char* kernel, int* status, long* us)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    const std::string k(kernel ? kernel : "");
+    int s = g_memo.bad.count(k) ? -1 : g_memo.good(k) ? 1 : 0;
+    if (status) *status = s;
+    if (us) *us = (s == 1) ? g_memo.bench[k][0] : -1;
+    return CHAI_OK;
+}
+
+int chai_hip_memo_record(const char* kernel, long value)
+{
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!kernel) { set_error("memo_record: null kernel"); return CHAI_ERR_ARG; }
+    return g_memo.add(kernel, value) ? CHAI_OK : CHAI_ERR_IO;
+}
+
+int chai_hip_gemm_set_variant(const chai_gemm_desc* d, int variant)
+{
+    if (!d) { set_error("gemm_set_variant: null descriptor"); return CHAI_ERR_ARG; }
+    const int mode = mode_of(*d);
+    if (variant < 0 || variant >= gemm_num_variants(mode)) { set_error("gemm_set_variant: bad variant"); return CHAI_ERR_ARG; }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_best[exo_key(*d, mode)] = variant;
+    return CHAI_OK;
+}
+
+int chai_hip_gemm_time_variant(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
+                               int variant, int trials, float* core_ms, void* stream)
+{
+    if (!desc || !core_ms || trials < 1) { set_error("gemm_time_variant: bad arguments"); return CHAI_ERR_ARG; }
+    chai_gemm_desc d = *desc;
+    const int mode = mode_of(d);
+    d.mode = mode;
+    d.variant = variant;
+    const std::string name = kernel_name(d, mode, variant);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (g_memo.bad.count(name)) { *core_ms = -1.f; return CHAI_OK; }
+        if (g_memo.good(name)) {                 // memo hit (AutoTuneBench.cpp:27-40)
+            *core_ms = (float)(g_memo.bench[name][0] / 1000.0);
+            return CHAI_OK;
+        }
+        g_memo.add(name, PRINT_SOURCE);
+        g_memo.add(name, COMPILE_KERNEL);       // kernels are precompiled; the state keeps the journal shape
+        g_memo.add(name, SET_KERNEL_ARGS);
+    }
+    int rc = chai_hip_profile_enable(1);
+    // warm-up launch, then timed trials
+    rc = rc ? rc : chai_hip_gemm(&d, A, B, C, stream);
+    { std::lock_guard<std::mutex> lk(g_mu); g_memo.add(name, START_TIMER); }
+    double best = 1e30;
+    for (int t = 0; t < trials && rc == CHAI_OK; t++) {
+        rc = chai_hip_gemm(&d, A, B, C, stream);
+        float prep = 0.f, core = 0.f;
+        if (rc == CHAI_OK) rc = chai_hip_gemm_last_timing(&prep, &core);
+        if (rc == CHAI_OK && core < best) best = core;
+    }
+    chai_hip_profile_enable(0);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (rc != CHAI_OK) {
+        g_memo.add(name, BAD_OUTPUT);
+        *core_ms = -1.f;
+        return rc;
+    }
+    g_memo.add(name, STOP_TIMER);
+    g_memo.add(name, GOOD_OUTPUT);
+    g_memo.add(name, (long)std::llround(best * 1000.0));
+    *core_ms = (float)best;
+    return CHAI_OK;
+}
+
+int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B, void* C, int trials,
+                       int* best_variant, float* best_ms, void* stream)
+{
+    if (!desc) { set_error("gemm_tune: null descriptor"); return CHAI_ERR_ARG; }
+    const int mode = mode_of(*desc);
+    const int nv = gemm_num_variants(mode);
+    int bv = -1;
+    float bt = 1e30f;
+    for (int v = 0; v < nv; v++) {
+        float ms = -1.f;
+        const int rc = chai_hip_gemm_time_variant(desc, A, B, C, v, trials, &ms, stream);
+        if (rc != CHAI_OK && rc != CHAI_ERR_ARG) return rc;
+        if (ms > 0.f && ms < bt) { bt = ms; bv = v; }
+    }
+    if (bv < 0) { set_error("gemm_tune: no variant ran"); return CHAI_ERR_STATE; }
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        g_best[exo_key(*desc, mode)] = bv;
+    }
+    if (best_variant) *best_variant = bv;
+    if (best_ms) *best_ms = bt;
+    return CHAI_OK;
+}
+
+}  // extern "C"

@@ -135,6 +135,24 @@ def matmul(a, b): return Expr(120, [a, b])
 def matmul_g(a, b, c, alpha=1.0, beta=1.0): return Expr(121, [a, b, c], s0=alpha, s1=beta)
 
 
+def set_jit(on):
+    """Fused kernels: True = hipRTC-compiled straight-line HIP, False = trace VM kernel."""
+    L = lib()
+    L.chai_hip_trace_set_jit.argtypes = [ctypes.c_int]
+    check(L.chai_hip_trace_set_jit(int(on)), "trace_set_jit")
+
+
+def emit_source(trace, kernel=0):
+    """JIT source of a trace's fused kernel, produced by the front end alone (no GPU)."""
+    c = trace.to_c()
+    L = lib()
+    L.chai_hip_trace_emit_source.argtypes = [ctypes.POINTER(TraceC), ctypes.c_int, ctypes.c_char_p,
+                                             ctypes.c_size_t]
+    buf = ctypes.create_string_buffer(1 << 20)
+    check(L.chai_hip_trace_emit_source(ctypes.byref(c), kernel, buf, len(buf)), "trace_emit_source")
+    return buf.value.decode()
+
+
 # --------------------------------------------------------------------- trace
 class Trace:
     def __init__(self):
@@ -225,6 +243,16 @@ class Program:
                                               ctypes.POINTER(ctypes.c_int)]
         check(lib().chai_hip_trace_info(self.h, ctypes.byref(k), ctypes.byref(n)), "trace_info")
         return k.value, n.value
+
+    def source(self, kernel=0):
+        """(generated JIT source or JIT log, jitted?) of fused kernel `kernel`."""
+        L = lib()
+        L.chai_hip_trace_kernel_source.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p,
+                                                   ctypes.c_size_t, ctypes.POINTER(ctypes.c_int)]
+        buf = ctypes.create_string_buffer(1 << 20)
+        j = ctypes.c_int()
+        check(L.chai_hip_trace_kernel_source(self.h, kernel, buf, len(buf), ctypes.byref(j)), "trace_kernel_source")
+        return buf.value.decode(errors="replace"), bool(j.value)
 
     def run(self, buffers):
         """buffers: {Var or index: cuda tensor} for every INPUT and OUTPUT variable."""

@@ -144,6 +144,34 @@ int chai_hip_gemm_variant_params(int mode, int variant, int* params);
  * kernels (enqueue/EnqueueMatmul.cpp:24-161, autotune/EvergreenMatmulMM.cpp:1077-1630). */
 int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C, void* stream);
 
+/* ------------------------------------------------------------- autotuning */
+/* Journal memo (autotune/AutoTuneMemo.cpp:16-318): lines "<device> <kernel>
+ * <state|microseconds>" appended as each step is entered; on open, kernels
+ * left in COMPILE_KERNEL / START_TIMER / BEFORE_CHECK are blacklisted
+ * (they crashed or hung).  The file name convention of the reference is
+ * journal<DEVICE>.<VENDOR> (vendor/OCLhacks.cpp:483-491). */
+int chai_hip_memo_open(const char* journal_path, int device);
+int chai_hip_memo_close(void);
+int chai_hip_memo_save(const char* path);
+/* status: 1 good (timed), 0 unknown, -1 blacklisted; us = first bench time */
+int chai_hip_memo_kernel_status(const char* kernel, int* status, long* us);
+/* append one journal value (state code < 0, or a time in microseconds) */
+int chai_hip_memo_record(const char* kernel, long value);
+/* Time one tile variant (AutoTuneBench::runTime, autotune/AutoTuneBench.cpp:
+ * 17-251): memo hit returns the journaled time; otherwise one warm-up and
+ * `trials` timed runs of the core kernel (hipEvents), best time journaled.
+ * core_ms = -1 for a blacklisted variant. */
+int chai_hip_gemm_time_variant(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
+                               int variant, int trials, float* core_ms, void* stream);
+/* Exhaustive search over the family's variants (StandardEM::searchLoop /
+ * AutoTuneEM, autotune/StandardEM.cpp:69-121); the winner is recorded for
+ * the shape's exogenous key (enqueue/JustInTimeMemo.cpp:92-114) and used by
+ * chai_hip_gemm whenever desc->variant < 0. */
+int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B, void* C, int trials,
+                       int* best_variant, float* best_ms, void* stream);
+/* record a variant for the shape's exogenous key (e.g. a model prediction) */
+int chai_hip_gemm_set_variant(const chai_gemm_desc* desc, int variant);
+
 /* ----------------------------------------------------------------- traces */
 /* A trace is what one VectorTrace hands a device (common/VectorTrace.cpp):
  * variables (arrays) and statements "var = expression", each expression a
@@ -201,6 +229,15 @@ int chai_hip_trace_launch(void* handle, void* const* buffers, void* stream);
 /* number of kernels the trace compiled into, and of VM instructions */
 int chai_hip_trace_info(void* handle, int* num_kernels, int* num_instructions);
 int chai_hip_trace_release(void* handle);
+/* Fused kernels run either on the pre-compiled trace VM kernel or as
+ * straight-line HIP generated from the same VM program and compiled at run
+ * time with hipRTC (default; env CHAI_TRACE_JIT=0 or set_jit(0) selects the
+ * VM).  A failed JIT compile falls back to the VM. */
+int chai_hip_trace_set_jit(int on);
+/* Generated source (or the compile log when the JIT failed) of kernel k. */
+int chai_hip_trace_kernel_source(void* handle, int kernel, char* buf, size_t len, int* jitted);
+/* Front end only (no device): the JIT source kernel k of a trace would get. */
+int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, char* buf, size_t len);
 
 /* ------------------------------------------------------------------- GEMV */
 /* y = op(A) x or y = alpha op(A) x + beta y (Evergreen MV,

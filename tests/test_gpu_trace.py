@@ -19,6 +19,15 @@ from tests.trace_cases import (MSGS, md5_expected, md5_hex, md5_inputs, md5_trac
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(params=["jit", "vm"])
+def engine(request, chai):
+    """every fused kernel is checked on both engines: hipRTC JIT and trace VM"""
+    from chai_amd.trace import set_jit
+    set_jit(request.param == "jit")
+    yield request.param
+    set_jit(True)
+
 TDT = {U32: torch.int32, I32: torch.int32, F32: torch.float32, F64: torch.float64}
 NDT = {U32: np.uint32, I32: np.int32, F32: np.float32, F64: np.float64}
 
@@ -43,17 +52,18 @@ def run_gpu(t, inputs):
     return out, prog
 
 
-def test_md5_known_messages(chai, oracle):
+def test_md5_known_messages(chai, oracle, engine):
     t, M, outs = md5_trace(len(MSGS))
     ins = md5_inputs(MSGS)
     r, prog = run_gpu(t, {m.index: ins[k] for k, m in enumerate(M)})
     assert md5_hex([r[o.index].ravel() for o in outs]) == md5_expected(MSGS)
     nk, ni = prog.info()
     assert nk == 1, "the whole MD5 trace fuses into one kernel"
+    assert prog.source(0)[1] == (engine == "jit"), prog.source(0)[0][-2000:]
     print("md5 trace: kernels", nk, "VM instructions", ni)
 
 
-def test_md5_many_messages(chai, oracle):
+def test_md5_many_messages(chai, oracle, engine):
     rng = np.random.default_rng(0)
     msgs = [bytes(rng.integers(0, 256, size=int(rng.integers(0, 56)), dtype=np.uint8)) for _ in range(3001)]
     t, M, outs = md5_trace(len(msgs))
@@ -63,7 +73,7 @@ def test_md5_many_messages(chai, oracle):
 
 
 @pytest.mark.parametrize("seed", range(24))
-def test_random_int_traces_bit_exact(chai, oracle, seed):
+def test_random_int_traces_bit_exact(chai, oracle, seed, engine):
     t, data, outs, red = random_trace(seed, "int")
     print("seed", seed, flush=True)
     o = oracle.trace_eval(t, data)
@@ -73,7 +83,7 @@ def test_random_int_traces_bit_exact(chai, oracle, seed):
 
 
 @pytest.mark.parametrize("seed", range(16))
-def test_random_fp_traces(chai, oracle, seed):
+def test_random_fp_traces(chai, oracle, seed, engine):
     t, data, outs, red = random_trace(seed, "fp")
     o = oracle.trace_eval(t, data)
     g, _ = run_gpu(t, data)
@@ -84,7 +94,7 @@ def test_random_fp_traces(chai, oracle, seed):
     assert np.all(np.abs(g[red.index].ravel() - o[red.index].ravel()) <= tol)
 
 
-def test_transcendentals_within_ulp(chai, oracle):
+def test_transcendentals_within_ulp(chai, oracle, engine):
     t = Trace()
     x = t.input(F32, 4096)
     outs = [t.assign(t.output(F32, 4096), f(x)) for f in (exp, log, sin, cos, sqrt, tanh)]
@@ -96,7 +106,7 @@ def test_transcendentals_within_ulp(chai, oracle):
         assert np.all(np.abs(a - b) <= 4 * 2.0 ** -23 * np.abs(b) + 1e-30), v.index
 
 
-def test_mc_trace_full_size(chai, oracle):
+def test_mc_trace_full_size(chai, oracle, engine):
     """2^24 Philox4x32-10 normals -> S = 100 exp(0.03 + 0.2 z) -> max(S-105,0) -> mean, one kernel."""
     n = 1 << 24
     t, m = mc_trace(n)
@@ -143,7 +153,7 @@ def test_gemv_in_trace(chai, oracle):
     assert np.all(np.abs(g[y.index].ravel() - o[y.index].ravel()) <= tol)
 
 
-def test_slots_and_broadcast(chai, oracle):
+def test_slots_and_broadcast(chai, oracle, engine):
     t = Trace()
     a = t.input(F32, 33, 7, 3)
     row = t.input(F32, 33, 1, 1)
