@@ -155,9 +155,20 @@ def gemm_variants(mode=GEMM_BF16X3):
     return out
 
 
+class Epilogue(ctypes.Structure):
+    """chai_epilogue: kind 1 = max(c * scale + shift, lo) in f32."""
+    _fields_ = [("kind", ctypes.c_int), ("scale", ctypes.c_float), ("shift", ctypes.c_float),
+                ("lo", ctypes.c_float)]
+
+
+def epilogue(scale, shift, lo):
+    return Epilogue(1, scale, shift, lo)
+
+
 def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, beta=0.0,
-         general=False, out_dtype=None, prec_a=None, prec_b=None, variant=-1):
-    """C = op(A) op(B) (or alpha op(A) op(B) + beta C) on 2-D / 3-D (batched) tensors."""
+         general=False, out_dtype=None, prec_a=None, prec_b=None, variant=-1, epi=None):
+    """C = op(A) op(B) (or alpha op(A) op(B) + beta C) on 2-D / 3-D (batched) tensors;
+    `epi` (an Epilogue) is fused into the store."""
     import torch
     init()
     batched = A.dim() == 3 or B.dim() == 3
@@ -189,7 +200,14 @@ def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, 
     d.stride_c = C.stride(0) if C.dim() == 3 else 0
     d.mode = mode
     d.variant = variant
-    check(lib().chai_hip_gemm(ctypes.byref(d), _ptr(A), _ptr(B), _ptr(C), _stream()), "chai_hip_gemm")
+    if epi is None:
+        check(lib().chai_hip_gemm(ctypes.byref(d), _ptr(A), _ptr(B), _ptr(C), _stream()), "chai_hip_gemm")
+    else:
+        L = lib()
+        L.chai_hip_gemm_epilogue.argtypes = [ctypes.POINTER(GemmDesc)] + [ctypes.c_void_p] * 3 + [
+            ctypes.POINTER(Epilogue), ctypes.c_void_p]
+        check(L.chai_hip_gemm_epilogue(ctypes.byref(d), _ptr(A), _ptr(B), _ptr(C), ctypes.byref(epi),
+                                       _stream()), "chai_hip_gemm_epilogue")
     return C
 
 

@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "gemm_device.hpp"
 #include "gemm_internal.hpp"
+#include "gemm_split.hpp"
 #include "gemm_x3.hpp"
 
 #include <type_traits>
@@ -28,12 +29,6 @@ namespace chai_hip {
 // split / pack pre-pass
 // =====================================================================
 
-template <typename T>
-__device__ __forceinline__ float to_f32(T v) { return (float)v; }
-
-// op(X) element (r, k) lives at src[r * ld + k] (TRANS = false) or
-// src[k * ld + r] (TRANS = true).  Output panels hi/lo are [Rp][Kp] bf16.
-// 64 x 64 tile per 256-thread block; the transposed case goes through LDS.
 template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, uint64_t ld,
                                                     uint64_t R, uint64_t K, uint64_t Kp,
@@ -42,49 +37,9 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, u
                                                     uint16_t* __restrict__ lo)
 {
     __shared__ float tile[64][65];
-    const uint64_t r0 = (uint64_t)blockIdx.y * 64, k0 = (uint64_t)blockIdx.x * 64;
-    src += (uint64_t)blockIdx.z * src_bstride;
-    hi += (uint64_t)blockIdx.z * dst_bstride;
-    lo += (uint64_t)blockIdx.z * dst_bstride;
-    const int t = threadIdx.x;
-    if (TRANS) {
-        // coalesced along r: 64 lanes read 64 consecutive r at one k
-        for (int i = 0; i < 16; i++) {
-            const int kk = i * 4 + (t >> 6), rr = t & 63;
-            const uint64_t r = r0 + rr, k = k0 + kk;
-            tile[kk][rr] = (r < R && k < K) ? to_f32(src[k * ld + r]) : 0.0f;
-        }
-    } else {
-        for (int i = 0; i < 16; i++) {
-            const int rr = i * 4 + (t >> 6), kk = t & 63;
-            const uint64_t r = r0 + rr, k = k0 + kk;
-            tile[kk][rr] = (r < R && k < K) ? to_f32(src[r * ld + k]) : 0.0f;
-        }
-    }
-    __syncthreads();
-    // each thread writes 16 consecutive k of one row: 2 x 16 bytes per panel
-    const int rr = t >> 2, kq = (t & 3) * 16;
-    const uint64_t r = r0 + rr;
-    uint16_t h[16], l[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) split_bf16(tile[kq + j][rr], h[j], l[j]);
-    if (k0 + kq < Kp) {
-        uint4* dh = reinterpret_cast<uint4*>(hi + r * Kp + k0 + kq);
-        uint4* dl = reinterpret_cast<uint4*>(lo + r * Kp + k0 + kq);
-        uint4 vh0, vh1, vl0, vl1;
-        vh0.x = h[0] | ((uint32_t)h[1] << 16);   vh0.y = h[2] | ((uint32_t)h[3] << 16);
-        vh0.z = h[4] | ((uint32_t)h[5] << 16);   vh0.w = h[6] | ((uint32_t)h[7] << 16);
-        vh1.x = h[8] | ((uint32_t)h[9] << 16);   vh1.y = h[10] | ((uint32_t)h[11] << 16);
-        vh1.z = h[12] | ((uint32_t)h[13] << 16); vh1.w = h[14] | ((uint32_t)h[15] << 16);
-        vl0.x = l[0] | ((uint32_t)l[1] << 16);   vl0.y = l[2] | ((uint32_t)l[3] << 16);
-        vl0.z = l[4] | ((uint32_t)l[5] << 16);   vl0.w = l[6] | ((uint32_t)l[7] << 16);
-        vl1.x = l[8] | ((uint32_t)l[9] << 16);   vl1.y = l[10] | ((uint32_t)l[11] << 16);
-        vl1.z = l[12] | ((uint32_t)l[13] << 16); vl1.w = l[14] | ((uint32_t)l[15] << 16);
-        dh[0] = vh0;
-        if (k0 + kq + 8 < Kp) dh[1] = vh1;
-        dl[0] = vl0;
-        if (k0 + kq + 8 < Kp) dl[1] = vl1;
-    }
+    split_tile<T, TRANS>(src + (uint64_t)blockIdx.z * src_bstride, ld, R, K, Kp,
+                         hi + (uint64_t)blockIdx.z * dst_bstride, lo + (uint64_t)blockIdx.z * dst_bstride,
+                         (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64, tile);
 }
 
 // Pack op(X) into a dense zero-padded [Rp][Cp] array of TO (element (r, c) of
@@ -130,19 +85,10 @@ struct X3Variant {
 
 #define CHAI_X3V(BM, BN, BK, WGM, WGN, MINB)                                              \
     {BM, BN, BK, WGM, WGN, X3Shape<BM, BN, BK, WGM, WGN>::LDS_BYTES, WGM * WGN * 64,        \
-     gemm_x3_kernel<BM, BN, BK, WGM, WGN, MINB>}
+     gemm_x3_kernel<BM, BN, BK, WGM, WGN, MINB>},
 
-// The autotuner's search space for CHAI_GEMM_BF16X3 (index = variant id).
-static const X3Variant kX3Variants[] = {
-    CHAI_X3V(256, 256, 32, 2, 4, 1),   // 0: 8 waves of 128x64
-    CHAI_X3V(256, 256, 32, 2, 2, 1),   // 1: 4 waves of 128x128
-    CHAI_X3V(128, 128, 32, 2, 2, 2),   // 2: 4 waves of 64x64, 2 blocks / CU
-    CHAI_X3V(256, 128, 32, 2, 2, 1),   // 3: 4 waves of 128x64
-    CHAI_X3V(128, 256, 32, 2, 2, 1),   // 4: 4 waves of 64x128
-    CHAI_X3V(128, 128, 64, 2, 2, 1),   // 5: BK 64
-    CHAI_X3V(128, 256, 32, 2, 4, 1),   // 6: 8 waves of 64x64
-    CHAI_X3V(256, 128, 32, 4, 2, 1),   // 7: 8 waves of 64x64
-};
+// index = variant id
+static const X3Variant kX3Variants[] = {CHAI_X3_VARIANTS(CHAI_X3V)};
 static constexpr int kNumX3 = sizeof(kX3Variants) / sizeof(kX3Variants[0]);
 static int g_x3_default = 0;
 
@@ -397,9 +343,25 @@ static void pack_any(int prec, const void* src, uint64_t ld, bool trans, uint64_
     }
 }
 
-int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
-                int variant, hipStream_t st)
+static void set_epilogue(GemmArgs& g, const chai_epilogue* epi)
 {
+    if (epi && epi->kind == CHAI_EPI_AFFINE_MAX) {
+        g.epi = 1;
+        g.epi_scale = epi->scale; g.epi_shift = epi->shift; g.epi_lo = epi->lo;
+    }
+}
+
+int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
+                int variant, hipStream_t st, const chai_epilogue* epi)
+{
+    if (epi && epi->kind != CHAI_EPI_NONE && epi->kind != CHAI_EPI_AFFINE_MAX) {
+        set_error("gemm: unknown epilogue kind %d", epi->kind);
+        return CHAI_ERR_ARG;
+    }
+    if (epi && epi->kind != CHAI_EPI_NONE && mode == CHAI_GEMM_F64) {
+        set_error("gemm: epilogues apply to f32 results only");
+        return CHAI_ERR_ARG;
+    }
     const uint64_t M = d.M, N = d.N, K = d.K, nb = d.batch ? d.batch : 1;
     const uint64_t a_rows = d.trans_a ? K : M;   // stored layout of A
     (void)a_rows;
@@ -432,6 +394,7 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         g.M = M; g.N = N;
         g.tiles_m = (uint32_t)(Mp / v.BM); g.tiles_n = (uint32_t)(Np / v.BN);
         g.general = d.general; g.alpha = d.alpha; g.beta = d.beta;
+        set_epilogue(g, epi);
         static bool attr_set[kNumX3] = {};
         if (!attr_set[vid]) {
             CHAI_HIP_TRY(hipFuncSetAttribute((const void*)v.kernel,
@@ -474,6 +437,7 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         g.M = M; g.N = N;
         g.tiles_m = (uint32_t)(Mp / BMx); g.tiles_n = (uint32_t)(Np / BNx);
         g.general = d.general; g.alpha = d.alpha; g.beta = d.beta;
+        set_epilogue(g, epi);
         dim3 grid(g.tiles_m * g.tiles_n, 1, (unsigned)nb);
         (void)BKx;
         prof_mark(1, st);

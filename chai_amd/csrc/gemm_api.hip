@@ -4,8 +4,8 @@
 
 using namespace chai_hip;
 
-extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
-                             void* stream)
+extern "C" int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A, const void* B,
+                                      void* C, const chai_epilogue* epi, void* stream)
 {
     if (!desc) { set_error("gemm: null descriptor"); return CHAI_ERR_ARG; }
     const chai_gemm_desc& d = *desc;
@@ -50,7 +50,13 @@ extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const vo
         return CHAI_ERR_ARG;
     }
     const int variant = d.variant >= 0 ? d.variant : tuned_variant(d, mode);
-    return gemm_launch(d, A, B, C, mode, variant, as_stream(stream));
+    return gemm_launch(d, A, B, C, mode, variant, as_stream(stream), epi);
+}
+
+extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
+                             void* stream)
+{
+    return chai_hip_gemm_epilogue(desc, A, B, C, nullptr, stream);
 }
 
 extern "C" int chai_hip_gemm_num_variants(int mode, int* count)

@@ -22,11 +22,15 @@ struct GemmArgs {
     uint32_t tiles_m, tiles_n;
     int general;
     double alpha, beta;
+    // fused elementwise epilogue on the f32 result (after alpha / beta):
+    // epi == 1: c = max(c * epi_scale + epi_shift, epi_lo), each op rounded in f32
+    int epi;
+    float epi_scale, epi_shift, epi_lo;
 };
 
 // variant < 0 selects the current default (set by the tuner) for the mode.
 int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, int mode,
-                int variant, hipStream_t st);
+                int variant, hipStream_t st, const chai_epilogue* epi = nullptr);
 int gemm_num_variants(int mode);
 int gemm_variant_params(int mode, int vid, int* params);   // BM, BN, BK, WGM, WGN
 void gemm_set_default_variant(int mode, int vid);

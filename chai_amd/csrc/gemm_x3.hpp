@@ -31,8 +31,9 @@ __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v)
 
 __device__ __forceinline__ void store_c(float* C, uint64_t idx, float v, const GemmArgs& a)
 {
-    if (a.general) C[idx] = (float)(a.alpha * (double)v + a.beta * (double)C[idx]);
-    else C[idx] = v;
+    if (a.general) v = (float)(a.alpha * (double)v + a.beta * (double)C[idx]);
+    if (a.epi) v = fmaxf(v * a.epi_scale + a.epi_shift, a.epi_lo);   // no contraction (-ffp-contract=off)
+    C[idx] = v;
 }
 
 template <int BM, int BN, int BK, int WGM, int WGN>
@@ -53,14 +54,13 @@ struct X3Shape {
     static_assert(PA % THREADS == 0 && PB % THREADS == 0, "panel chunks must split evenly");
 };
 
-template <int BM, int BN, int BK, int WGM, int WGN, int MINB>
-__global__ __launch_bounds__(WGM * WGN * 64, MINB) void gemm_x3_kernel(GemmArgs a)
+// One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is
+// free on entry and on exit (the last k step ends with a barrier), so a
+// persistent kernel may call it for tile after tile.
+template <int BM, int BN, int BK, int WGM, int WGN>
+__device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, uint64_t bz, u32x4* lds)
 {
     using S = X3Shape<BM, BN, BK, WGM, WGN>;
-    extern __shared__ u32x4 lds[];
-    uint32_t tm, tn;
-    tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
-    const uint64_t bz = blockIdx.z;
     const uint64_t Kp = a.Kp;
     float* C = a.C + bz * a.c_bstride;
 
@@ -196,6 +196,27 @@ __global__ __launch_bounds__(WGM * WGN * 64, MINB) void gemm_x3_kernel(GemmArgs 
                 if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
             }
         }
+}
+
+// The autotuner's search space for the split-BF16x3 family, in variant-id
+// order: X(BM, BN, BK, waves_m, waves_n, min blocks per CU).
+#define CHAI_X3_VARIANTS(X)                                                   \
+    X(256, 256, 32, 2, 4, 1)   /* 0: 8 waves of 128x64               */      \
+    X(256, 256, 32, 2, 2, 1)   /* 1: 4 waves of 128x128              */      \
+    X(128, 128, 32, 2, 2, 2)   /* 2: 4 waves of 64x64, 2 blocks / CU */      \
+    X(256, 128, 32, 2, 2, 1)   /* 3: 4 waves of 128x64               */      \
+    X(128, 256, 32, 2, 2, 1)   /* 4: 4 waves of 64x128               */      \
+    X(128, 128, 64, 2, 2, 1)   /* 5: BK 64                           */      \
+    X(128, 256, 32, 2, 4, 1)   /* 6: 8 waves of 64x64                */      \
+    X(256, 128, 32, 4, 2, 1)   /* 7: 8 waves of 64x64                */
+
+template <int BM, int BN, int BK, int WGM, int WGN, int MINB>
+__global__ __launch_bounds__(WGM * WGN * 64, MINB) void gemm_x3_kernel(GemmArgs a)
+{
+    extern __shared__ u32x4 lds[];
+    uint32_t tm, tn;
+    tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
+    x3_tile<BM, BN, BK, WGM, WGN>(a, tm, tn, blockIdx.z, lds);
 }
 
 }  // namespace chai_hip

@@ -59,6 +59,12 @@ __global__ __launch_bounds__(VM_THREADS) void trace_vm_kernel(VmLaunchArgs L)
         switch (k) { VM_RNG_CASE(0) VM_RNG_CASE(1) VM_RNG_CASE(2) VM_RNG_CASE(3) VM_RNG_CASE(4)
                      VM_RNG_CASE(5) VM_RNG_CASE(6) default: return L.rng[7]; }
     };
+#define VM_CHAIN_CASE(n) case n + 1: return L.chains[n];
+    auto chain_at = [&](uint32_t k) -> VmChain {   // k = chain index + 1; 0 = none
+        switch (k) { case 0: return VmChain{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0, 0, 0, 0};
+                     VM_CHAIN_CASE(0) VM_CHAIN_CASE(1) VM_CHAIN_CASE(2) VM_CHAIN_CASE(3)
+                     VM_CHAIN_CASE(4) VM_CHAIN_CASE(5) VM_CHAIN_CASE(6) default: return L.chains[7]; }
+    };
     const uint64_t stride = (uint64_t)gridDim.x * VM_THREADS;
     for (uint64_t v = (uint64_t)blockIdx.x * VM_THREADS + threadIdx.x; v < L.vps; v += stride) {
         const uint64_t y = v / L.vpr;
@@ -71,17 +77,23 @@ __global__ __launch_bounds__(VM_THREADS) void trace_vm_kernel(VmLaunchArgs L)
             switch (I.s1) { VM_SLOTS(VM_RD_CASE) }
             a = v;
             switch (I.op) {
-                case VM_LOAD: v = vm_load(var_at(I.a), slot, y, x0, nval, L.W, L.H); break;
+                case VM_LOAD:
+                    if (I.p2) v = vm_load_c(var_at(I.a), chain_at(I.p2), slot, y, x0, nval);
+                    else v = vm_load(var_at(I.a), slot, y, x0, nval, L.W, L.H);
+                    break;
                 case VM_IMM: v.l[0] = v.l[1] = v.l[2] = v.l[3] = I.u; break;
                 case VM_RNG: {
                     const RngArgs ra = {I.a, I.b, I.c, I.p, I.u, I.d0, I.d1};
-                    v = vm_rng(ra, rng_at(I.e), slot, y, x0, nval);
+                    v = vm_rng(ra, rng_at(I.e), slot, y, x0, nval, chain_at(I.p2));
                     break;
                 }
-                case VM_INDEX:
+                case VM_INDEX: {
+                    const VmChain C = chain_at(I.p2);
                     for (int i = 0; i < 4; i++)
-                        v.l[i] = from_double(I.p, (double)(I.a == 0 ? x0 + i : y));
+                        v.l[i] = from_double(I.p, (double)(I.a == 0 ? chain_mod(x0 + i, C.W, C.nW)
+                                                                    : chain_mod(y, C.H, C.nH)));
                     break;
+                }
                 case VM_CVT:
                     for (int i = 0; i < 4; i++) v.l[i] = from_double(I.p, as_double(I.p2, a.l[i]));
                     break;
@@ -228,6 +240,7 @@ struct Group {
     std::vector<VmIns> prog;
     std::vector<uint32_t> var_of_slot;   // VmVar table: trace variable per entry
     std::vector<VmRngDims> rng;
+    std::vector<VmChain> chains;         // broadcast chains of leaf instructions
     std::vector<uint32_t> red_vars;      // output variable of each accumulator
     std::vector<double> red_div;
     VmIns* dprog = nullptr;
@@ -407,6 +420,41 @@ struct Emitter {
     std::map<uint32_t, uint32_t> vtab;    // trace variable -> VmVar table index
     bool temp_used[VM_NSLOTS] = {};      // register slots holding live temporaries
     int max_stack = 0;
+    std::vector<Dims> path;               // dims of the operators above the node being emitted
+
+    // Broadcast chain of a leaf with dims `leaf` under the operators in `path`
+    // (see VmChain); the leaf's own modulus is part of the chain only when
+    // include_leaf (index leaves; loads and rng apply it themselves).
+    // Returns the chain index + 1, 0 for none, -1 on overflow.
+    int chain_for(const Dims& leaf, bool include_leaf)
+    {
+        VmChain c;
+        std::memset(&c, 0, sizeof c);
+        auto one = [&](uint64_t Dims::*f, uint64_t space, uint64_t* out, uint32_t& n) -> bool {
+            std::vector<uint64_t> st;
+            auto push = [&](uint64_t v) {
+                if (v >= space) return;                     // coordinates are already below it
+                while (!st.empty() && st.back() % v == 0) st.pop_back();
+                st.push_back(v);
+            };
+            for (const Dims& a : path) push(a.*f);
+            push(leaf.*f);
+            if (!include_leaf && !st.empty() && st.back() == leaf.*f) st.pop_back();
+            if (st.size() > 3) return false;
+            n = (uint32_t)st.size();
+            for (size_t i = 0; i < st.size(); i++) out[i] = st[i];
+            return true;
+        };
+        if (!one(&Dims::W, G.space.W, c.W, c.nW) || !one(&Dims::H, G.space.H, c.H, c.nH) ||
+            !one(&Dims::S, G.space.S, c.S, c.nS))
+            return -1;
+        if ((c.nW | c.nH | c.nS) == 0) return 0;
+        for (size_t i = 0; i < G.chains.size(); i++)
+            if (std::memcmp(&G.chains[i], &c, sizeof c) == 0) return (int)i + 1;
+        if (G.chains.size() >= 8) return -1;
+        G.chains.push_back(c);
+        return (int)G.chains.size();
+    }
 
     int min_temp() const
     {
@@ -449,10 +497,19 @@ struct Emitter {
         if (tgt >= min_temp()) { err = fail("expression too deep for the VM registers"); return 0; }
         const ENode& e = s.nodes[n];
         const chai_trace_node& src = *e.src;
+        int chain = 0;
+        if (e.code == CHAI_TRACE_VAR || is_rng(e.code) || is_index(e.code)) {
+            chain = chain_for(e.d, is_index(e.code));
+            if (chain < 0) { err = fail("broadcast nesting too deep for the VM"); return 0; }
+        }
         if (e.code == CHAI_TRACE_VAR) {
             auto it = reg_of_var.find(src.arg);
-            if (it != reg_of_var.end()) return it->second;
+            if (it != reg_of_var.end()) {
+                if (chain) { err = fail("register-resident variable under a broadcast"); return 0; }
+                return it->second;
+            }
             VmIns I = ins(VM_LOAD, e.prec, tgt);
+            I.p2 = (uint8_t)chain;
             I.a = var_index(src.arg);
             G.prog.push_back(I);
             return tgt;
@@ -470,6 +527,7 @@ struct Emitter {
         }
         if (is_rng(e.code)) {
             VmIns I = ins(VM_RNG, e.prec, tgt);
+            I.p2 = (uint8_t)chain;
             I.a = src.arg;
             const int svl = e.prec == CHAI_F64 ? 2 : 4;
             const int vl = src.vec_len ? (int)src.vec_len : svl;
@@ -489,14 +547,19 @@ struct Emitter {
             R.W = e.d.W; R.H = e.d.H; R.S = e.d.S;
             R.vpr = pad_width(e.prec, e.d.W) / (uint64_t)vl;
             R.same = (e.d == G.space) ? 1u : 0u;
-            if (G.rng.size() >= 8) { err = fail("more than 8 rng calls in one kernel"); return 0; }
-            I.e = (uint32_t)G.rng.size();
-            G.rng.push_back(R);
+            size_t ri = 0;
+            while (ri < G.rng.size() && std::memcmp(&G.rng[ri], &R, sizeof R) != 0) ri++;
+            if (ri == G.rng.size()) {
+                if (G.rng.size() >= 8) { err = fail("more than 8 rng shapes in one kernel"); return 0; }
+                G.rng.push_back(R);
+            }
+            I.e = (uint32_t)ri;
             G.prog.push_back(I);
             return tgt;
         }
         if (is_index(e.code)) {
             VmIns I = ins(VM_INDEX, e.prec, tgt);
+            I.p2 = (uint8_t)chain;
             I.a = src.arg ? 1 : 0;
             G.prog.push_back(I);
             return tgt;
@@ -508,6 +571,7 @@ struct Emitter {
         if (e.code == 23) want[0] = s.nodes[e.kids[0]].prec;   // condition keeps its type
         int slots[3] = {0, 0, 0};
         int next = tgt;
+        path.push_back(e.d);
         for (size_t k = 0; k < e.kids.size(); k++) {
             int sl = emit(s, e.kids[k], next, err);
             if (err) return 0;
@@ -522,6 +586,7 @@ struct Emitter {
             slots[k] = sl;
             if (sl == next) next++;
         }
+        path.pop_back();
         VmIns I = ins(VM_MOV, e.prec, tgt);
         I.s1 = (uint8_t)slots[0];
         I.s2 = (uint8_t)slots[1];
@@ -795,8 +860,12 @@ static std::string jit_source(const TraceProgram& P, const Group& G)
     const uint64_t W = G.space.W, H = G.space.H, S = G.space.S;
     const uint64_t vpr = (W + 3) / 4, vps = vpr * H;
     std::string s;
-    char b[512];
+    char b[1024];
     s += "#include \"trace_vm.hpp\"\n";
+    for (size_t k = 0; k < G.var_of_slot.size(); k++) {
+        snprintf(b, sizeof b, "// L.vars[%zu] = trace variable %u\n", k, G.var_of_slot[k]);
+        s += b;
+    }
     s += "extern \"C\" __global__ __launch_bounds__(256) void chai_trace_kernel(chai_hip::VmLaunchArgs L)\n{\n";
     s += "    using namespace chai_hip;\n    const uint64_t slot = blockIdx.y;\n";
     for (int r = 0; r < VM_NSLOTS; r++) { snprintf(b, sizeof b, "    V4 R%d = {{0, 0, 0, 0}};\n", r); s += b; }
@@ -821,10 +890,27 @@ static std::string jit_source(const TraceProgram& P, const Group& G)
                      tv.prec, same);
             return std::string(t);
         };
+        const auto chain_lit = [&](uint32_t k) {
+            if (k == 0) return std::string("VmChain{{0, 0, 0}, {0, 0, 0}, {0, 0, 0}, 0u, 0u, 0u, 0u}");
+            const VmChain& c = G.chains[k - 1];
+            char t[400];
+            snprintf(t, sizeof t,
+                     "VmChain{{%lluull, %lluull, %lluull}, {%lluull, %lluull, %lluull}, {%lluull, %lluull, %lluull}, "
+                     "%uu, %uu, %uu, 0u}",
+                     (unsigned long long)c.W[0], (unsigned long long)c.W[1], (unsigned long long)c.W[2],
+                     (unsigned long long)c.H[0], (unsigned long long)c.H[1], (unsigned long long)c.H[2],
+                     (unsigned long long)c.S[0], (unsigned long long)c.S[1], (unsigned long long)c.S[2], c.nW, c.nH,
+                     c.nS);
+            return std::string(t);
+        };
         switch (I.op) {
             case VM_LOAD:
-                snprintf(b, sizeof b, "        R%d = vm_load(%s, slot, y, x0, nval, %lluull, %lluull);\n", I.dst,
-                         var_lit(I.a, true).c_str(), (unsigned long long)W, (unsigned long long)H);
+                if (I.p2)
+                    snprintf(b, sizeof b, "        R%d = vm_load_c(%s, %s, slot, y, x0, nval);\n", I.dst,
+                             var_lit(I.a, true).c_str(), chain_lit(I.p2).c_str());
+                else
+                    snprintf(b, sizeof b, "        R%d = vm_load(%s, slot, y, x0, nval, %lluull, %lluull);\n", I.dst,
+                             var_lit(I.a, true).c_str(), (unsigned long long)W, (unsigned long long)H);
                 break;
             case VM_IMM:
                 snprintf(b, sizeof b, "        R%d.l[0] = R%d.l[1] = R%d.l[2] = R%d.l[3] = %lluull;\n", I.dst, I.dst,
@@ -834,16 +920,23 @@ static std::string jit_source(const TraceProgram& P, const Group& G)
                 const VmRngDims& R = G.rng[I.e];
                 snprintf(b, sizeof b,
                          "        R%d = vm_rng(RngArgs{%uu, %uu, %uu, %uu, %lluull, %s, %s}, "
-                         "VmRngDims{%lluull, %lluull, %lluull, %lluull, %uu, 0u}, slot, y, x0, nval);\n",
+                         "VmRngDims{%lluull, %lluull, %lluull, %lluull, %uu, 0u}, slot, y, x0, nval, %s);\n",
                          I.dst, I.a, I.b, I.c, (unsigned)I.p, (unsigned long long)I.u, hexd(I.d0).c_str(),
                          hexd(I.d1).c_str(), (unsigned long long)R.W, (unsigned long long)R.H,
-                         (unsigned long long)R.S, (unsigned long long)R.vpr, R.same);
+                         (unsigned long long)R.S, (unsigned long long)R.vpr, R.same, chain_lit(I.p2).c_str());
                 break;
             }
             case VM_INDEX:
-                snprintf(b, sizeof b,
-                         "        for (int i = 0; i < 4; i++) R%d.l[i] = from_double(%u, (double)(%s));\n",
-                         I.dst, (unsigned)I.p, I.a == 0 ? "x0 + i" : "y");
+                if (I.p2)
+                    snprintf(b, sizeof b,
+                             "        { const VmChain C = %s; for (int i = 0; i < 4; i++) R%d.l[i] = "
+                             "from_double(%u, (double)(%s)); }\n",
+                             chain_lit(I.p2).c_str(), I.dst, (unsigned)I.p,
+                             I.a == 0 ? "chain_mod(x0 + i, C.W, C.nW)" : "chain_mod(y, C.H, C.nH)");
+                else
+                    snprintf(b, sizeof b,
+                             "        for (int i = 0; i < 4; i++) R%d.l[i] = from_double(%u, (double)(%s));\n",
+                             I.dst, (unsigned)I.p, I.a == 0 ? "x0 + i" : "y");
                 break;
             case VM_CVT:
                 snprintf(b, sizeof b,
@@ -976,6 +1069,7 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
             V.same = (V.W == L.W && V.H == L.H && V.S == L.S) ? 1u : 0u;
         }
         for (size_t k = 0; k < G.rng.size(); k++) L.rng[k] = G.rng[k];
+        for (size_t k = 0; k < G.chains.size(); k++) L.chains[k] = G.chains[k];
         uint64_t bps = (L.vps + VM_THREADS - 1) / VM_THREADS;
         const uint64_t cap = L.nacc ? 1024 : 65535;
         if (bps > cap) bps = cap;

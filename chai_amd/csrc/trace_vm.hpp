@@ -5,8 +5,8 @@
 // owns a vector of 4 consecutive row elements (the reference JIT's float4
 // vectors, enqueue/EnqueueIdSpace.cpp:19-320); all lanes of a wavefront run the
 // same instruction, so dispatch is a uniform scalar branch and the instruction
-// stream is read through the scalar cache.  Values live in 16 register slots
-// R0..R15 (4 lanes x 64 bits each) addressed through uniform switches, so no
+// stream is read through the scalar cache.  Values live in 12 register slots
+// R0..R11 (4 lanes x 64 bits each) addressed through uniform switches, so no
 // per-thread array is ever indexed dynamically (nothing goes to scratch).
 //
 // Scalar semantics follow Chai's interpreter (misc/SimpleFun{1,2,3}.cpp):
@@ -60,6 +60,25 @@ struct VmRngDims {
     uint64_t W, H, S, vpr;     // vpr = padded width / vec_len
     uint32_t same, pad;
 };
+
+// Nested broadcast: the interpreter evaluates every operator into an array of
+// its own dims (the max of its operands') and reads operands by modulo
+// (interp/BaseInterp.cpp:50-55), so a leaf under intermediate results smaller
+// than the index space sees the coordinate x % d1 % d2 % ... (one modulus per
+// smaller ancestor, outermost first).  A chain holds those moduli per
+// dimension, already reduced (a modulus divisible by the next one is dropped).
+struct VmChain {
+    uint64_t W[3], H[3], S[3];
+    uint32_t nW, nH, nS, pad;
+};
+
+__device__ __forceinline__ uint64_t chain_mod(uint64_t v, const uint64_t* m, uint32_t n)
+{
+#pragma unroll
+    for (int l = 0; l < 3; l++)
+        if (l < (int)n) v %= m[l];
+    return v;
+}
 
 constexpr int VM_MAX_ACC = 4;
 constexpr int VM_THREADS = 256;
@@ -362,14 +381,15 @@ struct RngArgs {
 };
 
 __device__ __forceinline__ V4 vm_rng(const RngArgs A, const VmRngDims R, uint64_t slot, uint64_t y,
-                                     uint64_t x0, int nval)
+                                     uint64_t x0, int nval, const VmChain C)
 {
     V4 r = {{0, 0, 0, 0}};
+    const bool chained = (C.nW | C.nH | C.nS) != 0;
     const int variant = (int)A.variant, vl = (int)A.vl, normal = A.flags & 1, lim = (A.flags >> 1) & 1;
     const int prec = (int)A.prec;
     struct { uint64_t u; double d0, d1; } I = {A.seed, A.lo, A.hi};
     if (prec != CHAI_F64) {
-        if (R.same && vl == 4) {   // one stream vector covers the thread's 4 lanes
+        if (R.same && vl == 4 && !chained) {   // one stream vector covers the thread's 4 lanes
             uint32_t w[4];
             words32(variant, 4, (uint32_t)(slot * R.H * R.vpr + y * R.vpr + x0 / 4), (uint32_t)I.u, w);
             if (normal && prec == CHAI_F32) {
@@ -384,7 +404,8 @@ __device__ __forceinline__ V4 vm_rng(const RngArgs A, const VmRngDims R, uint64_
             return r;
         }
         for (int i = 0; i < 4; i++) if (i < nval) {
-            const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
+            const uint64_t xr = chain_mod(x0 + i, C.W, C.nW) % R.W;
+            const uint64_t yr = chain_mod(y, C.H, C.nH) % R.H, sr = chain_mod(slot, C.S, C.nS) % R.S;
             uint32_t w[4];
             words32(variant, vl, (uint32_t)(sr * R.H * R.vpr + yr * R.vpr + xr / vl), (uint32_t)I.u, w);
             r.l[i] = rng_elem32(w, (int)(xr % vl), prec, normal, lim, I.d0, I.d1);
@@ -392,11 +413,25 @@ __device__ __forceinline__ V4 vm_rng(const RngArgs A, const VmRngDims R, uint64_
         return r;
     }
     for (int i = 0; i < 4; i++) if (i < nval) {
-        const uint64_t xr = (x0 + i) % R.W, yr = y % R.H, sr = slot % R.S;
+        const uint64_t xr = chain_mod(x0 + i, C.W, C.nW) % R.W;
+        const uint64_t yr = chain_mod(y, C.H, C.nH) % R.H, sr = chain_mod(slot, C.S, C.nS) % R.S;
         uint64_t w[2];
         words64(variant, sr * R.H * R.vpr + yr * R.vpr + xr / vl, I.u, w);
         r.l[i] = rng_elem64(w, (int)(xr % vl), normal, lim, I.d0, I.d1);
     }
+    return r;
+}
+
+// load through a broadcast chain (never the `same` case: a leaf with the dims
+// of the index space has no smaller ancestor)
+__device__ __forceinline__ V4 vm_load_c(const VmVar& V, const VmChain& C, uint64_t slot, uint64_t y,
+                                        uint64_t x0, int nval)
+{
+    V4 r = {{0, 0, 0, 0}};
+    const uint64_t sr = chain_mod(slot, C.S, C.nS) % V.S, yr = chain_mod(y, C.H, C.nH) % V.H;
+    const uint64_t rowoff = (sr * V.H + yr) * V.W;
+    for (int i = 0; i < 4; i++)
+        if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, rowoff + chain_mod(x0 + i, C.W, C.nW) % V.W);
     return r;
 }
 
@@ -519,6 +554,7 @@ struct VmLaunchArgs {
     double* partial;
     VmVar vars[24];
     VmRngDims rng[8];
+    VmChain chains[8];         // leaf instruction field p2 = chain index + 1 (0: none)
 };
 
 

@@ -144,6 +144,77 @@ int chai_hip_gemm_variant_params(int mode, int variant, int* params);
  * kernels (enqueue/EnqueueMatmul.cpp:24-161, autotune/EvergreenMatmulMM.cpp:1077-1630). */
 int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C, void* stream);
 
+/* Elementwise epilogue fused into the store of an f32 GEMM result c (after
+ * alpha / beta).  CHAI_EPI_AFFINE_MAX computes max(c * scale + shift, lo) with
+ * each operation rounded in f32: the statements `D = C * 0.5f + 1;
+ * E = max(D, 0)` of a trace that follows a matmul, which the reference runs
+ * as a second kernel over C (enqueue/EnqueueIdSpace.cpp:19-320). */
+enum { CHAI_EPI_NONE = 0, CHAI_EPI_AFFINE_MAX = 1 };
+typedef struct chai_epilogue {
+    int kind;                       /* CHAI_EPI_*                               */
+    float scale, shift, lo;
+} chai_epilogue;
+
+/* chai_hip_gemm with an epilogue (FMA / BF16X3 families; epi may be NULL). */
+int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
+                           const chai_epilogue* epi, void* stream);
+
+/* --------------------------------------------------------- ragged batches */
+/* The replacement for gathering identical traces from many host threads into
+ * one VectorTrace (runtime/Scheduler.cpp:48-307, common/VectorTrace.cpp:9-100,
+ * enqueue/EnqueueMatmulAuto.cpp:57-68 setBatching): instances of
+ * C = A B + epilogue with DIFFERENT shapes run as one split pre-pass launch and
+ * one persistent split-BF16x3 launch.  The persistent kernel pulls output
+ * tiles from eight per-XCD queues, each filled largest-instance-first
+ * (LPT by M N K) and balanced by work; a block drains its own XCD's queue, then
+ * steals from the others.  f32 operands, row-major, no transposes. */
+typedef struct chai_ragged_item {
+    const float* A;                 /* M x K, row stride lda                    */
+    const float* B;                 /* K x N, row stride ldb                    */
+    float* C;                       /* M x N, row stride ldc                    */
+    size_t M, N, K;
+    size_t lda, ldb, ldc;
+} chai_ragged_item;
+
+/* Build a plan: validates the items, allocates the split workspace and the
+ * tile queues (the plan owns them), picks the tile variant (-1 = the one the
+ * ragged tuner chose last, else variant 0). */
+int chai_hip_ragged_plan(const chai_ragged_item* items, size_t count, const chai_epilogue* epi,
+                         int variant, void** plan);
+/* Run the plan on `stream` (asynchronous).  use_graph = 1 captures the two
+ * launches into a hipGraph on the first call and replays it afterwards. */
+int chai_hip_ragged_run(void* plan, int use_graph, void* stream);
+/* info[0..6]: output tiles, persistent blocks, workspace bytes, useful flops
+ * (sum 2MNK), padded flops (tiles x BM x BN x 2Kp), variant, split blocks. */
+int chai_hip_ragged_info(void* plan, uint64_t* info);
+/* hipEvent durations (ms) of the last run's split pre-pass and persistent
+ * kernel (not available for graph replays: returns CHAI_ERR_STATE). */
+int chai_hip_ragged_last_timing(void* plan, float* split_ms, float* core_ms);
+int chai_hip_ragged_release(void* plan);
+/* Time every tile variant on this batch (one warm-up and `trials` runs each)
+ * and remember the fastest for chai_hip_ragged_plan(..., -1, ...). */
+int chai_hip_ragged_tune(const chai_ragged_item* items, size_t count, const chai_epilogue* epi,
+                         int trials, int* best_variant, float* best_ms, void* stream);
+
+/* Thread-safe batcher: any number of host threads submit items; the thread
+ * whose item fills the batch (`expected` items) runs it on the batcher's own
+ * stream, and every submitter returns when the batches holding its items have
+ * finished (Scheduler::wait semantics, runtime/Scheduler.cpp:307).  When
+ * timeout_ms > 0, a submitter that has waited that long for a batch that is
+ * not full runs it as it is (the reference's boss loop likewise runs what it
+ * has gathered after a wait, runtime/Scheduler.cpp:48-100); timeout_ms = 0
+ * waits for a full batch.  A batch with the same items as the previous one
+ * replays its hipGraph.  variant -1 tunes the tile variant on the first batch. */
+int chai_hip_batcher_create(size_t expected, const chai_epilogue* epi, int variant, int use_graph,
+                            int timeout_ms, void** batcher);
+/* queue `count` items (one host thread's trace) and wait for them */
+int chai_hip_batcher_submit(void* batcher, const chai_ragged_item* items, size_t count);
+/* run whatever is queued now (for a last, partial batch) */
+int chai_hip_batcher_flush(void* batcher);
+/* stats[0..4]: batches run, items run, plans built, graph replays, variant */
+int chai_hip_batcher_stats(void* batcher, uint64_t* stats);
+int chai_hip_batcher_destroy(void* batcher);
+
 /* ------------------------------------------------------------- autotuning */
 /* Journal memo (autotune/AutoTuneMemo.cpp:16-318): lines "<device> <kernel>
  * <state|microseconds>" appended as each step is entered; on open, kernels

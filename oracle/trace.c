@@ -502,7 +502,7 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
         const chai_trace_var* lv = &t->vars[st->lhs];
         Arr v = stk[0];
         const size_t n = lv->W * lv->H * lv->slots;
-        if (v.W * v.H * v.S != n) { free(v.d); rc = -1; break; }
+        if (v.W * v.H * v.S != n) { free(v.d); sp = 0; rc = -1; break; }
         if ((int)lv->prec == v.prec) {
             memcpy(bufs[st->lhs], v.d, esz(v.prec) * n);
         } else {

@@ -14,8 +14,8 @@ import torch
 from chai_amd import F32, F64, I32, U32
 from chai_amd.trace import (Trace, exp, f32, log, maxe, matmul, mean, sin, sqrt, sum_, tanh,
                             u32, cos, Expr)
-from tests.trace_cases import (MSGS, md5_expected, md5_hex, md5_inputs, md5_trace, mc_trace,
-                               random_trace)
+from tests.trace_cases import (MSGS, bcast_trace, md5_expected, md5_hex, md5_inputs, md5_trace,
+                               mc_trace, random_trace)
 
 pytestmark = pytest.mark.gpu
 
@@ -79,6 +79,17 @@ def test_random_int_traces_bit_exact(chai, oracle, seed, engine):
     o = oracle.trace_eval(t, data)
     g, _ = run_gpu(t, data)
     for v in outs + [red]:
+        assert np.array_equal(g[v.index], o[v.index]), v.index
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_nested_broadcast_traces_bit_exact(chai, oracle, seed, engine):
+    """intermediate results smaller than the index space: x % d1 % d2 chains"""
+    t, data, outs = bcast_trace(seed)
+    print("bcast seed", seed, flush=True)
+    o = oracle.trace_eval(t, data)
+    g, _ = run_gpu(t, data)
+    for v in outs:
         assert np.array_equal(g[v.index], o[v.index]), v.index
 
 

@@ -137,3 +137,43 @@ def random_trace(seed, kind="int", n_in=3, W=37, H=5, slots=2, depth=4, n_stmt=3
             pool.append(tv)
     red = t.assign(t.output(F64 if kind == "fp" else U32, 1, 1, slots), sum_(outs[-1]))
     return t, data, outs, red
+
+
+def bcast_trace(seed, W=37, H=5, slots=2, depth=4, n_stmt=3):
+    """Nested modulo broadcast: integer leaves (inputs, rng and index arrays) of
+    many small shapes combined so that intermediate results are themselves
+    smaller than the statement's index space (interp/BaseInterp.cpp:50-55
+    evaluates every operator into an array of its own dims)."""
+    rng = np.random.default_rng(10_000 + seed)
+    t = Trace()
+    widths = [1, 2, 3, 5, 6, 12, 13, 24, W]
+    heights = [1, H]
+    sl = [1, slots]
+    data, leaves = {}, []
+    for k in range(4):
+        p = [U32, I32][rng.integers(2)]
+        w, h, s = int(rng.choice(widths)), int(rng.choice(heights)), int(rng.choice(sl))
+        v = t.input(p, w, h, s)
+        data[v.index] = rng.integers(-1000, 1000, size=(s, h, w), dtype=np.int64).astype(
+            np.int32 if p == I32 else np.uint32)
+        leaves.append(lambda v=v: v)
+    for k in range(3):
+        w, h, s = int(rng.choice(widths)), int(rng.choice(heights)), int(rng.choice(sl))
+        sd = int(rng.integers(1000))
+        leaves.append(lambda w=w, h=h, s=s, sd=sd: Trace.rng_uniform(U32, w, h, s, seed=sd))
+        w2, h2, s2 = int(rng.choice(widths)), int(rng.choice(heights)), int(rng.choice(sl))
+        dim = int(rng.integers(2))
+        leaves.append(lambda w=w2, h=h2, s=s2, dim=dim: Trace.index(U32, dim, w, h, s))
+    ops = ["add", "sub", "mul", "xor", "max", "min", "and", "or"]
+
+    def gen(d):
+        if d == 0 or rng.random() < 0.3:
+            return leaves[rng.integers(len(leaves))]()
+        return _bin(ops[rng.integers(len(ops))], gen(d - 1), gen(d - 1))
+
+    outs = []
+    for i in range(n_stmt):
+        e = gen(depth)
+        full = t.rng_uniform(U32, W, H, slots, seed=500 + i)
+        outs.append(t.assign(t.output(U32, W, H, slots), e + (full & u32(0))))
+    return t, data, outs
