@@ -243,6 +243,38 @@ int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B,
 /* record a variant for the shape's exogenous key (e.g. a model prediction) */
 int chai_hip_gemm_set_variant(const chai_gemm_desc* desc, int variant);
 
+/* ------------------------------------------------- learned tile-cost model */
+/* A 3 x 64 ReLU MLP (FP32) that predicts the core-kernel time of a GEMM tile
+ * variant from CHAI_GEMM_FEATURES features, trained on the tuner's own
+ * timings with the library's HIP forward / backward / update kernels.  It has
+ * no counterpart in the reference, whose only parameter choice is the timed
+ * search (autotune/StandardEM.cpp:69-121); that search stays the fallback and
+ * measures the model's regret.  X is [B][inputs] row-major f32, Y / out are
+ * [B] f32, all device pointers.  The loss is mean_b (out_b - Y_b)^2. */
+enum { CHAI_GEMM_FEATURES = 16 };
+enum { CHAI_MLP_SGD = 0, CHAI_MLP_ADAM = 1 };
+int chai_hip_mlp_create(int inputs, uint64_t seed, void** mlp);
+int chai_hip_mlp_num_params(void* mlp, size_t* count);
+/* parameter order: W1 [inputs][64], b1, W2 [64][64], b2, W3 [64][64], b3, w4 [64], b4 */
+int chai_hip_mlp_get_params(void* mlp, float* host, size_t count);
+int chai_hip_mlp_set_params(void* mlp, const float* host, size_t count);
+int chai_hip_mlp_forward(void* mlp, const float* X, size_t B, float* out, void* stream);
+/* grad (device, num_params floats) = d loss / d params */
+int chai_hip_mlp_backward(void* mlp, const float* X, const float* Y, size_t B, float* grad, void* stream);
+int chai_hip_mlp_update(void* mlp, const float* grad, int kind, float lr, void* stream);
+/* backward + update without materialising the gradient */
+int chai_hip_mlp_train_step(void* mlp, const float* X, const float* Y, size_t B, int kind, float lr,
+                            void* stream);
+int chai_hip_mlp_release(void* mlp);
+/* The model input of (shape, BF16X3 tile variant): CHAI_GEMM_FEATURES floats. */
+int chai_hip_gemm_features(const chai_gemm_desc* desc, int variant, float* out);
+/* One inference over every variant of the shape; the lowest predicted time
+ * wins and is recorded for the shape's exogenous key (as
+ * chai_hip_gemm_set_variant).  predicted (nullable) receives each variant's
+ * model output. */
+int chai_hip_gemm_predict_variant(const chai_gemm_desc* desc, void* mlp, int* variant, float* predicted,
+                                  void* stream);
+
 /* ----------------------------------------------------------------- traces */
 /* A trace is what one VectorTrace hands a device (common/VectorTrace.cpp):
  * variables (arrays) and statements "var = expression", each expression a

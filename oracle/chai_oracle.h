@@ -85,6 +85,12 @@ uint64_t chai_or_tea_hash(const uint64_t* data, size_t n);
 struct chai_trace;
 int chai_or_trace_eval(const struct chai_trace* trace, void* const* bufs);
 
+/* ---- tile-cost MLP (3 x 64 ReLU, scalar output), evaluated in double from f32
+ * parameters laid out as W1 [F][64], b1, W2 [64][64], b2, W3, b3, w4 [64], b4 */
+size_t chai_or_mlp_num_params(int F);
+int chai_or_mlp_forward(const float* params, int F, const float* X, size_t B, double* out);
+double chai_or_mlp_gradient(const float* params, int F, const float* X, const float* Y, size_t B, double* grad);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,11 @@ def lib():
         L.chai_or_tea_hash.restype = ctypes.c_uint64
         L.chai_or_tea_hash.argtypes = [vp, sz]
         L.chai_or_trace_eval.argtypes = [vp, vp]
+        L.chai_or_mlp_num_params.restype = sz
+        L.chai_or_mlp_num_params.argtypes = [i]
+        L.chai_or_mlp_forward.argtypes = [vp, i, vp, sz, vp]
+        L.chai_or_mlp_gradient.restype = ctypes.c_double
+        L.chai_or_mlp_gradient.argtypes = [vp, i, vp, vp, sz, vp]
         _lib = L
     return _lib
 
@@ -211,3 +216,27 @@ def trace_eval(trace, inputs):
 def tea_hash(words):
     w = np.ascontiguousarray(words, dtype=np.uint64)
     return int(lib().chai_or_tea_hash(_p(w), w.size))
+
+
+# ------------------------------------------------------------ tile-cost MLP
+def mlp_num_params(F):
+    return int(lib().chai_or_mlp_num_params(F))
+
+
+def mlp_forward(params, X):
+    """outputs (float64) of the 3 x 64 MLP for rows of X (f32 [B][F])"""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.empty(X.shape[0], dtype=np.float64)
+    lib().chai_or_mlp_forward(_p(p), X.shape[1], _p(X), X.shape[0], _p(out))
+    return out
+
+
+def mlp_gradient(params, X, Y):
+    """(loss, gradient) of mean (y - Y)^2 with respect to the f32 parameters, in double"""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    Y = np.ascontiguousarray(Y, dtype=np.float32)
+    g = np.empty(p.size, dtype=np.float64)
+    loss = lib().chai_or_mlp_gradient(_p(p), X.shape[1], _p(X), _p(Y), X.shape[0], _p(g))
+    return loss, g
