@@ -1,18 +1,27 @@
-"""Benchmark: FP32 GEMM M = N = K = 8192 on one MI355X, split-BF16x3 MFMA vs
-the plain FP32-FMA kernel, errors against an FP64 reference.
+"""Benchmarks of the MI355X back end; each run prints ONE JSON line.
 
-One "step" = one chai_hip_gemm call in split-BF16x3 mode on resident inputs
-(operand split pre-pass + MFMA kernel).  A and B are uniform[-1,1] float32
-from Chai's Philox4x32-10 stream (one generator, seed 42: A then B, the seed
-advancing by W*H between the two calls as api/RngFun.cpp:30 does).
+Default workload (what a plain `python bench.py` measures): FP32 GEMM
+M = N = K = 8192 on one MI355X, split-BF16x3 MFMA vs the plain FP32-FMA kernel,
+errors against an FP64 reference.  One "step" = one chai_hip_gemm call in
+split-BF16x3 mode on resident inputs (operand split pre-pass + MFMA kernel).
+A and B are uniform[-1,1] float32 from Chai's Philox4x32-10 stream (one
+generator, seed 42: A then B, the seed advancing by W*H between the two calls
+as api/RngFun.cpp:30 does).
 
-Prints ONE JSON line (see the fields below).  --gpus N > 1 is refused: this
-build measures one GPU.
+Other workloads (--workload): ragged (4096-instance ragged GEMM batch + fused
+epilogue), autotune (learned tile model: regret and cold start), mc (fused
+Monte Carlo trace, Philox normals, GEMV 16384^2), dgemm (FP64 MFMA 4096^3),
+mixed (i32 x f32 4096^3).  --profile-run is the short program profiled with
+rocprofv3; --pmc-summary turns two counter runs into the traffic summary.
+
+--gpus N > 1 is refused: this build measures one GPU.
 """
 import argparse
+import csv
+import glob
 import json
 import os
-import subprocess
+import re
 import sys
 import time
 
@@ -20,8 +29,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_BF16_TFLOPS = 2500.0      # MI355X dense BF16 MFMA
-PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector (incl. packed dual issue)
+PEAK_FP32_TFLOPS = 157.3       # MI355X FP32 vector
+PEAK_FP64_TFLOPS = 78.6        # MI355X FP64 MFMA
 PEAK_HBM_GBS = 8000.0
+METRIC = "GEMM TFLOP/s (bf16x3 MFMA vs FP32) at FP32-level error; autotune cold-start s"
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_bf16x3_8192_pmc.txt")
+MODEL_FILE = os.path.join(ROOT, "chai_amd", "data", "tile_mlp.npz")
 
 
 def parse():
@@ -29,19 +42,63 @@ def parse():
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--workload", default="sgemm",
+                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed"])
     p.add_argument("--size", type=int, default=8192)
     p.add_argument("--fp32-steps", type=int, default=3)
     p.add_argument("--cpu-size", type=int, default=1024,
                    help="edge of the interpreter-restatement sample timed on one host core")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--err-rows", type=int, default=16)
+    p.add_argument("--train-shapes", type=int, default=2000)
+    p.add_argument("--test-shapes", type=int, default=500)
+    p.add_argument("--save-model", default=None, help="autotune: write the trained model here")
+    p.add_argument("--profile-run", action="store_true",
+                   help="sgemm: 3 GEMMs plus the calibration kernels, for rocprofv3")
+    p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                   help="summarise two rocprofv3 --pmc runs of --profile-run (no GPU needed)")
     return p.parse_args()
 
 
-def cpu_baseline(n):
+def emit(line):
+    print(json.dumps(line), flush=True)
+
+
+def base_line(args, value, unit, ms, dtype, config, higher=True):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": higher, "scaling": "weak",
+            "vs_baseline": None, "dtype": dtype,
+            "data": "synthetic: uniform[-1,1] from Chai's Philox4x32-10 stream", "config": config}
+
+
+def timed(fn, steps, warmup):
+    """hipEvent time (ms) per call of fn() on torch's current stream"""
+    import torch
+    import chai_amd as chai
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    tm = chai.Timer()
+    tm.start()
+    for _ in range(steps):
+        fn()
+    tm.stop()
+    torch.cuda.synchronize()
+    return tm.ms() / steps
+
+
+def uniform(shape, seed, dtype=None):
+    import torch
+    import chai_amd as chai
+    t = torch.empty(*shape, dtype=dtype or torch.float32, device="cuda")
+    chai.rng_fill(t, seed=seed, limits=(-1.0, 1.0), W=shape[-1], H=t.numel() // shape[-1])
+    return t
+
+
+# ------------------------------------------------------------------ sgemm
+def cpu_baseline_gemm(n):
     """The oracle's interpreter-semantics matmul (interp/InterpMatmulBase.hpp:49-61) on one
     host core, timed on an n^3 sample of the same workload (f32 uniform[-1,1])."""
-    import numpy as np
     from oracle import oracle as o
     A = o.rng_fill(1230, 4, 4, 42, n, n, limits=(-1.0, 1.0)).reshape(n, n)
     B = o.rng_fill(1230, 4, 4, 42 + n * n, n, n, limits=(-1.0, 1.0)).reshape(n, n)
@@ -53,74 +110,150 @@ def cpu_baseline(n):
                       % (n, dt)}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: --gpus %d is not supported by this build "
-              "(the GEMM would shard by 2-D output tiles across GPUs; see DESIGN.md)" % args.gpus,
-              file=sys.stderr)
-        sys.exit(2)
+def read_traffic():
+    """HBM bytes per launch of the bf16x3 GEMM kernel from the committed counter summary."""
+    try:
+        with open(PMC_SUMMARY) as f:
+            m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
+        return float(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
+def cold_start(A, B, C):
+    """Wall seconds from a new shape to a chosen tile variant: exhaustive timed
+    search vs one inference of the shipped tile model (None without a model)."""
     import numpy as np
     import torch
     import chai_amd as chai
+    from chai_amd import tuner
+    n = A.shape[0]
+    shape = (n, n, n, 0, 0, chai.F32)
+    out = {}
+    if os.path.exists(MODEL_FILE):
+        m = tuner.load_model(MODEL_FILE)
+        tuner.predict(m, shape)        # first call pays one-time setup (module load)
+        v, _, secs = tuner.predict(m, shape)
+        out.update({"model_s": round(secs, 6), "model_variant": v})
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bv, _ = chai.gemm_tune(A, B, C, trials=3)
+    out.update({"exhaustive_s": round(time.perf_counter() - t0, 4), "exhaustive_variant": bv})
+    if "model_s" in out:
+        out["speedup"] = round(out["exhaustive_s"] / out["model_s"], 1)
+    return out
 
+
+def profile_run(args):
+    """Short program for rocprofv3: calibration kernels of known traffic, then
+    three bf16x3 GEMMs (8192^3)."""
+    import torch
+    import chai_amd as chai
+    chai.init(0)
+    n = args.size
+    big = torch.empty(1 << 28, dtype=torch.float32, device="cuda")     # 1 GiB
+    chai.rng_fill(big, seed=1)            # writes exactly 2^30 bytes
+    chai.reduce(big, "sum")               # reads exactly 2^30 bytes
+    A, B = uniform((n, n), 42), uniform((n, n), 42 + n * n)
+    C = torch.empty(n, n, device="cuda")
+    for _ in range(3):
+        chai.gemm(A, B, C=C, mode=chai.GEMM_BF16X3)
+    torch.cuda.synchronize()
+    print("profile run done", flush=True)
+
+
+def pmc_summary(fetch_dir, write_dir):
+    """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two rocprofv3 runs of
+    --profile-run, scaled by the calibration kernels (rng_fill writes 2^30 B, the
+    reduction's partial kernel reads 2^30 B)."""
+    def load(d, counter):
+        rows = {}
+        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(fn) as f:
+                for r in csv.DictReader(f):
+                    if r.get("Counter_Name") != counter:
+                        continue
+                    k = r.get("Kernel_Name", "?")
+                    rows.setdefault(k, []).append(float(r["Counter_Value"]))
+        return rows
+    fe, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+
+    def pick(rows, pat):
+        for k, v in rows.items():
+            if re.search(pat, k):
+                return k, sum(v) / len(v)
+        return None, None
+    _, f_cal = pick(fe, r"reduce_partial")
+    _, w_cal = pick(wr, r"rng_fill_32")
+    kf, f_gemm = pick(fe, r"gemm_x3_kernel")
+    kw, w_gemm = pick(wr, r"gemm_x3_kernel")
+    fetch_scale = (2 ** 30) / (f_cal * 1024) if f_cal else None
+    write_scale = (2 ** 30) / (w_cal * 1024) if w_cal else None
+    lines = ["# bf16x3 GEMM 8192^3: HBM traffic per launch from rocprofv3 counters",
+             "# commands (from the repository root):",
+             "#   rocprofv3 --pmc FETCH_SIZE -d <fetch_dir> -- python3 bench.py --profile-run",
+             "#   rocprofv3 --pmc WRITE_SIZE -d <write_dir> -- python3 bench.py --profile-run",
+             "#   python3 bench.py --pmc-summary <fetch_dir> <write_dir>",
+             "calibration_fetch_KiB(reduce 2^30 B read) = %s" % f_cal,
+             "calibration_write_KiB(rng_fill 2^30 B written) = %s" % w_cal,
+             "fetch_scale = %s" % fetch_scale, "write_scale = %s" % write_scale,
+             "kernel = %s" % (kf or kw),
+             "fetch_KiB_per_dispatch = %s" % f_gemm, "write_KiB_per_dispatch = %s" % w_gemm]
+    if f_gemm and w_gemm and fetch_scale and write_scale:
+        t = f_gemm * 1024 * fetch_scale + w_gemm * 1024 * write_scale
+        lines.append("traffic_bytes_per_launch = %.0f" % t)
+        n = 8192
+        lines.append("algorithmic_bytes_per_launch = %d  # bf16 hi+lo panels of A and B^T read once, C written"
+                     % (2 * 2 * 2 * n * n + 4 * n * n))
+    for k in sorted(set(fe) | set(wr)):
+        lines.append("  %-90.90s fetch_KiB %-14s write_KiB %s" % (
+            k, "%.0f" % (sum(fe[k]) / len(fe[k])) if k in fe else "-",
+            "%.0f" % (sum(wr[k]) / len(wr[k])) if k in wr else "-"))
+    print("\n".join(lines))
+
+
+def run_sgemm(args):
+    import numpy as np
+    import torch
+    import chai_amd as chai
     n = args.size
     chai.init(0)
     dev = torch.device("cuda:0")
-    A = torch.empty(n, n, dtype=torch.float32, device=dev)
-    B = torch.empty(n, n, dtype=torch.float32, device=dev)
-    chai.rng_fill(A, seed=42, limits=(-1.0, 1.0), W=n, H=n)
-    chai.rng_fill(B, seed=42 + n * n, limits=(-1.0, 1.0), W=n, H=n)
+    A, B = uniform((n, n), 42), uniform((n, n), 42 + n * n)
     C = torch.empty(n, n, dtype=torch.float32, device=dev)
     flops = 2.0 * n * n * n
 
     def run(mode):
         chai.gemm(A, B, C=C, mode=mode)
 
-    # ---- split-BF16x3: warmup, then exactly `steps` timed steps
-    for _ in range(args.warmup):
-        run(chai.GEMM_BF16X3)
-    torch.cuda.synchronize()
-    tm = chai.Timer()
+    # split-BF16x3: warmup, then exactly `steps` timed steps (hipEvents on the launch stream)
     t0 = time.perf_counter()
-    tm.start()
-    for _ in range(args.steps):
-        run(chai.GEMM_BF16X3)
-    tm.stop()
-    torch.cuda.synchronize()
+    ms_x3 = timed(lambda: run(chai.GEMM_BF16X3), args.steps, args.warmup)
     wall = time.perf_counter() - t0
-    ms_x3 = tm.ms() / args.steps
     tflops_x3 = flops / (ms_x3 * 1e-3) / 1e12
     rows = np.random.default_rng(0).choice(n, args.err_rows, replace=False)
-    Cx3 = C[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.float64)
+    ridx = torch.from_numpy(rows).to(dev)
+    Cx3 = C[ridx].cpu().numpy().astype(np.float64)
 
-    # ---- per-kernel durations on the launch stream (hipEvents inside the library)
+    # per-kernel durations on the launch stream (hipEvents inside the library)
     chai.profile(True)
     prep, core = [], []
-    for _ in range(3):
+    for _ in range(5):
         run(chai.GEMM_BF16X3)
         a, b = chai.gemm_last_timing()
-        prep.append(a); core.append(b)
+        prep.append(a)
+        core.append(b)
     chai.profile(False)
     core_ms = sum(core) / len(core)
     prep_ms = sum(prep) / len(prep)
 
-    # ---- plain FP32-FMA kernel on the same inputs
-    run(chai.GEMM_FMA)
-    torch.cuda.synchronize()
-    tm.start()
-    for _ in range(args.fp32_steps):
-        run(chai.GEMM_FMA)
-    tm.stop()
-    ms_f32 = tm.ms() / args.fp32_steps
+    ms_f32 = timed(lambda: run(chai.GEMM_FMA), args.fp32_steps, 1)
     tflops_f32 = flops / (ms_f32 * 1e-3) / 1e12
-    Cf32 = C[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.float64)
+    Cf32 = C[ridx].cpu().numpy().astype(np.float64)
 
-    # ---- errors against FP64 on sampled rows (exact f32 products, float64 sums)
-    Ah = A[torch.from_numpy(rows).to(dev)].cpu().numpy().astype(np.float64)
-    Bh = B.cpu().numpy().astype(np.float64)
-    C64 = Ah @ Bh
-    e3 = np.abs(Cx3 - C64); ef = np.abs(Cf32 - C64)
+    # errors against FP64 on sampled rows
+    C64 = A[ridx].cpu().numpy().astype(np.float64) @ B.cpu().numpy().astype(np.float64)
+    e3, ef = np.abs(Cx3 - C64), np.abs(Cf32 - C64)
     err = {"rows_sampled": int(args.err_rows),
            "bf16x3_max_abs": float(e3.max()), "bf16x3_rms": float(np.sqrt((e3 ** 2).mean())),
            "fp32_fma_max_abs": float(ef.max()), "fp32_fma_rms": float(np.sqrt((ef ** 2).mean())),
@@ -129,37 +262,247 @@ def main():
 
     mfma_flops = 3 * flops           # three bf16 products per FP32 multiply-add
     achieved = mfma_flops / (core_ms * 1e-3) / 1e12
-    line = {
-        "metric": "GEMM TFLOP/s (bf16x3 MFMA vs FP32) at FP32-level error; autotune cold-start s",
-        "value": round(tflops_x3, 2),
-        "unit": "TFLOP/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_x3, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32 (split-bf16x3 MFMA, f32 accumulate)",
-        "data": "synthetic: uniform[-1,1] f32 from Chai's Philox4x32-10 stream, seed 42",
-        "config": {"workload": "sgemm %dx%dx%d row-major f32, split-bf16x3 vs plain fp32-fma" % (n, n, n),
-                   "M": n, "N": n, "K": n},
-        "roofline": {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
-                     "traffic": None,
-                     "kernel": "gemm_bf16x3", "kernel_ms": round(core_ms, 4),
-                     "algorithmic_flops": mfma_flops},
-        "fp32_fma": {"tflops": round(tflops_f32, 2), "ms": round(ms_f32, 3),
-                     "frac_of_fp32_peak": round(tflops_f32 / PEAK_FP32_TFLOPS, 4)},
-        "speedup_vs_fp32": round(tflops_x3 / tflops_f32, 3),
-        "split_prepass_ms": round(prep_ms, 4),
-        "errors_vs_fp64": err,
-        "autotune_cold_start_s": None,
-        "wall_s_timed": round(wall, 3),
-    }
+    line = base_line(args, round(tflops_x3, 2), "TFLOP/s", round(ms_x3, 4),
+                     "f32 (split-bf16x3 MFMA, f32 accumulate)",
+                     {"workload": "sgemm %dx%dx%d row-major f32, split-bf16x3 vs plain fp32-fma" % (n, n, n),
+                      "M": n, "N": n, "K": n})
+    line["roofline"] = {"bound": "mfma", "achieved": round(achieved, 1), "peak": PEAK_BF16_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(achieved / PEAK_BF16_TFLOPS, 4),
+                        "traffic": read_traffic(), "kernel": "gemm_x3_kernel", "kernel_ms": round(core_ms, 4),
+                        "algorithmic_flops_per_launch": mfma_flops}
+    line["fp32_fma"] = {"tflops": round(tflops_f32, 2), "ms": round(ms_f32, 3),
+                        "frac_of_fp32_peak": round(tflops_f32 / PEAK_FP32_TFLOPS, 4)}
+    line["speedup_vs_fp32"] = round(tflops_x3 / tflops_f32, 3)
+    line["split_prepass_ms"] = round(prep_ms, 4)
+    line["errors_vs_fp64"] = err
+    line["autotune_cold_start_s"] = cold_start(A, B, C)
+    line["wall_s_timed"] = round(wall, 3)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(args.cpu_size)
-    print(json.dumps(line), flush=True)
+        line["cpu_baseline"] = cpu_baseline_gemm(args.cpu_size)
+    emit(line)
+
+
+# ------------------------------------------------------------------ ragged
+def run_ragged(args):
+    import numpy as np
+    import torch
+    import chai_amd as chai
+    from chai_amd import ragged
+    from chai_amd.workloads import RAGGED_EPILOGUE, ragged_sizes
+    chai.init(0)
+    sizes = ragged_sizes()
+    batch = []
+    for i, k in enumerate(sizes):
+        A, B = uniform((k, k), 42 + 2 * i), uniform((k, k), 43 + 2 * i)
+        batch.append((A, B, torch.empty(k, k, device="cuda")))
+    epi = chai.epilogue(*RAGGED_EPILOGUE)
+    items = [ragged.item(*x) for x in batch]
+    t0 = time.perf_counter()
+    v, _ = ragged.tune(items, epi=epi, trials=2)
+    tune_s = time.perf_counter() - t0
+    plan = ragged.Plan(items, epi=epi, variant=v)
+    plan.run()
+    split_ms, core_ms = plan.last_timing()
+    info = plan.info()
+    ms = timed(lambda: plan.run(graph=True), args.steps, args.warmup)
+    flops = info["useful_flops"]
+    # per-instance launches of the same kernels (fused epilogue), as the reference runs them
+    ms_pi = timed(lambda: [chai.gemm(A, B, C, mode=chai.GEMM_BF16X3, epi=epi, variant=v) for A, B, C in batch],
+                  1, 1)
+    line = base_line(args, round(flops / (ms * 1e-3) / 1e12, 2), "TFLOP/s", round(ms, 4),
+                     "f32 (split-bf16x3 MFMA, f32 accumulate)",
+                     {"workload": "ragged batch: 4096 square f32 GEMMs, n log-uniform in [32,1024] step 16 "
+                                  "(seed 42), epilogue max(C*0.5+1,0), one persistent launch in a hipGraph",
+                      "instances": len(sizes), "sum_2n3": flops})
+    line["roofline"] = {"bound": "mfma", "achieved": round(3 * info["padded_flops"] / (core_ms * 1e-3) / 1e12, 1),
+                        "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(3 * info["padded_flops"] / (core_ms * 1e-3) / 1e12 / PEAK_BF16_TFLOPS, 4),
+                        "traffic": None, "kernel": "ragged_x3_kernel", "kernel_ms": round(core_ms, 4)}
+    line["split_prepass_ms"] = round(split_ms, 4)
+    line["tile_variant"] = v
+    line["tune_s"] = round(tune_s, 2)
+    line["plan"] = info
+    line["per_instance_ms"] = round(ms_pi, 3)
+    line["speedup_vs_per_instance"] = round(ms_pi / ms, 2)
+    if not args.no_cpu:
+        from oracle import oracle as o
+        rng = np.random.default_rng(1)
+        sub = rng.choice(len(sizes), 64, replace=False)
+        t0 = time.perf_counter()
+        f_sub = 0.0
+        for i in sub:
+            k = int(sizes[i])
+            Ah = batch[i][0].cpu().numpy()
+            Bh = batch[i][1].cpu().numpy()
+            Ch = o.matmul_interp(Ah, Bh)
+            np.maximum(Ch.astype(np.float32) * np.float32(0.5) + np.float32(1), 0)
+            f_sub += 2.0 * k ** 3
+            if time.perf_counter() - t0 > 25:
+                break
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": f_sub / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+                                "sample": "interpreter-semantics matmul + epilogue on a random subsample "
+                                          "of the instances (%.2e flop, %.1f s, 1 core)" % (f_sub, dt)}
+    emit(line)
+
+
+# ----------------------------------------------------------------- autotune
+def run_autotune(args):
+    import numpy as np
+    import torch
+    import chai_amd as chai
+    from chai_amd import tuner
+    chai.init(0)
+    ws = tuner.Workspace()
+    train = tuner.random_shapes(args.train_shapes, seed=2024)
+    test = tuner.random_shapes(args.test_shapes, seed=2025)
+    t_train, t_test, cold_ex = [], [], []
+    t0 = time.perf_counter()
+    for i, s in enumerate(train):
+        t_train.append(tuner.time_variants(ws, s)[0])
+        if i % 250 == 0:
+            print("# timed %d/%d train shapes (%.0f s)" % (i, len(train), time.perf_counter() - t0),
+                  file=sys.stderr, flush=True)
+    for s in test:
+        ms, secs = tuner.time_variants(ws, s)
+        t_test.append(ms)
+        cold_ex.append(secs)
+    collect_s = time.perf_counter() - t0
+    t_train, t_test = np.array(t_train), np.array(t_test)
+    X, Y = tuner.dataset(train, t_train)
+    mu, sd = float(Y.mean()), float(Y.std())
+    m = tuner.MLP(tuner.NFEAT, seed=0)
+    t1 = time.perf_counter()
+    m.fit(torch.from_numpy(X).cuda(), torch.from_numpy((Y - mu) / sd).cuda(), epochs=args.steps * 20,
+          batch=256, lr=2e-3)
+    train_s = time.perf_counter() - t1
+    tuner.predict(m, test[0])
+    choice, cold_m = [], []
+    for s in test:
+        v, _, secs = tuner.predict(m, s)
+        choice.append(v)
+        cold_m.append(secs)
+    reg = tuner.regret(choice, t_test)
+    fixed = int(np.argmin(np.log(t_train).sum(0)))
+    reg_fixed = tuner.regret([fixed] * len(test), t_test)
+    reg_v0 = tuner.regret([0] * len(test), t_test)
+    perf_ratio = float(np.mean(t_test.min(1) / t_test[np.arange(len(test)), choice]))
+    if args.save_model:
+        tuner.save_model(m, args.save_model, mu, sd)
+    line = base_line(args, round(float(np.mean(cold_m)), 6), "s", round(float(np.mean(cold_m)) * 1e3, 4),
+                     "f32 (MLP 3x64)", {"workload": "learned autotune: %d train / %d held-out shapes, M,N,K "
+                                        "log-uniform [64,16384] step 16, bf16x3 tile variants"
+                                        % (len(train), len(test))}, higher=False)
+    line["metric"] = METRIC
+    line["autotune"] = {
+        "cold_start_model_s_mean": float(np.mean(cold_m)), "cold_start_exhaustive_s_mean": float(np.mean(cold_ex)),
+        "cold_start_speedup": float(np.mean(cold_ex) / np.mean(cold_m)),
+        "regret_mean": float(reg.mean()), "regret_median": float(np.median(reg)),
+        "regret_p90": float(np.quantile(reg, 0.9)), "perf_vs_exhaustive_mean": perf_ratio,
+        "regret_fixed_best_variant": float(reg_fixed.mean()), "fixed_variant": fixed,
+        "regret_variant0": float(reg_v0.mean()),
+        "train_rows": int(X.shape[0]), "train_s": round(train_s, 2), "collect_s": round(collect_s, 1)}
+    emit(line)
+
+
+# ----------------------------------------------------------------------- mc
+def run_mc(args):
+    import torch
+    import chai_amd as chai
+    from chai_amd.workloads import mc_trace
+    chai.init(0)
+    n = 1 << 24
+    t, m = mc_trace(n)
+    prog = t.compile()
+    out = torch.zeros(1, device="cuda")
+    ms_mc = timed(lambda: prog.run({m: out}), args.steps, args.warmup)
+    z = torch.empty(n, device="cuda")
+    ms_norm = timed(lambda: chai.rng_fill(z, seed=7, normal=True), args.steps, args.warmup)
+    N = 16384
+    A, x = uniform((N, N), 42), uniform((N,), 42 + N * N)
+    y = torch.empty(N, device="cuda")
+    ms_gemv = timed(lambda: chai.gemv(A, x, y=y), args.steps, args.warmup)
+    gemv_bytes = 4.0 * (N * N + 2 * N)
+    gbs = gemv_bytes / (ms_gemv * 1e-3) / 1e9
+    line = base_line(args, round(n / (ms_mc * 1e-3) / 1e9, 3), "Gelem/s", round(ms_mc, 4), "f32",
+                     {"workload": "fused MC trace 2^24 Philox4x32-10 normals (seed 7) -> payoff mean; "
+                                  "f32 GEMV 16384^2"})
+    line["mc_mean"] = float(out.item())
+    line["philox_normals_2^24_ms"] = round(ms_norm, 4)
+    line["philox_normals_GBs"] = round(4.0 * n / (ms_norm * 1e-3) / 1e9, 1)
+    line["gemv_ms"] = round(ms_gemv, 4)
+    line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "gemv_rows"}
+    if not args.no_cpu:
+        from oracle import oracle as o
+        k = 1 << 20
+        tc, mc = mc_trace(k)
+        t0 = time.perf_counter()
+        o.trace_eval(tc, {})
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": k / dt / 1e9, "unit": "Gelem/s", "cores": 1, "kind": "port",
+                                "sample": "oracle trace evaluation of the MC trace on 2^20 elements (%.1f s)" % dt}
+    emit(line)
+
+
+# ------------------------------------------------------------ dgemm / mixed
+def run_dense(args, mixed):
+    import numpy as np
+    import torch
+    import chai_amd as chai
+    chai.init(0)
+    n = 4096
+    if mixed:
+        W = torch.empty(n, n, dtype=torch.int32, device="cuda")
+        chai.rng_fill(W, seed=42, prec=chai.U32)               # raw Philox words
+        A = (W & 255) - 128                                    # lo + (w mod 256) in [-128, 127]
+        B = uniform((n, n), 42 + n * n)
+        fn = lambda: chai.gemm(A, B, C=C, mode=chai.GEMM_BF16X3)   # noqa: E731
+        C = torch.empty(n, n, device="cuda")
+        dtype, peak, bound = "i32 x f32 -> f32 (bf16x3 MFMA)", PEAK_BF16_TFLOPS, "mfma"
+    else:
+        A = uniform((n, n), 42, torch.float64)
+        B = uniform((n, n), 42 + n * n, torch.float64)
+        C = torch.empty(n, n, dtype=torch.float64, device="cuda")
+        fn = lambda: chai.gemm(A, B, C=C, mode=chai.GEMM_F64)     # noqa: E731
+        dtype, peak, bound = "f64 (FP64 MFMA)", PEAK_FP64_TFLOPS, "mfma"
+    ms = timed(fn, args.steps, args.warmup)
+    tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
+    ach = tf * (3 if mixed else 1)
+    rows = np.random.default_rng(0).choice(n, 8, replace=False)
+    ridx = torch.from_numpy(rows).cuda()
+    ref = A[ridx].cpu().numpy().astype(np.float64) @ B.cpu().numpy().astype(np.float64)
+    err = float(np.abs(C[ridx].cpu().numpy().astype(np.float64) - ref).max())
+    line = base_line(args, round(tf, 2), "TFLOP/s", round(ms, 4), dtype,
+                     {"workload": ("mixed i32[-128,127] x f32 " if mixed else "dgemm f64 ") + "4096^3"})
+    line["roofline"] = {"bound": bound, "achieved": round(ach, 1), "peak": peak, "unit": "TFLOP/s",
+                        "frac": round(ach / peak, 4), "traffic": None}
+    line["max_abs_err_vs_fp64"] = err
+    emit(line)
+
+
+def main():
+    args = parse()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: --gpus %d is not supported by this build "
+              "(the GEMM would shard by 2-D output tiles across GPUs, all-gathering B panels over xGMI)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if args.profile_run:
+        profile_run(args)
+    elif args.workload == "sgemm":
+        run_sgemm(args)
+    elif args.workload == "ragged":
+        run_ragged(args)
+    elif args.workload == "autotune":
+        run_autotune(args)
+    elif args.workload == "mc":
+        run_mc(args)
+    else:
+        run_dense(args, args.workload == "mixed")
 
 
 if __name__ == "__main__":

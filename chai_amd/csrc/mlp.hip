@@ -7,8 +7,8 @@
 // and remembers the winner per exogenous key (enqueue/JustInTimeMemo.cpp:
 // 92-114).  The model predicts the core-kernel time of every tile variant of a
 // shape it has never seen, so a new shape gets its variant from one inference
-// instead of a timed search; the search remains the fallback and the judge of
-// the model's regret.
+// instead of a timed search; the search remains the fallback and measures the
+// model's regret.
 //
 // Network: x (F <= 32 features) -> 64 ReLU -> 64 ReLU -> 64 ReLU -> 1, FP32.
 // Kernel layout: one wavefront per sample, lane j = hidden unit j (64 units =

@@ -7,10 +7,11 @@ enqueue/EnqueueMatmulAuto.cpp:71-161).  Here the same search
 small MLP (chai_hip_mlp_*: 3 x 64 ReLU, FP32, trained by the library's HIP
 kernels) that predicts every variant's core-kernel time from the shape; a new
 shape then gets its variant from one inference (chai_hip_gemm_predict_variant)
-instead of a search, and the search remains the fallback and the judge of the
+instead of a search, and the search remains the fallback and measures the
 model's regret.
 """
 import ctypes
+import os
 import time
 
 import numpy as np
@@ -218,3 +219,18 @@ def regret(choice, times):
     best = t.min(axis=1)
     picked = t[np.arange(len(t)), np.asarray(choice)]
     return 1.0 - best / picked
+
+
+def save_model(model, path, y_mean=0.0, y_std=1.0):
+    """Parameters (+ the target normalisation) as a numpy .npz (no pickle)."""
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=model.params(), inputs=np.int32(model.inputs), y_mean=np.float64(y_mean),
+             y_std=np.float64(y_std))
+
+
+def load_model(path):
+    with np.load(path, allow_pickle=False) as z:
+        m = MLP(int(z["inputs"]), seed=0)
+        m.set_params(z["params"])
+        m.y_mean, m.y_std = float(z["y_mean"]), float(z["y_std"])
+    return m

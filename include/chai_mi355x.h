@@ -10,7 +10,7 @@
  *   autotune/AutoTuneMemo, enqueue/JustInTimeMemo   -> chai_hip_memo_* / chai_hip_jit_*
  *   vendor/OCLdevice (buffers, queue, timing)       -> chai_hip_init / alloc / h2d / d2h / timer
  *
- * Conventions (kept from the reference, SURVEY-level contract):
+ * Conventions (kept from the reference):
  *   - Every entry point returns an int status: CHAI_OK (0) or a negative code.
  *     No exception crosses the ABI.  A failure maps to sub_evaluate() returning
  *     false, which makes the scheduler re-queue the trace on another device

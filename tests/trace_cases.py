@@ -61,15 +61,7 @@ def md5_hex(outs):
 MSGS = [("chai md5 message %d %s" % (i, "x" * (i % 23))).encode() for i in range(28)]
 
 # ------------------------------------------------------------ Monte Carlo trace
-def mc_trace(W, H=1, seed=7):
-    """S = 100*exp((0.05-0.02)*1 + 0.2*z), payoff = max(S-105, 0), mean(payoff);
-    z = 2^24 Philox4x32-10 normals (Box-Muller).  Float literals keep it f32."""
-    t = Trace()
-    z = t.rng_normal(F32, W, H, seed=seed)
-    drift = float(np.float32(np.float32(0.05) - np.float32(0.02)) * np.float32(1))
-    S = f32(100.0) * exp(f32(drift) + f32(0.2) * z)
-    m = t.assign(t.output(F32, 1), mean(maxe(S - f32(105.0), f32(0.0))))
-    return t, m
+from chai_amd.workloads import mc_trace  # noqa: E402,F401  (shared with bench.py)
 
 
 # ------------------------------------------------------------- random traces
