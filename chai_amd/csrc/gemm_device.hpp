@@ -34,6 +34,22 @@ __device__ __forceinline__ void split_bf16(float a, uint16_t& hi, uint16_t& lo)
     lo = f32_to_bf16_rne(rem);
 }
 
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+
+// The same split for two values with gfx950's v_cvt_pk_bf16_f32 (round to
+// nearest even, equal to f32_to_bf16_rne on finite inputs): packed hi and lo
+// words, element 0 in the low half.
+__device__ __forceinline__ void split_bf16x2(float a0, float a1, uint32_t& hi, uint32_t& lo)
+{
+    const f32x2_t v = {a0, a1};
+    const bf16x2_t h = __builtin_convertvector(v, bf16x2_t);
+    hi = __builtin_bit_cast(uint32_t, h);
+    const f32x2_t hf = {__uint_as_float(hi << 16), __uint_as_float(hi & 0xffff0000u)};
+    const f32x2_t r = v - hf;                     // exact (Sterbenz)
+    lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2_t));
+}
+
 // XCD-aware tile order: the dispatcher hands consecutive workgroups to the 8
 // XCDs round-robin; remap so that the tiles one XCD runs together are
 // neighbours (shared A row panels / B column panels stay in that XCD's L2),

@@ -5,8 +5,10 @@
 // src[k * ld + r] (TRANS = true).  The output panels hi / lo are [Rp][Kp]
 // bf16, K contiguous, with X = hi + lo (+ a residual below 2^-17 |X|) and zeros
 // wherever r >= R or k >= K.  One call covers the 64 x 64 block at (r0, k0)
-// with 256 threads; reads are coalesced along the contiguous source index and
-// go through a padded LDS tile so that each thread can write 16 consecutive k.
+// with 256 threads; reads are 16-byte vectors along the contiguous source
+// index.  A K-contiguous source is split straight from registers; a
+// transposed one goes through a padded LDS tile so that each thread can write
+// 16 consecutive k.
 #pragma once
 #include "gemm_device.hpp"
 
@@ -15,6 +17,38 @@ namespace chai_hip {
 template <typename T>
 __device__ __forceinline__ float to_f32(T v) { return (float)v; }
 
+template <typename T>
+__device__ __forceinline__ f32x4 load4_f32(const T* p)
+{
+    if constexpr (sizeof(T) == 4 && (T)0.5 != 0) {
+        return *reinterpret_cast<const f32x4*>(p);
+    } else {
+        typedef T t4 __attribute__((ext_vector_type(4)));
+        const t4 v = *reinterpret_cast<const t4*>(p);
+        return f32x4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+    }
+}
+
+// 16 consecutive k of one row -> 2 x 16 bytes of each panel
+__device__ __forceinline__ void split_store16(const float* v, uint16_t* hi, uint16_t* lo, uint64_t off,
+                                             bool second)
+{
+    u32x4 vh0, vh1, vl0, vl1;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint32_t h, l;
+        split_bf16x2(v[2 * j], v[2 * j + 1], h, l);
+        vh0[j] = h; vl0[j] = l;
+        split_bf16x2(v[8 + 2 * j], v[8 + 2 * j + 1], h, l);
+        vh1[j] = h; vl1[j] = l;
+    }
+    u32x4* dh = reinterpret_cast<u32x4*>(hi + off);
+    u32x4* dl = reinterpret_cast<u32x4*>(lo + off);
+    dh[0] = vh0;
+    dl[0] = vl0;
+    if (second) { dh[1] = vh1; dl[1] = vl1; }
+}
+
 template <typename T, bool TRANS>
 __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t ld, uint64_t R,
                                            uint64_t K, uint64_t Kp, uint16_t* __restrict__ hi,
@@ -22,42 +56,55 @@ __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t l
                                            float (*tile)[65])
 {
     const int t = threadIdx.x;
-    if (TRANS) {
-        // 64 lanes read 64 consecutive r at one k
-        for (int i = 0; i < 16; i++) {
-            const int kk = i * 4 + (t >> 6), rr = t & 63;
-            const uint64_t r = r0 + rr, k = k0 + kk;
-            tile[kk][rr] = (r < R && k < K) ? to_f32(src[k * ld + r]) : 0.0f;
+    // each thread finally writes 16 consecutive k of one row
+    const int rr = t >> 2, kq = (t & 3) * 16;
+    const uint64_t r = r0 + rr;
+    const bool vec = ((ld & 3) == 0) && (((uintptr_t)src & 15) == 0);
+    if (!TRANS) {
+        // K-contiguous source: no transpose needed, straight from registers
+        if (k0 + kq >= Kp) return;
+        float v[16];
+        const uint64_t kb = k0 + kq;
+        if (vec && r < R && kb + 16 <= K) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const f32x4 x = load4_f32(src + r * ld + kb + 4 * q);
+                v[4 * q] = x[0]; v[4 * q + 1] = x[1]; v[4 * q + 2] = x[2]; v[4 * q + 3] = x[3];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; j++) v[j] = (r < R && kb + j < K) ? to_f32(src[r * ld + kb + j]) : 0.0f;
         }
-    } else {
-        for (int i = 0; i < 16; i++) {
-            const int rr = i * 4 + (t >> 6), kk = t & 63;
-            const uint64_t r = r0 + rr, k = k0 + kk;
-            tile[kk][rr] = (r < R && k < K) ? to_f32(src[r * ld + k]) : 0.0f;
+        split_store16(v, hi, lo, r * Kp + kb, kb + 8 < Kp);
+        return;
+    }
+    // source element (r, k) at src[k * ld + r]: read 4 consecutive r per lane
+    // (16 lanes per 64-wide row), transpose through the padded LDS tile
+    {
+        const int r4 = (t & 15) * 4, kk0 = t >> 4;   // 16 k rows per pass
+        const bool fullr = vec && r0 + r4 + 4 <= R;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int kk = kk0 + 16 * i;
+            const uint64_t k = k0 + kk;
+            if (fullr && k < K) {
+                const f32x4 x = load4_f32(src + k * ld + r0 + r4);
+                tile[kk][r4] = x[0]; tile[kk][r4 + 1] = x[1]; tile[kk][r4 + 2] = x[2]; tile[kk][r4 + 3] = x[3];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint64_t rj = r0 + r4 + j;
+                    tile[kk][r4 + j] = (rj < R && k < K) ? to_f32(src[k * ld + rj]) : 0.0f;
+                }
+            }
         }
     }
     __syncthreads();
-    // each thread writes 16 consecutive k of one row: 2 x 16 bytes per panel
-    const int rr = t >> 2, kq = (t & 3) * 16;
-    const uint64_t r = r0 + rr;
-    uint16_t h[16], l[16];
-#pragma unroll
-    for (int j = 0; j < 16; j++) split_bf16(tile[kq + j][rr], h[j], l[j]);
     if (k0 + kq < Kp) {
-        u32x4* dh = reinterpret_cast<u32x4*>(hi + r * Kp + k0 + kq);
-        u32x4* dl = reinterpret_cast<u32x4*>(lo + r * Kp + k0 + kq);
-        u32x4 vh0, vh1, vl0, vl1;
+        float v[16];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            vh0[j] = h[2 * j] | ((uint32_t)h[2 * j + 1] << 16);
-            vh1[j] = h[8 + 2 * j] | ((uint32_t)h[8 + 2 * j + 1] << 16);
-            vl0[j] = l[2 * j] | ((uint32_t)l[2 * j + 1] << 16);
-            vl1[j] = l[8 + 2 * j] | ((uint32_t)l[8 + 2 * j + 1] << 16);
-        }
-        dh[0] = vh0;
-        if (k0 + kq + 8 < Kp) dh[1] = vh1;
-        dl[0] = vl0;
-        if (k0 + kq + 8 < Kp) dl[1] = vl1;
+        for (int j = 0; j < 16; j++) v[j] = tile[kq + j][rr];
+        split_store16(v, hi, lo, r * Kp + k0 + kq, k0 + kq + 8 < Kp);
     }
 }
 

@@ -27,9 +27,15 @@ __global__ __launch_bounds__(256) void rng_fill_32(RngGeom g, T* __restrict__ ou
 {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.nvec) return;
-    const uint64_t row = v / g.vpr;            // slot * H + y
-    const uint64_t x0 = (v - row * g.vpr) * VL;
-    if (x0 >= g.W) return;                     // a vector entirely in the padding
+    // rows without padding (W a multiple of VL) are contiguous in the vector
+    // index: element offset v * VL, no division
+    uint64_t row = 0, x0 = v * VL, n = VL;
+    if (g.W != g.vpr * VL) {
+        row = v / g.vpr;                       // slot * H + y
+        x0 = (v - row * g.vpr) * VL;
+        if (x0 >= g.W) return;                 // a vector entirely in the padding
+        n = (g.W - x0) < (uint64_t)VL ? (g.W - x0) : (uint64_t)VL;
+    }
     uint32_t w[4];
     rng_words32<VARIANT, VL>((uint32_t)v, (uint32_t)g.seed, w);
     T val[4];
@@ -50,7 +56,6 @@ __global__ __launch_bounds__(256) void rng_fill_32(RngGeom g, T* __restrict__ ou
         }
     }
     T* dst = out + row * g.W + x0;
-    const uint64_t n = (g.W - x0) < (uint64_t)VL ? (g.W - x0) : (uint64_t)VL;
     if (VL == 4 && n == 4 && (g.W & 3) == 0 && ((uintptr_t)out & 15) == 0) {
         typedef T T4 __attribute__((ext_vector_type(4)));
         T4 q = {val[0], val[1], val[2], val[3]};
@@ -65,9 +70,12 @@ __global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict
 {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.nvec) return;
-    const uint64_t row = v / g.vpr;
-    const uint64_t x0 = (v - row * g.vpr) * VL;
-    if (x0 >= g.W) return;
+    uint64_t row = 0, x0 = v * VL;
+    if (g.W != g.vpr * VL) {
+        row = v / g.vpr;
+        x0 = (v - row * g.vpr) * VL;
+        if (x0 >= g.W) return;
+    }
     uint64_t w[2];
     rng_words64<VARIANT>(v, g.seed, w);
     double val[2];

@@ -28,8 +28,10 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        const uint32_t lo0 = 0xD2511F53u * c[0], hi0 = __umulhi(0xD2511F53u, c[0]);
-        const uint32_t lo1 = 0xCD9E8D57u * c[2], hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32)
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+        const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
         const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
     }
@@ -40,7 +42,8 @@ __device__ __forceinline__ void philox2x32_10(uint32_t c[2], uint32_t k0)
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r) k0 += 0x9E3779B9u;
-        const uint32_t lo = 0xd256d193u * c[0], hi = __umulhi(0xd256d193u, c[0]);
+        const uint64_t p = (uint64_t)0xd256d193u * c[0];
+        const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
         c[0] = hi ^ k0 ^ c[1];
         c[1] = lo;
     }

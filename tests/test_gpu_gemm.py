@@ -48,7 +48,7 @@ MODES = {"fma": 1, "bf16x3": 2}
 
 @pytest.mark.parametrize("mode", ["fma", "bf16x3"])
 @pytest.mark.parametrize("M,N,K", [(256, 256, 256), (1, 1, 1), (17, 33, 65), (300, 129, 1000),
-                                   (512, 768, 96), (1024, 1024, 1024)])
+                                   (512, 768, 96), (1024, 1024, 1024), (512, 512, 1)])
 def test_gemm_f32(chai, oracle, mode, M, N, K):
     print("case", mode, M, N, K, flush=True)
     A = unif(oracle, 42, M, K); B = unif(oracle, 43, K, N)
