@@ -215,6 +215,71 @@ __device__ __forceinline__ void box_muller(double ua, double ub, double& n0, dou
     n1 = r * c;
 }
 
+// ------------------------------------------------------ Box-Muller math (f32)
+// The f32 stream evaluates its normals in f32: the same range reductions as
+// above with series truncated for single precision, every operation an IEEE
+// f32 op (explicit fma, correctly rounded divide and sqrt), so the host
+// restatement reproduces them bit for bit.
+// log(x) = e*ln2 + 2s + s z P(z), s = f/(2+f), z = s^2, P(z) = sum_{k=1..5} 2/(2k+1) z^(k-1).
+__device__ __forceinline__ float bm_logf(float x)
+{
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (__builtin_isinf(x)) return x;
+    uint32_t b = __float_as_uint(x);
+    int e = (int)((b >> 23) & 0xff);
+    if (e == 0) {
+        x = x * 0x1p25f;
+        b = __float_as_uint(x);
+        e = (int)((b >> 23) & 0xff) - 25;
+    }
+    e -= 127;
+    float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    float p = 0x1.745d18p-3f;
+    p = __builtin_fmaf(p, z, 0x1.c71c72p-3f);
+    p = __builtin_fmaf(p, z, 0x1.24924ap-2f);
+    p = __builtin_fmaf(p, z, 0x1.99999ap-2f);
+    p = __builtin_fmaf(p, z, 0x1.555556p-1f);
+    const float logm = __builtin_fmaf(s * z, p, 2.0f * s);
+    const float de = (float)e;
+    return __builtin_fmaf(de, 0x1.62e4p-1f, __builtin_fmaf(de, 0x1.7f7d1cp-20f, logm));
+}
+
+// sin/cos(2 pi u): q = rint(4u), t = u - q/4 in [-1/8, 1/8] (exact); series in t
+// with coefficients (2 pi)^k / k! through t^9 (sin) and t^10 (cos).
+__device__ __forceinline__ void bm_sincos2pif(float u, float& s, float& c)
+{
+    const float q = __builtin_rintf(4.0f * u);
+    const float t = u - 0.25f * q;
+    const int qi = ((int)q) & 3;
+    const float t2 = t * t;
+    float ps = 0x1.507834p+5f;
+    ps = __builtin_fmaf(ps, t2, -0x1.32d2ccp+6f);
+    ps = __builtin_fmaf(ps, t2, 0x1.466bc6p+6f);
+    ps = __builtin_fmaf(ps, t2, -0x1.4abbcep+5f);
+    const float sn = __builtin_fmaf(t * t2, ps, t * 0x1.921fb6p+2f);
+    float pc = -0x1.a6d1f2p+4f;
+    pc = __builtin_fmaf(pc, t2, 0x1.e1f506p+5f);
+    pc = __builtin_fmaf(pc, t2, -0x1.55d3c8p+6f);
+    pc = __builtin_fmaf(pc, t2, 0x1.03c1f0p+6f);
+    pc = __builtin_fmaf(pc, t2, -0x1.3bd3ccp+4f);
+    const float cs = __builtin_fmaf(t2, pc, 1.0f);
+    s = qi == 0 ? sn : qi == 1 ? cs : qi == 2 ? -sn : -cs;
+    c = qi == 0 ? cs : qi == 1 ? -sn : qi == 2 ? -cs : sn;
+}
+
+__device__ __forceinline__ void box_muller_f32(float ua, float ub, float& n0, float& n1)
+{
+    const float r = __builtin_sqrtf(-2.0f * bm_logf(ua));
+    float s, c;
+    bm_sincos2pif(ub, s, c);
+    n0 = r * s;
+    n1 = r * c;
+}
+
 // ---------------------------------------------------------- stream vectors
 // 32-bit words of one generated vector (4 words for vl 4, 2 for vl 1/2).
 template <int VARIANT, int VL>
@@ -246,12 +311,7 @@ __device__ __forceinline__ void rng_words64(uint64_t vidx, uint64_t seed, uint64
 // f32 normal vector: v[0..3] from the 4 (or 2) words.
 __device__ __forceinline__ void normals_f32(const uint32_t w[4], int pairs, float v[4])
 {
-    for (int p = 0; p < pairs; p++) {
-        double n0, n1;
-        box_muller((double)unit_f32(w[2 * p]), (double)unit_f32(w[2 * p + 1]), n0, n1);
-        v[2 * p] = (float)n0;
-        v[2 * p + 1] = (float)n1;
-    }
+    for (int p = 0; p < pairs; p++) box_muller_f32(unit_f32(w[2 * p]), unit_f32(w[2 * p + 1]), v[2 * p], v[2 * p + 1]);
 }
 
 }  // namespace chai_hip

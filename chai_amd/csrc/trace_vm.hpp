@@ -326,9 +326,9 @@ __device__ __forceinline__ uint64_t rng_elem32(const uint32_t w[4], int comp, in
     if (prec == CHAI_U32 || prec == CHAI_I32) return (uint64_t)wc;
     if (normal) {
         const uint32_t wa = comp < 2 ? w[0] : w[2], wb = comp < 2 ? w[1] : w[3];
-        double n0, n1;
-        box_muller((double)unit_f32(wa), (double)unit_f32(wb), n0, n1);
-        return bf((float)((comp & 1) ? n1 : n0));
+        float n0, n1;
+        box_muller_f32(unit_f32(wa), unit_f32(wb), n0, n1);
+        return bf((comp & 1) ? n1 : n0);
     }
     float u = unit_f32(wc);
     if (has_lim) {
@@ -393,10 +393,10 @@ __device__ __forceinline__ V4 vm_rng(const RngArgs A, const VmRngDims R, uint64_
             uint32_t w[4];
             words32(variant, 4, (uint32_t)(slot * R.H * R.vpr + y * R.vpr + x0 / 4), (uint32_t)I.u, w);
             if (normal && prec == CHAI_F32) {
-                double n0, n1, n2, n3;
-                box_muller((double)unit_f32(w[0]), (double)unit_f32(w[1]), n0, n1);
-                box_muller((double)unit_f32(w[2]), (double)unit_f32(w[3]), n2, n3);
-                r.l[0] = bf((float)n0); r.l[1] = bf((float)n1); r.l[2] = bf((float)n2); r.l[3] = bf((float)n3);
+                float n0, n1, n2, n3;
+                box_muller_f32(unit_f32(w[0]), unit_f32(w[1]), n0, n1);
+                box_muller_f32(unit_f32(w[2]), unit_f32(w[3]), n2, n3);
+                r.l[0] = bf(n0); r.l[1] = bf(n1); r.l[2] = bf(n2); r.l[3] = bf(n3);
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; i++) r.l[i] = rng_elem32(w, i, prec, 0, lim, I.d0, I.d1);

@@ -38,6 +38,8 @@ void chai_or_threefry2x64_20(const uint64_t ctr[2], const uint64_t key[2], uint6
 
 /* ---- canonical Box-Muller math (double, no contraction, explicit fma) */
 double chai_or_log(double x);
+float chai_or_logf(float x);
+void chai_or_sincos2pif(float u, float* s, float* c);
 void   chai_or_sincos2pi(double u, double* s, double* c);
 
 /* ---- canonical RNG stream (api/RngFun.cpp:15-96, kernel/Function.cpp:433-503,

@@ -34,6 +34,10 @@ def lib():
                                        ctypes.c_double, ctypes.c_double, vp]
         L.chai_or_log.restype = ctypes.c_double
         L.chai_or_log.argtypes = [ctypes.c_double]
+        L.chai_or_logf.restype = ctypes.c_float
+        L.chai_or_logf.argtypes = [ctypes.c_float]
+        L.chai_or_sincos2pif.argtypes = [ctypes.c_float, ctypes.POINTER(ctypes.c_float),
+                                         ctypes.POINTER(ctypes.c_float)]
         L.chai_or_sincos2pi.argtypes = [ctypes.c_double, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]
         L.chai_or_matmul_interp.argtypes = [i, i, i, i, i, sz, sz, sz, vp, sz, vp, sz, vp, sz,
@@ -116,6 +120,16 @@ def threefry2x64(ctr, key):
 
 def log(x):
     return lib().chai_or_log(float(x))
+
+
+def logf(x):
+    return lib().chai_or_logf(float(x))
+
+
+def sincos2pif(u):
+    s = ctypes.c_float(); c = ctypes.c_float()
+    lib().chai_or_sincos2pif(float(u), ctypes.byref(s), ctypes.byref(c))
+    return s.value, c.value
 
 
 def sincos2pi(u):

@@ -21,10 +21,10 @@
  *
  * Deliberate differences from the reference, fixed here as the canonical stream:
  *   - 2*pi is the exact double constant, not the 6-digit "3.14159" the JIT prints.
- *   - log/sin/cos are this file's own double-precision routines (series given
- *     below, no vendor math library), evaluated in double from the float/double
- *     uniforms and rounded once to the element type, so host and device agree
- *     bit for bit.
+ *   - log/sin/cos are this file's own routines (series given below, no vendor
+ *     math library): double precision for the f64 stream, single precision for
+ *     the f32 stream, every operation an IEEE op, so host and device agree bit
+ *     for bit.
  *   - The reference interpreter's RNG (interp/InterpRNG.cpp, glibc random_r,
  *     seeded from an address) is not reproducible and is not followed.
  *
@@ -270,6 +270,63 @@ void chai_or_sincos2pi(double u, double* sp, double* cp)
     *sp = s; *cp = c;
 }
 
+/* f32 normals are evaluated in f32 (the f32 stream's own routines; every
+ * operation an IEEE f32 op with explicit fmaf and correctly rounded divide /
+ * sqrt, exactly as the device code does):
+ * log(x) = e*ln2 + 2s + s z P(z), s = f/(2+f), z = s^2, P(z) = sum_{k=1..5} 2/(2k+1) z^(k-1);
+ * sin/cos(2 pi u) from q = rint(4u), t = u - q/4, series in t with coefficients
+ * (2 pi)^k / k! through t^9 / t^10 (truncation below 2^-27 for |t| <= 1/8). */
+static float f_from_bits(uint32_t b) { float f; memcpy(&f, &b, 4); return f; }
+static uint32_t f_to_bits(float f) { uint32_t b; memcpy(&b, &f, 4); return b; }
+
+float chai_or_logf(float x)
+{
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (isinf(x)) return x;
+    uint32_t b = f_to_bits(x);
+    int e = (int)((b >> 23) & 0xff);
+    if (e == 0) {
+        x = x * 0x1p25f;
+        b = f_to_bits(x);
+        e = (int)((b >> 23) & 0xff) - 25;
+    }
+    e -= 127;
+    float m = f_from_bits((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 0x1.6a09e6p+0f) { m = m * 0.5f; e += 1; }
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float p = 0x1.745d18p-3f;
+    p = fmaf(p, z, 0x1.c71c72p-3f);
+    p = fmaf(p, z, 0x1.24924ap-2f);
+    p = fmaf(p, z, 0x1.99999ap-2f);
+    p = fmaf(p, z, 0x1.555556p-1f);
+    float logm = fmaf(s * z, p, 2.0f * s);
+    float de = (float)e;
+    return fmaf(de, 0x1.62e4p-1f, fmaf(de, 0x1.7f7d1cp-20f, logm));
+}
+
+void chai_or_sincos2pif(float u, float* sp, float* cp)
+{
+    float q = rintf(4.0f * u);
+    float t = u - 0.25f * q;
+    int qi = ((int)q) & 3;
+    float t2 = t * t;
+    float ps = 0x1.507834p+5f;
+    ps = fmaf(ps, t2, -0x1.32d2ccp+6f);
+    ps = fmaf(ps, t2, 0x1.466bc6p+6f);
+    ps = fmaf(ps, t2, -0x1.4abbcep+5f);
+    float sn = fmaf(t * t2, ps, t * 0x1.921fb6p+2f);
+    float pc = -0x1.a6d1f2p+4f;
+    pc = fmaf(pc, t2, 0x1.e1f506p+5f);
+    pc = fmaf(pc, t2, -0x1.55d3c8p+6f);
+    pc = fmaf(pc, t2, 0x1.03c1f0p+6f);
+    pc = fmaf(pc, t2, -0x1.3bd3ccp+4f);
+    float cs = fmaf(t2, pc, 1.0f);
+    *sp = qi == 0 ? sn : qi == 1 ? cs : qi == 2 ? -sn : -cs;
+    *cp = qi == 0 ? cs : qi == 1 ? -sn : qi == 2 ? -cs : sn;
+}
+
 /* ------------------------------------------------------------ the stream */
 
 static size_t std_vec_len(int prec) { return prec == 8 ? 2 : 4; }
@@ -380,12 +437,10 @@ int chai_or_rng_fill(int variant, int prec, int vec_len, uint64_t seed,
             } else {
                 float v[4];
                 for (int p = 0; p < 2; p++) {
-                    double ua = (double)u32_to_unit_f32(w[2 * p]);
-                    double ub = (double)u32_to_unit_f32(w[2 * p + 1]);
-                    double r = sqrt(-2.0 * chai_or_log(ua)), sn, cs;
-                    chai_or_sincos2pi(ub, &sn, &cs);
-                    v[2 * p] = (float)(r * sn);
-                    v[2 * p + 1] = (float)(r * cs);
+                    float r = sqrtf(-2.0f * chai_or_logf(u32_to_unit_f32(w[2 * p]))), sn, cs;
+                    chai_or_sincos2pif(u32_to_unit_f32(w[2 * p + 1]), &sn, &cs);
+                    v[2 * p] = r * sn;
+                    v[2 * p + 1] = r * cs;
                 }
                 for (size_t i = 0; i < n; i++) ((float*)out)[base + i] = v[i];
             }

@@ -144,18 +144,64 @@ def test_sincos2pi_against_libm(oracle):
     assert abs(s - math.sqrt(0.5)) < 2e-16 and abs(c - math.sqrt(0.5)) < 2e-16
 
 
+def test_logf_against_libm(oracle):
+    """f32 log of the f32 stream: within 2 ulp of the correctly rounded value
+    (math.log in double, then rounded) on the uniforms' range and beyond; the
+    2-ulp cases (cancellation of e*ln2 against log m for e = -1) are rare."""
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.random(20000).astype(np.float32), (rng.random(2000) * 1e-30).astype(np.float32),
+                         np.float32([1.0, 0.5, 2.0 ** -32, 1 - 2 ** -24, 1e-45, 3e38, 1.4142135, 0.70710677])])
+    errs = []
+    for x in xs:
+        if x <= 0:
+            continue
+        got = oracle.logf(x)
+        ref = np.float32(math.log(float(x)))
+        ulp = float(np.spacing(np.abs(ref))) if ref != 0 else 1e-45
+        errs.append(abs(got - float(ref)) / ulp)
+    errs = np.array(errs)
+    assert errs.max() <= 2.0 and (errs > 1).mean() < 1e-3, (errs.max(), (errs > 1).mean())
+    assert oracle.logf(1.0) == 0.0
+    assert oracle.logf(0.0) == -math.inf
+    assert math.isnan(oracle.logf(-1.0))
+
+
+def test_sincos2pif_against_libm(oracle):
+    """f32 sin/cos(2 pi u): within 2^-23 absolute of the exact values (the
+    exactly reduced argument evaluated in double), exact at multiples of 1/4."""
+    rng = np.random.default_rng(6)
+    us = np.concatenate([rng.random(20000).astype(np.float32), np.arange(0, 1.0001, 1 / 64, dtype=np.float32)])
+    worst = 0.0
+    for u in us:
+        s, c = oracle.sincos2pif(u)
+        q = round(4 * float(u))
+        r = float(u) - q / 4
+        ss, cc = math.sin(2 * math.pi * r), math.cos(2 * math.pi * r)
+        quad = [(ss, cc), (cc, -ss), (-ss, -cc), (-cc, ss)][q % 4]
+        worst = max(worst, abs(s - quad[0]), abs(c - quad[1]))
+    assert worst <= 2.0 ** -23, worst
+    assert oracle.sincos2pif(0.0) == (0.0, 1.0)
+    assert oracle.sincos2pif(0.25) == (1.0, -0.0)
+    assert oracle.sincos2pif(0.5) == (-0.0, -1.0)
+
+
 def test_normal_stream_definition(oracle):
-    """f32 normals = (float)(sqrt(-2 ln u0) * sin/cos(2 pi u1)) evaluated in double."""
+    """f32 normals = sqrt(-2 logf(u0)) * sin/cos(2 pi u1) in f32 (the f32 routines
+    above, bit for bit), within 2^-20 r of the exact Box-Muller values."""
     u = oracle.rng_fill(1230, 4, 4, 7, 16, 4)
     n = oracle.rng_fill(1230, 4, 4, 7, 16, 4, normal=True)
-    flat_u = u.reshape(-1, 4).astype(np.float64)
+    flat_u = u.reshape(-1, 4)
     flat_n = n.reshape(-1, 4)
     for uu, nn in zip(flat_u, flat_n):
         for p in range(2):
-            r = math.sqrt(-2.0 * math.log(uu[2 * p]))
-            s, c = math.sin(2 * math.pi * uu[2 * p + 1]), math.cos(2 * math.pi * uu[2 * p + 1])
-            assert abs(nn[2 * p] - r * s) <= 1e-6 * max(1.0, abs(r))
-            assert abs(nn[2 * p + 1] - r * c) <= 1e-6 * max(1.0, abs(r))
+            r32 = np.sqrt(np.float32(-2.0) * np.float32(oracle.logf(uu[2 * p])), dtype=np.float32)
+            s32, c32 = oracle.sincos2pif(uu[2 * p + 1])
+            assert nn[2 * p] == np.float32(r32 * np.float32(s32))
+            assert nn[2 * p + 1] == np.float32(r32 * np.float32(c32))
+            r = math.sqrt(-2.0 * math.log(float(uu[2 * p])))
+            s, c = math.sin(2 * math.pi * float(uu[2 * p + 1])), math.cos(2 * math.pi * float(uu[2 * p + 1]))
+            assert abs(nn[2 * p] - r * s) <= 2.0 ** -20 * max(1.0, r)
+            assert abs(nn[2 * p + 1] - r * c) <= 2.0 ** -20 * max(1.0, r)
 
 
 def test_normal_statistics(oracle):

@@ -85,3 +85,17 @@ def test_emitted_md5_matches_hashlib(tmp_path):
     got, ref = emulate(t, data, tmp_path)
     digests = md5_hex([got[v.index].reshape(-1) for v in outs])
     assert digests == md5_expected(MSGS)
+
+
+@pytest.mark.parametrize("vl", [0, 1, 2])
+def test_emitted_rng_normals_match_oracle(vl, tmp_path):
+    """f32 and f64 Box-Muller normals of the device code (host build) equal the
+    oracle's stream bit for bit, including vector lengths 1 and 2."""
+    from chai_amd import F32, F64
+    from chai_amd.trace import Trace
+    t = Trace()
+    a = t.assign(t.output(F32, 1001, 3, 2), Trace.rng_normal(F32, 1001, 3, 2, seed=7, vec_len=vl))
+    b = t.assign(t.output(F64, 1001, 3, 2), Trace.rng_normal(F64, 1001, 3, 2, seed=9, vec_len=min(vl, 2) or 0))
+    got, ref = emulate(t, {}, tmp_path)
+    np.testing.assert_array_equal(got[a.index], ref[a.index])
+    np.testing.assert_array_equal(got[b.index], ref[b.index])
