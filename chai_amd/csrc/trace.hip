@@ -189,7 +189,9 @@ int prec_of_suffix(int k) { return k == 0 ? CHAI_U32 : k == 1 ? CHAI_I32 : k == 
 bool is_unary(uint32_t c, bool& fp_only)
 {
     static const uint32_t fp1[] = {223, 261, 256, 219, 220, 221, 222, 237, 238, 239, 240, 257, 214, 262,
-                                   203, 200, 206, 259, 215, 263, 212, 253, 255, 266, 211, 218, 217};
+                                   203, 200, 206, 259, 215, 263, 212, 253, 255, 266, 211, 218, 217,
+                                   201, 202, 204, 205, 208, 209, 216, 233, 235, 241, 246, 260, 264, 265,
+                                   301, 305, 308, 507, 508, 509, 510, 513};
     fp_only = false;
     if (c == 10 || c == 15 || c == 16 || c == 400 || c == 404 || c == 225) return true;
     for (uint32_t f : fp1)
@@ -200,7 +202,8 @@ bool is_unary(uint32_t c, bool& fp_only)
 bool is_binary(uint32_t c, int& kind)   // kind 0: any, 1: fp only, 2: int only
 {
     static const uint32_t any[] = {11, 12, 13, 14, 32, 33, 34, 35, 36, 37, 302, 303};
-    static const uint32_t fp[] = {227, 228, 248, 207, 229, 232, 213, 224};
+    static const uint32_t fp[] = {227, 228, 248, 207, 229, 232, 213, 224, 210, 234, 243, 244, 247, 249,
+                                  250, 251, 254, 306, 500, 501, 502, 503, 504, 505, 506, 511, 512};
     static const uint32_t in[] = {17, 18, 19, 20, 21, 22, 30, 31, 401, 402, 405, 411, 412, 413, 414, 415};
     for (uint32_t x : any) if (c == x) { kind = 0; return true; }
     for (uint32_t x : fp) if (c == x) { kind = 1; return true; }
@@ -208,7 +211,11 @@ bool is_binary(uint32_t c, int& kind)   // kind 0: any, 1: fp only, 2: int only
     return false;
 }
 
-bool is_ternary(uint32_t c) { return c == 23 || c == 300 || c == 226 || c == 242; }
+bool is_ternary(uint32_t c)
+{
+    return c == 23 || c == 300 || c == 226 || c == 242 || c == 304 || c == 307 || c == 406 || c == 407 ||
+           c == 408;
+}
 
 struct Dims {
     uint64_t W = 1, H = 1, S = 1;
@@ -379,7 +386,10 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
                 e.prec = a.prec > b.prec ? a.prec : b.prec;
                 e.prec = e.prec > c.prec ? e.prec : c.prec;
                 e.d = dmax(dmax(a.d, b.d), c.d);
-                if ((n.code == 226 || n.code == 242) && !is_fp(e.prec)) return fail("fma/mad need floating point");
+                if ((n.code == 226 || n.code == 242 || n.code == 304 || n.code == 307) && !is_fp(e.prec))
+                    return fail("bytecode %u needs floating point", n.code);
+                if ((n.code == 406 || n.code == 407 || n.code == 408) && is_fp(e.prec))
+                    return fail("bytecode %u needs integers", n.code);
             }
         }
         s.nodes.push_back(e);

@@ -133,9 +133,63 @@ __device__ __forceinline__ uint32_t shl32(uint32_t a, uint32_t n) { return a << 
 __device__ __forceinline__ uint32_t shr32u(uint32_t a, uint32_t n) { return a >> (n & 31u); }
 __device__ __forceinline__ int32_t shr32s(int32_t a, uint32_t n) { return a >> (n & 31u); }
 
+// Ops whose device math is large (transcendentals other than exp / log and
+// the remaining SimpleFun1 functions) are kept out of line in the VM kernel,
+// whose generic dispatch would otherwise inline every one of them; the JIT
+// (one op per call site, known at compile time) inlines everything.
+#ifdef __HIPCC_RTC__
+#define CHAI_VM_HEAVY __forceinline__
+#else
+#define CHAI_VM_HEAVY __noinline__
+#endif
+
+// pown in double: square-and-multiply from the low bit, reciprocal for n < 0
+// (what the reference's pow(double, int) resolves to in its C++98 build)
+__device__ __forceinline__ double powi_sq(double x, int m)
+{
+    unsigned n = m < 0 ? 0u - (unsigned)m : (unsigned)m;
+    double y = (n & 1u) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n & 1u) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
+// pi and 1/pi as the reference's M_PI / M_1_PI (double)
+constexpr double VM_PI = 0x1.921fb54442d18p+1;
+constexpr double VM_1_PI = 0x1.45f306dc9c883p-2;
+
+template <typename T>
+__device__ CHAI_VM_HEAVY T fp_bin_heavy(uint32_t op, T a, T b)
+{
+    switch (op) {
+        case 248: return sizeof(T) == 4 ? (T)powf((float)a, (float)b) : (T)pow((double)a, (double)b);
+        case 207: return sizeof(T) == 4 ? (T)atan2f((float)a, (float)b) : (T)atan2((double)a, (double)b);
+        case 229: return sizeof(T) == 4 ? (T)fmodf((float)a, (float)b) : (T)fmod((double)a, (double)b);
+        case 232: return sizeof(T) == 4 ? (T)hypotf((float)a, (float)b) : (T)hypot((double)a, (double)b);
+        case 210: return sizeof(T) == 4 ? (T)((double)atan2f((float)a, (float)b) / VM_PI)
+                                        : (T)(atan2((double)a, (double)b) / VM_PI);
+        case 234: return sizeof(T) == 4 ? (T)ldexpf((float)a, (int)b) : (T)ldexp((double)a, (int)b);
+        case 247: return sizeof(T) == 4 ? (T)nextafterf((float)a, (float)b) : (T)nextafter((double)a, (double)b);
+        case 249: return sizeof(T) == 4 ? (T)powf((float)a, (float)(int)b) : (T)powi_sq((double)a, (int)b);
+        case 250: return sizeof(T) == 4 ? (T)powf((float)a, (float)b) : (T)pow((double)a, (double)b);
+        case 251: return sizeof(T) == 4 ? (T)remainderf((float)a, (float)b) : (T)remainder((double)a, (double)b);
+        case 254: return sizeof(T) == 4 ? (T)powf((float)a, 1.0f / (float)b) : (T)pow((double)a, 1.0 / (double)b);
+        default: return (T)0;
+    }
+}
+
+__device__ __forceinline__ bool bin_is_heavy(uint32_t op)
+{
+    return op == 248 || op == 207 || op == 229 || op == 232 || op == 210 || op == 234 || op == 247 ||
+           op == 249 || op == 250 || op == 251 || op == 254;
+}
+
 template <typename T>
 __device__ __forceinline__ T fp_bin(uint32_t op, T a, T b)
 {
+    if (bin_is_heavy(op)) return fp_bin_heavy<T>(op, a, b);
     switch (op) {
         case 11: return a + b;
         case 12: return a / b;
@@ -148,15 +202,34 @@ __device__ __forceinline__ T fp_bin(uint32_t op, T a, T b)
         case 36: return (T)(a <= b);
         case 37: return (T)(a < b);
         case 302: return a < b ? b : a;
-        case 303: return b < a ? b : a;
+        case 303: return a < b ? a : b;
         case 227: return sizeof(T) == 4 ? (T)fmaxf((float)a, (float)b) : (T)fmax((double)a, (double)b);
         case 228: return sizeof(T) == 4 ? (T)fminf((float)a, (float)b) : (T)fmin((double)a, (double)b);
-        case 248: return sizeof(T) == 4 ? (T)powf((float)a, (float)b) : (T)pow((double)a, (double)b);
-        case 207: return sizeof(T) == 4 ? (T)atan2f((float)a, (float)b) : (T)atan2((double)a, (double)b);
-        case 229: return sizeof(T) == 4 ? (T)fmodf((float)a, (float)b) : (T)fmod((double)a, (double)b);
-        case 232: return sizeof(T) == 4 ? (T)hypotf((float)a, (float)b) : (T)hypot((double)a, (double)b);
         case 213: return sizeof(T) == 4 ? (T)copysignf((float)a, (float)b) : (T)copysign((double)a, (double)b);
         case 224: return sizeof(T) == 4 ? (T)fdimf((float)a, (float)b) : (T)fdim((double)a, (double)b);
+        // misc/SimpleFun2.cpp definitions (the pi variant divides in double)
+        case 243: {
+            const T fa = a < 0 ? -a : a, fb = b < 0 ? -b : b;
+            if (fa > fb) return a;
+            if (fa < fb) return b;
+            return sizeof(T) == 4 ? (T)fmaxf((float)a, (float)b) : (T)fmax((double)a, (double)b);
+        }
+        case 244: {
+            const T fa = a < 0 ? -a : a, fb = b < 0 ? -b : b;
+            if (fa < fb) return a;
+            if (fa > fb) return b;
+            return sizeof(T) == 4 ? (T)fminf((float)a, (float)b) : (T)fmin((double)a, (double)b);
+        }
+        case 306: return b < a ? (T)0 : (T)1;
+        case 500: return (T)(a == b);
+        case 501: return (T)(a != b);
+        case 502: return (T)(a > b);
+        case 503: return (T)(a >= b);
+        case 504: return (T)(a < b);
+        case 505: return (T)(a <= b);
+        case 506: return (T)((a < b) || (a > b));
+        case 511: return (T)(a == a && b == b);
+        case 512: return (T)(a != a || b != b);
         default: return (T)0;
     }
 }
@@ -233,21 +306,19 @@ __device__ __forceinline__ int32_t i_bin(uint32_t op, int32_t a, int32_t b)
     }
 }
 
+__device__ __forceinline__ bool un_is_light(uint32_t op)
+{
+    return op == 10 || op == 223 || op == 261 || op == 256 || op == 219 || op == 220 || op == 237 ||
+           op == 238 || op == 225 || op == 212 || op == 253 || op == 255 || op == 266;
+}
+
 template <typename T>
-__device__ __forceinline__ T fp_un(uint32_t op, T a)
+__device__ CHAI_VM_HEAVY T fp_un_heavy(uint32_t op, T a)
 {
     const bool f = sizeof(T) == 4;
     switch (op) {
-        case 10: return -a;
-        case 223: return f ? (T)fabsf((float)a) : (T)fabs((double)a);
-        case 261: return f ? (T)sqrtf((float)a) : (T)sqrt((double)a);
-        case 256: return f ? (T)(1.0f / sqrtf((float)a)) : (T)(1.0 / sqrt((double)a));
-        case 219: return f ? (T)expf((float)a) : (T)exp((double)a);
-        case 220: return f ? (T)exp2f((float)a) : (T)exp2((double)a);
         case 221: return f ? (T)exp10f((float)a) : (T)exp10((double)a);
         case 222: return f ? (T)expm1f((float)a) : (T)expm1((double)a);
-        case 237: return f ? (T)logf((float)a) : (T)log((double)a);
-        case 238: return f ? (T)log2f((float)a) : (T)log2((double)a);
         case 239: return f ? (T)log10f((float)a) : (T)log10((double)a);
         case 240: return f ? (T)log1pf((float)a) : (T)log1p((double)a);
         case 257: return f ? (T)sinf((float)a) : (T)sin((double)a);
@@ -259,14 +330,55 @@ __device__ __forceinline__ T fp_un(uint32_t op, T a)
         case 259: return f ? (T)sinhf((float)a) : (T)sinh((double)a);
         case 215: return f ? (T)coshf((float)a) : (T)cosh((double)a);
         case 263: return f ? (T)tanhf((float)a) : (T)tanh((double)a);
+        case 211: return f ? (T)cbrtf((float)a) : (T)cbrt((double)a);
+        case 218: return f ? (T)erff((float)a) : (T)erf((double)a);
+        case 217: return f ? (T)erfcf((float)a) : (T)erfc((double)a);
+        // misc/SimpleFun1.cpp definitions (pi variants, degrees, radians in double)
+        case 201: return f ? (T)acoshf((float)a) : (T)acosh((double)a);
+        case 202: return f ? (T)((double)acosf((float)a) / VM_PI) : (T)(acos((double)a) / VM_PI);
+        case 204: return f ? (T)asinhf((float)a) : (T)asinh((double)a);
+        case 205: return f ? (T)((double)asinf((float)a) / VM_PI) : (T)(asin((double)a) / VM_PI);
+        case 208: return f ? (T)atanhf((float)a) : (T)atanh((double)a);
+        case 209: return f ? (T)((double)atanf((float)a) / VM_PI) : (T)(atan((double)a) / VM_PI);
+        case 216: return f ? (T)cosf((float)(VM_PI * (double)a)) : (T)cos(VM_PI * (double)a);
+        case 233: return f ? (T)ilogbf((float)a) : (T)ilogb((double)a);
+        case 235: return f ? (T)lgammaf((float)a) : (T)lgamma((double)a);
+        case 241: return f ? (T)logbf((float)a) : (T)logb((double)a);
+        case 246: return (T)__builtin_nan("");
+        case 260: return f ? (T)sinf((float)(VM_PI * (double)a)) : (T)sin(VM_PI * (double)a);
+        case 264: return f ? (T)tanf((float)(VM_PI * (double)a)) : (T)tan(VM_PI * (double)a);
+        case 265: return f ? (T)tgammaf((float)a) : (T)tgamma((double)a);
+        case 301: return (T)(VM_1_PI * 180 * (double)a);
+        case 305: return (T)((double)a * VM_PI / 180);
+        case 308: return a > 0 ? (T)1 : a < 0 ? (T)-1 : a;
+        case 507: return (T)(__builtin_isfinite(a) ? 1 : 0);
+        case 508: return (T)(__builtin_isinf(a) ? 1 : 0);
+        case 509: return (T)(a != a ? 1 : 0);
+        case 510: return (T)(__builtin_isnormal(a) ? 1 : 0);
+        case 513: return __builtin_signbit(a) ? (sizeof(T) == 4 ? (T)-2147483648.0f : (T)1) : (T)0;
+        default: return (T)0;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T fp_un(uint32_t op, T a)
+{
+    if (!un_is_light(op)) return fp_un_heavy<T>(op, a);
+    const bool f = sizeof(T) == 4;
+    switch (op) {
+        case 10: return -a;
+        case 223: return f ? (T)fabsf((float)a) : (T)fabs((double)a);
+        case 261: return f ? (T)sqrtf((float)a) : (T)sqrt((double)a);
+        case 256: return f ? (T)(1.0f / sqrtf((float)a)) : (T)(1.0 / sqrt((double)a));
+        case 219: return f ? (T)expf((float)a) : (T)exp((double)a);
+        case 220: return f ? (T)exp2f((float)a) : (T)exp2((double)a);
+        case 237: return f ? (T)logf((float)a) : (T)log((double)a);
+        case 238: return f ? (T)log2f((float)a) : (T)log2((double)a);
         case 225: return f ? (T)floorf((float)a) : (T)floor((double)a);
         case 212: return f ? (T)ceilf((float)a) : (T)ceil((double)a);
         case 253: return f ? (T)rintf((float)a) : (T)rint((double)a);
         case 255: return f ? (T)roundf((float)a) : (T)round((double)a);
         case 266: return f ? (T)truncf((float)a) : (T)trunc((double)a);
-        case 211: return f ? (T)cbrtf((float)a) : (T)cbrt((double)a);
-        case 218: return f ? (T)erff((float)a) : (T)erf((double)a);
-        case 217: return f ? (T)erfcf((float)a) : (T)erfc((double)a);
         default: return (T)0;
     }
 }
@@ -274,6 +386,7 @@ __device__ __forceinline__ T fp_un(uint32_t op, T a)
 __device__ __forceinline__ uint32_t u_un(uint32_t op, uint32_t a)
 {
     switch (op) {
+        case 10: return 0u - a;
         case 15: return ~a;
         case 16: return (uint32_t)(!a);
         case 400: return a;
@@ -306,7 +419,40 @@ __device__ __forceinline__ T ter(uint32_t op, T a, T b, T c)
             else if constexpr ((T)0.5 != 0) return (T)__builtin_fma((double)a, (double)b, (double)c);
             else return (T)0;
         case 242: return a * b + c;
+        case 304: { const T d = b - a; const T e = d * c; return a + e; }   // mix
+        case 307: {                                                           // smoothstep(e0, e1, x)
+            const T q = (c - a) / (b - a);
+            const T q0 = q < 0 ? (T)0 : q, t = q0 < 1 ? q0 : (T)1;
+            const T t2 = t * t, u = (T)3 - (T)2 * t;
+            return t2 * u;
+        }
         default: return (T)0;
+    }
+}
+
+// integer ternary ops (misc/SimpleFun3.cpp): clamp, mad24, mad_hi, mad_sat
+__device__ __forceinline__ uint32_t u_ter(uint32_t op, uint32_t a, uint32_t b, uint32_t c)
+{
+    switch (op) {
+        case 300: { const uint32_t m = a < b ? b : a; return m < c ? m : c; }
+        case 406: return u_bin(411, a, b) + c;
+        case 407: return u_bin(412, a, b) + c;
+        case 408: { const uint64_t d = (uint64_t)a * b + c; return d > 0xffffffffull ? 0xffffffffu : (uint32_t)d; }
+        default: return 0u;
+    }
+}
+
+__device__ __forceinline__ int32_t i_ter(uint32_t op, int32_t a, int32_t b, int32_t c)
+{
+    switch (op) {
+        case 300: { const int32_t m = a < b ? b : a; return m < c ? m : c; }
+        case 406: return (int32_t)((uint32_t)i_bin(411, a, b) + (uint32_t)c);
+        case 407: return (int32_t)((uint32_t)i_bin(412, a, b) + (uint32_t)c);
+        case 408: {
+            const long long d = (long long)a * b + c;
+            return d < INT32_MIN ? INT32_MIN : d > INT32_MAX ? INT32_MAX : (int32_t)d;
+        }
+        default: return 0;
     }
 }
 
@@ -537,11 +683,11 @@ __device__ __forceinline__ V4 vm_ter(uint32_t op, int prec, const V4& a, const V
             break;
         case CHAI_I32:
 #pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bi(ter<int32_t>(op, i_(a.l[i]), i_(b.l[i]), i_(c.l[i])));
+            for (int i = 0; i < 4; i++) r.l[i] = bi(i_ter(op, i_(a.l[i]), i_(b.l[i]), i_(c.l[i])));
             break;
         default:
 #pragma unroll
-            for (int i = 0; i < 4; i++) r.l[i] = bu(ter<uint32_t>(op, u_(a.l[i]), u_(b.l[i]), u_(c.l[i])));
+            for (int i = 0; i < 4; i++) r.l[i] = bu(u_ter(op, u_(a.l[i]), u_(b.l[i]), u_(c.l[i])));
             break;
     }
     return r;

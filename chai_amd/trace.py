@@ -114,6 +114,11 @@ sin, cos, tan, tanh, fabs, floor, ceil = _u(257), _u(214), _u(262), _u(263), _u(
 rint, round_, trunc, erf, erfc, expm1, log1p = _u(253), _u(255), _u(266), _u(218), _u(217), _u(222), _u(240)
 iabs, clz, lnot = _u(400), _u(404), _u(16)
 sum_, mean = _u(132), _u(131)
+asin, acos, atan, sinh, cosh, cbrt, exp10 = _u(203), _u(200), _u(206), _u(259), _u(215), _u(211), _u(221)
+acosh, acospi, asinh, asinpi, atanh, atanpi = _u(201), _u(202), _u(204), _u(205), _u(208), _u(209)
+cospi, sinpi, tanpi, ilogb, lgamma, logb, tgamma = _u(216), _u(260), _u(264), _u(233), _u(235), _u(241), _u(265)
+nan, degrees, radians, sign = _u(246), _u(301), _u(305), _u(308)
+isfinite, isinf, isnan, isnormal, signbit = _u(507), _u(508), _u(509), _u(510), _u(513)
 convert_u32, convert_i32, convert_f32, convert_f64 = _u(1), _u(2), _u(3), _u(4)
 
 
@@ -125,12 +130,21 @@ maxe, mine, pow_, atan2, fmod, hypot = _b(302), _b(303), _b(248), _b(207), _b(22
 land, lor, mul_hi, rotate, add_sat, sub_sat = _b(30), _b(31), _b(412), _b(414), _b(402), _b(415)
 hadd, rhadd, abs_diff, mul24, fmaxe, fmine = _b(405), _b(413), _b(401), _b(411), _b(227), _b(228)
 dot = _b(130)
+copysign, fdim, atan2pi, ldexp, maxmag, minmag = _b(213), _b(224), _b(210), _b(234), _b(243), _b(244)
+nextafter, pown, powr, remainder, rootn, step = _b(247), _b(249), _b(250), _b(251), _b(254), _b(306)
+isequal, isnotequal, isgreater, isgreaterequal = _b(500), _b(501), _b(502), _b(503)
+isless, islessequal, islessgreater, isordered, isunordered = _b(504), _b(505), _b(506), _b(511), _b(512)
 
 
 def cond(c, a, b): return Expr(23, [lit(c), lit(a), lit(b)])
 def clamp(a, lo, hi): return Expr(300, [lit(a), lit(lo), lit(hi)])
 def fma(a, b, c): return Expr(226, [lit(a), lit(b), lit(c)])
 def mad(a, b, c): return Expr(242, [lit(a), lit(b), lit(c)])
+def mix(a, b, c): return Expr(304, [lit(a), lit(b), lit(c)])
+def smoothstep(e0, e1, x): return Expr(307, [lit(e0), lit(e1), lit(x)])
+def mad24(a, b, c): return Expr(406, [lit(a), lit(b), lit(c)])
+def mad_hi(a, b, c): return Expr(407, [lit(a), lit(b), lit(c)])
+def mad_sat(a, b, c): return Expr(408, [lit(a), lit(b), lit(c)])
 def matmul(a, b): return Expr(120, [a, b])
 def matmul_g(a, b, c, alpha=1.0, beta=1.0): return Expr(121, [a, b, c], s0=alpha, s1=beta)
 

@@ -75,6 +75,12 @@ def ref_lib():
                                     ctypes.c_void_p, ctypes.c_size_t]
     L.chai_ref_fun2_f32.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_size_t]
+    if hasattr(L, "chai_ref_fun3"):
+        L.chai_ref_fun1_fp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_size_t]
+        L.chai_ref_fun2_fp.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_size_t]
+        L.chai_ref_fun3.argtypes = [ctypes.c_int, ctypes.c_int] + [ctypes.c_void_p] * 4 + [ctypes.c_size_t]
     return L
 
 

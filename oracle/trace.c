@@ -30,6 +30,7 @@
  * (tests/test_oracle_ref.py); closed forms (MD5 of known messages as an
  * integer trace, sums of index ramps) in tests/test_oracle_trace.py.
  */
+#define _GNU_SOURCE
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -170,6 +171,24 @@ static int bin_int(uint32_t op, int is_signed, uint32_t ua, uint32_t ub, uint32_
 }
 
 /* floating point binary op in precision p (4: float arithmetic, 8: double) */
+/* pown in double: the reference's pow(double, int) call resolves, in the
+ * C++98 library it is built with, to repeated squaring (x^|n| by
+ * square-and-multiply from the low bit, reciprocal for n < 0) rather than to
+ * pow(double, double); the float overload calls powf. */
+static double powi_sq(double x, int m)
+{
+    unsigned n = m < 0 ? 0u - (unsigned)m : (unsigned)m;
+    double y = (n & 1u) ? x : 1.0;
+    while (n >>= 1) {
+        x = x * x;
+        if (n & 1u) y = y * x;
+    }
+    return m < 0 ? 1.0 / y : y;
+}
+
+/* signbit: in that build the float overload returns the raw sign bit
+ * (0x80000000 as an int, i.e. -2^31 once converted back to float), the
+ * double overload 1 -- pinned against oracle/_ref. */
 static int bin_fp(uint32_t op, int p, double a, double b, double* r)
 {
     if (p == 4) {
@@ -187,7 +206,7 @@ static int bin_fp(uint32_t op, int p, double a, double b, double* r)
             case 36: v = fa <= fb; break;
             case 37: v = fa < fb; break;
             case 302: v = fa < fb ? fb : fa; break;
-            case 303: v = fb < fa ? fb : fa; break;
+            case 303: v = fa < fb ? fa : fb; break;     /* SimpleFun2.cpp:349-356 */
             case 227: v = fmaxf(fa, fb); break;
             case 228: v = fminf(fa, fb); break;
             case 248: v = powf(fa, fb); break;
@@ -196,6 +215,25 @@ static int bin_fp(uint32_t op, int p, double a, double b, double* r)
             case 232: v = hypotf(fa, fb); break;
             case 213: v = copysignf(fa, fb); break;
             case 224: v = fdimf(fa, fb); break;
+            case 210: v = (float)(atan2f(fa, fb) / M_PI); break;
+            case 234: v = ldexpf(fa, (int)fb); break;
+            case 243: v = fabsf(fa) > fabsf(fb) ? fa : fabsf(fa) < fabsf(fb) ? fb : fmaxf(fa, fb); break;
+            case 244: v = fabsf(fa) < fabsf(fb) ? fa : fabsf(fa) > fabsf(fb) ? fb : fminf(fa, fb); break;
+            case 247: v = nextafterf(fa, fb); break;
+            case 249: v = powf(fa, (float)(int)fb); break;
+            case 250: v = powf(fa, fb); break;
+            case 251: v = remainderf(fa, fb); break;
+            case 254: v = powf(fa, 1.f / fb); break;
+            case 306: v = fb < fa ? 0.0f : 1.0f; break;
+            case 500: v = fa == fb; break;
+            case 501: v = fa != fb; break;
+            case 502: v = fa > fb; break;
+            case 503: v = fa >= fb; break;
+            case 504: v = fa < fb; break;
+            case 505: v = fa <= fb; break;
+            case 506: v = (fa < fb) || (fa > fb); break;
+            case 511: v = fa == fa && fb == fb; break;
+            case 512: v = isnan(fa) || isnan(fb); break;
             default: return -1;
         }
         *r = v;
@@ -213,7 +251,7 @@ static int bin_fp(uint32_t op, int p, double a, double b, double* r)
         case 36: *r = a <= b; return 0;
         case 37: *r = a < b; return 0;
         case 302: *r = a < b ? b : a; return 0;
-        case 303: *r = b < a ? b : a; return 0;
+        case 303: *r = a < b ? a : b; return 0;
         case 227: *r = fmax(a, b); return 0;
         case 228: *r = fmin(a, b); return 0;
         case 248: *r = pow(a, b); return 0;
@@ -222,6 +260,25 @@ static int bin_fp(uint32_t op, int p, double a, double b, double* r)
         case 232: *r = hypot(a, b); return 0;
         case 213: *r = copysign(a, b); return 0;
         case 224: *r = fdim(a, b); return 0;
+        case 210: *r = atan2(a, b) / M_PI; return 0;
+        case 234: *r = ldexp(a, (int)b); return 0;
+        case 243: *r = fabs(a) > fabs(b) ? a : fabs(a) < fabs(b) ? b : fmax(a, b); return 0;
+        case 244: *r = fabs(a) < fabs(b) ? a : fabs(a) > fabs(b) ? b : fmin(a, b); return 0;
+        case 247: *r = nextafter(a, b); return 0;
+        case 249: *r = powi_sq(a, (int)b); return 0;
+        case 250: *r = pow(a, b); return 0;
+        case 251: *r = remainder(a, b); return 0;
+        case 254: *r = pow(a, 1.0 / b); return 0;
+        case 306: *r = b < a ? 0.0 : 1.0; return 0;
+        case 500: *r = a == b; return 0;
+        case 501: *r = a != b; return 0;
+        case 502: *r = a > b; return 0;
+        case 503: *r = a >= b; return 0;
+        case 504: *r = a < b; return 0;
+        case 505: *r = a <= b; return 0;
+        case 506: *r = (a < b) || (a > b); return 0;
+        case 511: *r = a == a && b == b; return 0;
+        case 512: *r = isnan(a) || isnan(b); return 0;
     }
     return -1;
 }
@@ -238,7 +295,7 @@ static int un_fp(uint32_t op, int p, double a, double* r)
             case 256: v = 1.0f / sqrtf(f); break;
             case 219: v = expf(f); break;
             case 220: v = exp2f(f); break;
-            case 221: v = powf(10.0f, f); break;
+            case 221: v = exp10f(f); break;
             case 222: v = expm1f(f); break;
             case 237: v = logf(f); break;
             case 238: v = log2f(f); break;
@@ -261,6 +318,29 @@ static int un_fp(uint32_t op, int p, double a, double* r)
             case 211: v = cbrtf(f); break;
             case 218: v = erff(f); break;
             case 217: v = erfcf(f); break;
+            /* SimpleFun1.cpp: the pi variants and degrees / radians go through double */
+            case 201: v = acoshf(f); break;
+            case 202: v = (float)(acosf(f) / M_PI); break;
+            case 204: v = asinhf(f); break;
+            case 205: v = (float)(asinf(f) / M_PI); break;
+            case 208: v = atanhf(f); break;
+            case 209: v = (float)(atanf(f) / M_PI); break;
+            case 216: v = cosf((float)(M_PI * f)); break;
+            case 233: v = (float)ilogbf(f); break;
+            case 235: v = lgammaf(f); break;
+            case 241: v = logbf(f); break;
+            case 246: v = (float)nan(""); break;
+            case 260: v = sinf((float)(M_PI * f)); break;
+            case 264: v = tanf((float)(M_PI * f)); break;
+            case 265: v = tgammaf(f); break;
+            case 301: v = (float)(M_1_PI * 180 * f); break;
+            case 305: v = (float)(f * M_PI / 180); break;
+            case 308: v = f > 0.0 ? 1.0f : f < 0.0 ? -1.0f : f; break;
+            case 507: v = isfinite(f) ? 1.0f : 0.0f; break;
+            case 508: v = isinf(f) ? 1.0f : 0.0f; break;
+            case 509: v = isnan(f) ? 1.0f : 0.0f; break;
+            case 510: v = isnormal(f) ? 1.0f : 0.0f; break;
+            case 513: v = signbit(f) ? -2147483648.0f : 0.0f; break;   /* see below */
             default: return -1;
         }
         *r = v;
@@ -273,7 +353,7 @@ static int un_fp(uint32_t op, int p, double a, double* r)
         case 256: *r = 1.0 / sqrt(a); return 0;
         case 219: *r = exp(a); return 0;
         case 220: *r = exp2(a); return 0;
-        case 221: *r = pow(10.0, a); return 0;
+        case 221: *r = exp10(a); return 0;
         case 222: *r = expm1(a); return 0;
         case 237: *r = log(a); return 0;
         case 238: *r = log2(a); return 0;
@@ -296,6 +376,28 @@ static int un_fp(uint32_t op, int p, double a, double* r)
         case 211: *r = cbrt(a); return 0;
         case 218: *r = erf(a); return 0;
         case 217: *r = erfc(a); return 0;
+        case 201: *r = acosh(a); return 0;
+        case 202: *r = acos(a) / M_PI; return 0;
+        case 204: *r = asinh(a); return 0;
+        case 205: *r = asin(a) / M_PI; return 0;
+        case 208: *r = atanh(a); return 0;
+        case 209: *r = atan(a) / M_PI; return 0;
+        case 216: *r = cos(M_PI * a); return 0;
+        case 233: *r = (double)ilogb(a); return 0;
+        case 235: *r = lgamma(a); return 0;
+        case 241: *r = logb(a); return 0;
+        case 246: *r = nan(""); return 0;
+        case 260: *r = sin(M_PI * a); return 0;
+        case 264: *r = tan(M_PI * a); return 0;
+        case 265: *r = tgamma(a); return 0;
+        case 301: *r = M_1_PI * 180 * a; return 0;
+        case 305: *r = a * M_PI / 180; return 0;
+        case 308: *r = a > 0.0 ? 1.0 : a < 0.0 ? -1.0 : a; return 0;
+        case 507: *r = isfinite(a) ? 1.0 : 0.0; return 0;
+        case 508: *r = isinf(a) ? 1.0 : 0.0; return 0;
+        case 509: *r = isnan(a) ? 1.0 : 0.0; return 0;
+        case 510: *r = isnormal(a) ? 1.0 : 0.0; return 0;
+        case 513: *r = signbit(a) ? 1.0 : 0.0; return 0;
     }
     return -1;
 }
@@ -445,8 +547,11 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                     case 219: case 220: case 221: case 222: case 237: case 238: case 239: case 240: case 257:
                     case 214: case 262: case 203: case 200: case 206: case 259: case 215: case 263: case 212:
                     case 253: case 255: case 266: case 211: case 218: case 217:
+                    case 201: case 202: case 204: case 205: case 208: case 209: case 216: case 233:
+                    case 235: case 241: case 246: case 260: case 264: case 265: case 301: case 305:
+                    case 308: case 507: case 508: case 509: case 510: case 513:
                         arity = 1; break;
-                    case 300: case 226: case 242:
+                    case 300: case 226: case 242: case 304: case 307: case 406: case 407: case 408:
                         arity = 3; break;
                     default:
                         arity = 2; break;
@@ -485,6 +590,34 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                                 } else if (c == 242 && p >= 4) {
                                     if (p == 4) { float m = (float)d[0] * (float)d[1]; rd = m + (float)d[2]; }
                                     else { double m = d[0] * d[1]; rd = m + d[2]; }
+                                } else if (c == 304 && p >= 4) {        /* mix: a + (b - a) * c */
+                                    if (p == 4) { float t1 = (float)d[1] - (float)d[0]; float t2 = t1 * (float)d[2];
+                                                  rd = (float)d[0] + t2; }
+                                    else rd = d[0] + (d[1] - d[0]) * d[2];
+                                } else if (c == 307 && p >= 4) {        /* smoothstep(edge0, edge1, x) */
+                                    if (p == 4) {
+                                        const float a0 = (float)d[0], b0 = (float)d[1], x0 = (float)d[2];
+                                        const float q = (x0 - a0) / (b0 - a0);
+                                        const float q0 = q < 0.0 ? 0.0f : q, tt = q0 < 1.0 ? q0 : 1.0f;
+                                        const float t2 = tt * tt, u3 = 3 - 2 * tt;
+                                        rd = t2 * u3;
+                                    } else {
+                                        const double q = (d[2] - d[0]) / (d[1] - d[0]);
+                                        const double q0 = q < 0.0 ? 0.0 : q, tt = q0 < 1.0 ? q0 : 1.0;
+                                        rd = tt * tt * (3 - 2 * tt);
+                                    }
+                                } else if ((c == 406 || c == 407) && p <= 1) {   /* mad24 / mad_hi */
+                                    uint32_t m;
+                                    bin_int(c == 406 ? 411 : 412, p == 1, u[0], u[1], &m);
+                                    ru = m + u[2];
+                                } else if (c == 408 && p <= 1) {                 /* mad_sat */
+                                    if (p == 0) {
+                                        const uint64_t dd = (uint64_t)u[0] * u[1] + u[2];
+                                        ru = dd > 0xffffffffull ? 0xffffffffu : (uint32_t)dd;
+                                    } else {
+                                        const long long dd = (long long)(int32_t)u[0] * (int32_t)u[1] + (int32_t)u[2];
+                                        ru = (uint32_t)(int32_t)(dd < INT32_MIN ? INT32_MIN : dd > INT32_MAX ? INT32_MAX : dd);
+                                    }
                                 } else {
                                     e = -1;
                                 }

@@ -192,3 +192,77 @@ def test_compile_cache_returns_same_program(chai):
     t, m = mc_trace(1 << 10)
     p1 = t.compile(); p2 = t.compile()
     assert p1.info() == p2.info()
+
+
+# ops whose result is fixed by IEEE arithmetic (or by an exact definition such
+# as fmod / remainder / ldexp / nextafter / ilogb / logb): bit-exact on the GPU
+EXACT1 = [10, 223, 261, 256, 225, 212, 253, 255, 266, 233, 241, 246, 301, 305, 308, 507, 508, 509, 510, 513]
+EXACT2 = [11, 12, 13, 14, 32, 33, 34, 35, 36, 37, 302, 303, 227, 228, 213, 224, 229, 234, 243, 244, 247, 251,
+          306, 500, 501, 502, 503, 504, 505, 506, 511, 512]
+EXACT3 = [300, 226, 242, 304, 307]
+# device math library vs the host libm: within 8 ulp (relative to max(|r|, 1)
+# for results that pass through zero)
+LIBM1 = [219, 220, 221, 222, 237, 238, 239, 240, 257, 214, 262, 203, 200, 206, 259, 215, 263, 211, 218, 217,
+         201, 202, 204, 205, 208, 209, 216, 235, 260, 264, 265]
+LIBM2 = [248, 207, 232, 210, 250, 254, 249]
+
+
+def _ops_gpu(oracle, prec, arity, codes, seed):
+    from chai_amd.trace import Expr
+    from tests.test_oracle_ops import fp_values
+    n = 700
+    t = Trace()
+    ins, data = [], {}
+    for k in range(arity):
+        v = t.input(prec, n)
+        data[v.index] = np.roll(fp_values(prec, n, seed + k), 3 * k)[:n]
+        ins.append(v)
+    outs = [(c, t.assign(t.output(prec, n), Expr(c, ins))) for c in codes]
+    return outs, oracle.trace_eval(t, data), run_gpu(t, data)[0]
+
+
+@pytest.mark.parametrize("prec", [F32, F64])
+def test_exact_scalar_ops_bit_exact(chai, oracle, engine, prec):
+    from tests.test_oracle_ops import same_bits
+    bad = []
+    for arity, codes in ((1, EXACT1), (2, EXACT2), (3, EXACT3)):
+        for i in range(0, len(codes), 16):
+            outs, o, g = _ops_gpu(oracle, prec, arity, codes[i:i + 16], 50 * arity + i)
+            bad += [c for c, v in outs if not same_bits(g[v.index].ravel(), o[v.index].ravel())]
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("prec", [F32, F64])
+def test_libm_scalar_ops_within_ulp(chai, oracle, engine, prec):
+    eps = 2.0 ** -23 if prec == F32 else 2.0 ** -52
+    bad = []
+    for arity, codes in ((1, LIBM1), (2, LIBM2)):
+        for i in range(0, len(codes), 16):
+            outs, o, g = _ops_gpu(oracle, prec, arity, codes[i:i + 16], 70 * arity + i)
+            for c, v in outs:
+                a, b = g[v.index].ravel().astype(np.float64), o[v.index].ravel().astype(np.float64)
+                nan_ok = np.isnan(a) == np.isnan(b)
+                fin = np.isfinite(b) & np.isfinite(a)
+                inf_ok = np.where(np.isinf(b) | np.isinf(a), a == b, True)
+                # results near overflow (tgamma, pow) may round to inf on one side only
+                big = np.abs(b) > (3e38 if prec == F32 else 1.7e308)
+                close = np.abs(a - b) <= 8 * eps * np.maximum(np.abs(b), 1.0)
+                if not (np.all(nan_ok) and np.all(inf_ok | big) and np.all(close | ~fin)):
+                    bad.append(c)
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("prec", [U32, I32])
+def test_integer_ternary_ops_bit_exact(chai, oracle, engine, prec):
+    from chai_amd.trace import Expr
+    from tests.test_oracle_ops import int_values
+    n = 700
+    t = Trace()
+    ins = [t.input(prec, n) for _ in range(3)]
+    data = {v.index: np.roll(int_values(prec, n, 9 + k), 5 * k)[:n] for k, v in enumerate(ins)}
+    outs = [t.assign(t.output(prec, n), Expr(c, ins)) for c in (300, 406, 407, 408)]
+    outs.append(t.assign(t.output(prec, n), Expr(10, [ins[0]])))       # negation
+    o = oracle.trace_eval(t, data)
+    g, _ = run_gpu(t, data)
+    for v in outs:
+        assert np.array_equal(g[v.index], o[v.index]), v.index

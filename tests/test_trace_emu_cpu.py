@@ -43,6 +43,7 @@ def emulate(t, data, tmp_path, kernel=0):
     body = src[src.index('#include "trace_vm.hpp"'):]
     slots = [int(m) for m in re.findall(r"// L.vars\[\d+\] = trace variable (\d+)", body)]
     vps = int(re.search(r"v < (\d+)ull", body).group(1))
+    tmp_path.mkdir(parents=True, exist_ok=True)
     cpp, so = tmp_path / "k.cpp", tmp_path / "k.so"
     cpp.write_text(body.replace('extern "C" __global__', 'extern "C"') + DRIVER)
     subprocess.run([CLANG, "-O1", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-w",
@@ -99,3 +100,52 @@ def test_emitted_rng_normals_match_oracle(vl, tmp_path):
     got, ref = emulate(t, {}, tmp_path)
     np.testing.assert_array_equal(got[a.index], ref[a.index])
     np.testing.assert_array_equal(got[b.index], ref[b.index])
+
+
+def _ops_trace(codes, prec, arity, n, seed):
+    """one statement out_k = op_k(x0[, x1[, x2]]) per op code over shared inputs
+    of n mixed values (at most 20 ops, so that they fuse into one kernel)"""
+    from chai_amd.trace import Expr
+    from tests.test_oracle_ops import fp_values, int_values
+    t = Trace()
+    ins, data = [], {}
+    for k in range(arity):
+        v = t.input(prec, n)
+        vals = fp_values(prec, n, seed + k) if prec in (F32, F64) else int_values(prec, n, seed + k)
+        data[v.index] = np.roll(vals, 3 * k)[:n]
+        ins.append(v)
+    outs = [t.assign(t.output(prec, n), Expr(code, ins)) for code in codes]
+    return t, data, outs
+
+
+from chai_amd import F32, F64, I32, U32  # noqa: E402
+from chai_amd.trace import Trace  # noqa: E402
+
+
+def _check_ops(prec, groups, tmp_path):
+    from tests.test_oracle_ops import same_bits
+    bad = []
+    for arity, codes in groups:
+        for i in range(0, len(codes), 20):
+            chunk = codes[i:i + 20]
+            t, data, outs = _ops_trace(chunk, prec, arity, 300, 100 * arity + i)
+            got, ref = emulate(t, data, tmp_path / ("a%d_%d" % (arity, i)))
+            bad += [c for c, o in zip(chunk, outs) if not same_bits(got[o.index].ravel(), ref[o.index].ravel())]
+    assert not bad, bad
+
+
+@pytest.mark.parametrize("prec", [F32, F64])
+def test_emitted_scalar_fp_ops_match_oracle(prec, tmp_path):
+    """every floating point op of the VM (device semantics built for the host,
+    which calls the same libm as the oracle) equals the oracle bit for bit"""
+    from tests.test_oracle_ops import FP1, FP2, FP3
+    _check_ops(prec, ((1, FP1), (2, FP2), (3, FP3)), tmp_path)
+
+
+@pytest.mark.parametrize("prec", [U32, I32])
+def test_emitted_scalar_int_ops_match_oracle(prec, tmp_path):
+    from tests.test_oracle_ops import INT3
+    int2 = [11, 12, 13, 14, 17, 18, 19, 20, 21, 22, 30, 31, 32, 33, 34, 35, 36, 37, 302, 303, 401, 402, 405,
+            411, 412, 413, 414, 415]
+    int1 = [10, 15, 16, 400, 404, 225]
+    _check_ops(prec, ((1, int1), (2, int2), (3, INT3)), tmp_path)

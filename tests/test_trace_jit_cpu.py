@@ -6,7 +6,7 @@ import os
 import pytest
 
 from chai_amd.trace import emit_source
-from tests.trace_cases import md5_trace, mc_trace, random_trace
+from tests.trace_cases import bcast_trace, md5_trace, mc_trace, random_trace
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 CSRC = os.path.join(ROOT, "chai_amd", "csrc")
@@ -35,12 +35,14 @@ def hiprtc_compile(src):
     return rc, log.value.decode(errors="replace")
 
 
-@pytest.mark.parametrize("name", ["mc", "md5", "rand_int", "rand_fp"])
+@pytest.mark.parametrize("name", ["mc", "md5", "rand_int", "rand_fp", "bcast"])
 def test_jit_source_compiles(name):
     if name == "mc":
         t, _ = mc_trace(1 << 20)
     elif name == "md5":
         t = md5_trace(64)[0]
+    elif name == "bcast":
+        t = bcast_trace(13)[0]
     else:
         t = random_trace(3, name[5:])[0]
     src = emit_source(t, 0)
