@@ -57,6 +57,10 @@ def parse():
                    help="sgemm: 3 GEMMs plus the calibration kernels, for rocprofv3")
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                    help="summarise two rocprofv3 --pmc runs of --profile-run (no GPU needed)")
+    p.add_argument("--pmc-table", nargs="+", metavar="DIR",
+                   help="per-kernel counter values of rocprofv3 --pmc runs (no GPU needed)")
+    p.add_argument("--kernel-stats", nargs=3, metavar=("DIR", "TITLE", "COMMAND"),
+                   help="summarise a rocprofv3 --kernel-trace --stats run (no GPU needed)")
     return p.parse_args()
 
 
@@ -162,21 +166,30 @@ def profile_run(args):
     print("profile run done", flush=True)
 
 
+def _counter_rows(d, counter):
+    """{kernel name: [value per dispatch]} from a rocprofv3 output directory
+    (SQLite database, or CSV when written with --output-format csv)."""
+    rows = {}
+    for fn in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        import sqlite3
+        con = sqlite3.connect(fn)
+        for k, v in con.execute("select kernel_name, value from counters_collection where counter_name = ?",
+                                (counter,)):
+            rows.setdefault(k, []).append(float(v))
+        con.close()
+    for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                if r.get("Counter_Name") == counter:
+                    rows.setdefault(r.get("Kernel_Name", "?"), []).append(float(r["Counter_Value"]))
+    return rows
+
+
 def pmc_summary(fetch_dir, write_dir):
     """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two rocprofv3 runs of
-    --profile-run, scaled by the calibration kernels (rng_fill writes 2^30 B, the
-    reduction's partial kernel reads 2^30 B)."""
-    def load(d, counter):
-        rows = {}
-        for fn in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
-            with open(fn) as f:
-                for r in csv.DictReader(f):
-                    if r.get("Counter_Name") != counter:
-                        continue
-                    k = r.get("Kernel_Name", "?")
-                    rows.setdefault(k, []).append(float(r["Counter_Value"]))
-        return rows
-    fe, wr = load(fetch_dir, "FETCH_SIZE"), load(write_dir, "WRITE_SIZE")
+    --profile-run, scaled by the calibration kernels of known traffic (rng_fill
+    writes 2^30 B, the reduction's partial kernel reads 2^30 B)."""
+    fe, wr = _counter_rows(fetch_dir, "FETCH_SIZE"), _counter_rows(write_dir, "WRITE_SIZE")
 
     def pick(rows, pat):
         for k, v in rows.items():
@@ -191,24 +204,66 @@ def pmc_summary(fetch_dir, write_dir):
     write_scale = (2 ** 30) / (w_cal * 1024) if w_cal else None
     lines = ["# bf16x3 GEMM 8192^3: HBM traffic per launch from rocprofv3 counters",
              "# commands (from the repository root):",
-             "#   rocprofv3 --pmc FETCH_SIZE -d <fetch_dir> -- python3 bench.py --profile-run",
-             "#   rocprofv3 --pmc WRITE_SIZE -d <write_dir> -- python3 bench.py --profile-run",
+             "#   rocprofv3 --pmc FETCH_SIZE -d <fetch_dir> -o run -- python3 bench.py --profile-run",
+             "#   rocprofv3 --pmc WRITE_SIZE -d <write_dir> -o run -- python3 bench.py --profile-run",
              "#   python3 bench.py --pmc-summary <fetch_dir> <write_dir>",
-             "calibration_fetch_KiB(reduce 2^30 B read) = %s" % f_cal,
-             "calibration_write_KiB(rng_fill 2^30 B written) = %s" % w_cal,
+             "calibration_fetch_KiB(reduce_partial, 2^30 B read) = %s" % f_cal,
+             "calibration_write_KiB(rng_fill_32, 2^30 B written) = %s" % w_cal,
              "fetch_scale = %s" % fetch_scale, "write_scale = %s" % write_scale,
              "kernel = %s" % (kf or kw),
              "fetch_KiB_per_dispatch = %s" % f_gemm, "write_KiB_per_dispatch = %s" % w_gemm]
     if f_gemm and w_gemm and fetch_scale and write_scale:
         t = f_gemm * 1024 * fetch_scale + w_gemm * 1024 * write_scale
-        lines.append("traffic_bytes_per_launch = %.0f" % t)
         n = 8192
-        lines.append("algorithmic_bytes_per_launch = %d  # bf16 hi+lo panels of A and B^T read once, C written"
-                     % (2 * 2 * 2 * n * n + 4 * n * n))
+        alg = 2 * 2 * 2 * n * n + 4 * n * n
+        lines.append("traffic_bytes_per_launch = %.0f" % t)
+        lines.append("algorithmic_bytes_per_launch = %d  # bf16 hi+lo panels of A and B^T read once, C written" % alg)
+        lines.append("traffic_over_algorithmic = %.2f" % (t / alg))
+    lines.append("# per kernel, mean per dispatch (KiB, unscaled):")
     for k in sorted(set(fe) | set(wr)):
-        lines.append("  %-90.90s fetch_KiB %-14s write_KiB %s" % (
+        lines.append("  %-70.70s fetch %-12s write %s" % (
             k, "%.0f" % (sum(fe[k]) / len(fe[k])) if k in fe else "-",
             "%.0f" % (sum(wr[k]) / len(wr[k])) if k in wr else "-"))
+    print("\n".join(lines))
+
+
+def kernel_stats(d, title, command):
+    """Per-kernel call counts and durations from a rocprofv3 --kernel-trace --stats run."""
+    import sqlite3
+    lines = ["# %s" % title, "# command (from the repository root):", "#   %s" % command,
+             "# kernel | calls | total ms | average us | percent"]
+    for fn in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        con = sqlite3.connect(fn)
+        for name, calls, total, avg, pct in con.execute(
+                "select name, total_calls, total_duration, average, percentage from top_kernels"):
+            lines.append("%-100.100s | %5d | %10.3f | %10.3f | %5.2f" % (name, calls, total / 1e3, avg, pct))
+        con.close()
+    for fn in glob.glob(os.path.join(d, "**", "*kernel_stats.csv"), recursive=True):
+        with open(fn) as f:
+            for r in csv.DictReader(f):
+                lines.append("%-100.100s | %5s | %10.3f | %10.3f | %5s" % (
+                    r["Name"], r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
+                    r.get("Percentage", "")))
+    print("\n".join(lines))
+
+
+def pmc_table(dirs):
+    """Mean per-dispatch value of every counter collected in rocprofv3 --pmc runs
+    (one counter per run), per kernel."""
+    import sqlite3
+    vals = {}
+    for d in dirs:
+        for fn in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+            con = sqlite3.connect(fn)
+            for k, c, v in con.execute("select kernel_name, counter_name, avg(value) from counters_collection "
+                                       "group by kernel_name, counter_name"):
+                vals.setdefault(k, {})[c] = v
+            con.close()
+    lines = []
+    for k in sorted(vals):
+        lines.append(k[:110])
+        for c in sorted(vals[k]):
+            lines.append("    %-32s %.6g" % (c, vals[k][c]))
     print("\n".join(lines))
 
 
@@ -485,6 +540,12 @@ def main():
     args = parse()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary)
+        return
+    if args.kernel_stats:
+        kernel_stats(*args.kernel_stats)
+        return
+    if args.pmc_table:
+        pmc_table(args.pmc_table)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: --gpus %d is not supported by this build "

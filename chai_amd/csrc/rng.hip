@@ -70,11 +70,12 @@ __global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict
 {
     const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= g.nvec) return;
-    uint64_t row = 0, x0 = v * VL;
+    uint64_t row = 0, x0 = v * VL, n = VL;
     if (g.W != g.vpr * VL) {
         row = v / g.vpr;
         x0 = (v - row * g.vpr) * VL;
         if (x0 >= g.W) return;
+        n = (g.W - x0) < (uint64_t)VL ? (g.W - x0) : (uint64_t)VL;
     }
     uint64_t w[2];
     rng_words64<VARIANT>(v, g.seed, w);
@@ -92,7 +93,6 @@ __global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict
         }
     }
     double* dst = out + row * g.W + x0;
-    const uint64_t n = (g.W - x0) < (uint64_t)VL ? (g.W - x0) : (uint64_t)VL;
     for (uint64_t i = 0; i < n; i++) dst[i] = val[i];
 }
 

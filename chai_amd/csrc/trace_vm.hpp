@@ -116,13 +116,25 @@ __device__ __forceinline__ double as_double(int prec, uint64_t b)
     }
 }
 
+// Float -> integer conversions as the interpreter's x86 host performs them
+// (cvttsd2si / cvttss2si): NaN and out-of-range values give the "integer
+// indefinite" (the most negative value) instead of the GPU's clamped result.
+__device__ __forceinline__ long long x86_ll(double v)
+{
+    return (v != v || v >= 0x1p63 || v < -0x1p63) ? (long long)0x8000000000000000ull : (long long)v;
+}
+__device__ __forceinline__ int x86_i(double v)
+{
+    return (v != v || v >= 0x1p31 || v < -0x1p31) ? (int)0x80000000u : (int)v;
+}
+
 // double -> element bits, C conversion semantics of the interpreter
 // (InterpConvert.cpp: values pass through double)
 __device__ __forceinline__ uint64_t from_double(int prec, double v)
 {
     switch (prec) {
-        case 0: return bu((uint32_t)(long long)v);
-        case 1: return bi((int32_t)(long long)v);
+        case 0: return bu((uint32_t)x86_ll(v));
+        case 1: return bi((int32_t)x86_ll(v));
         case 4: return bf((float)v);
         default: return bd(v);
     }
@@ -170,9 +182,9 @@ __device__ CHAI_VM_HEAVY T fp_bin_heavy(uint32_t op, T a, T b)
         case 232: return sizeof(T) == 4 ? (T)hypotf((float)a, (float)b) : (T)hypot((double)a, (double)b);
         case 210: return sizeof(T) == 4 ? (T)((double)atan2f((float)a, (float)b) / VM_PI)
                                         : (T)(atan2((double)a, (double)b) / VM_PI);
-        case 234: return sizeof(T) == 4 ? (T)ldexpf((float)a, (int)b) : (T)ldexp((double)a, (int)b);
+        case 234: return sizeof(T) == 4 ? (T)ldexpf((float)a, x86_i(b)) : (T)ldexp((double)a, x86_i(b));
         case 247: return sizeof(T) == 4 ? (T)nextafterf((float)a, (float)b) : (T)nextafter((double)a, (double)b);
-        case 249: return sizeof(T) == 4 ? (T)powf((float)a, (float)(int)b) : (T)powi_sq((double)a, (int)b);
+        case 249: return sizeof(T) == 4 ? (T)powf((float)a, (float)x86_i(b)) : (T)powi_sq((double)a, x86_i(b));
         case 250: return sizeof(T) == 4 ? (T)powf((float)a, (float)b) : (T)pow((double)a, (double)b);
         case 251: return sizeof(T) == 4 ? (T)remainderf((float)a, (float)b) : (T)remainder((double)a, (double)b);
         case 254: return sizeof(T) == 4 ? (T)powf((float)a, 1.0f / (float)b) : (T)pow((double)a, 1.0 / (double)b);
@@ -341,7 +353,9 @@ __device__ CHAI_VM_HEAVY T fp_un_heavy(uint32_t op, T a)
         case 208: return f ? (T)atanhf((float)a) : (T)atanh((double)a);
         case 209: return f ? (T)((double)atanf((float)a) / VM_PI) : (T)(atan((double)a) / VM_PI);
         case 216: return f ? (T)cosf((float)(VM_PI * (double)a)) : (T)cos(VM_PI * (double)a);
-        case 233: return f ? (T)ilogbf((float)a) : (T)ilogb((double)a);
+        // ilogb: the host library's FP_ILOGB0 / FP_ILOGBNAN (INT_MIN) and INT_MAX for inf
+        case 233: return a != a || a == 0 ? (T)-2147483648.0 : __builtin_isinf(a) ? (T)2147483647.0
+                                          : f ? (T)ilogbf((float)a) : (T)ilogb((double)a);
         case 235: return f ? (T)lgammaf((float)a) : (T)lgamma((double)a);
         case 241: return f ? (T)logbf((float)a) : (T)logb((double)a);
         case 246: return (T)__builtin_nan("");

@@ -3,8 +3,9 @@ int x float) against the oracle's FP64 GEMM and interpreter restatement.
 
 Tolerances (stated per element, deterministic error bounds):
   FMA     |C - C64| <= gamma_K * sum_k |a_k b_k|,  gamma_K = K u / (1 - K u), u = 2^-24
-  BF16X3  |C - C64| <= (3 * 2^-18 + gamma_K) * sum_k |a_k b_k|   (split residual
-          |x - hi - lo| <= 2^-18 |x| per operand, dropped lo*lo term)
+  BF16X3  |C - C64| <= (3 * 2^-18 + gamma_3K) * sum_k |a_k b_k|   (split residual
+          |x - hi - lo| <= 2^-18 |x| per operand, dropped lo*lo term, three
+          products accumulated per k)
   F64     |C - C64| <= 2 K 2^-53 * sum_k |a_k b_k|
 and the interpreter's own result (product rounded to f32, double sum) lies
 within u * sum|ab| + u |C| of C64, so GPU-vs-interpreter uses the sum of both.
@@ -28,7 +29,9 @@ def bound(mode, K):
     if mode == "fma":
         return g
     if mode == "bf16x3":
-        return 3 * 2.0 ** -18 + g
+        # three products per k accumulate into one FP32 sum: 3K roundings
+        g3 = 3 * K * U / (1 - 3 * K * U)
+        return 3 * 2.0 ** -18 + g3
     return 2 * K * 2.0 ** -53
 
 
@@ -145,3 +148,30 @@ def test_gemm_rejects_bad_arguments(chai):
     Ai = torch.ones(4, 4, dtype=torch.int32, device="cuda")
     with pytest.raises(chai.ChaiError):
         chai.gemm(Ai, Ai, out_dtype=torch.float32)
+
+
+def _bf16_rne(x):
+    u = np.ascontiguousarray(x, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    r = ((u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000).astype(np.uint32)
+    return r.view(np.float32)
+
+
+def test_bf16x3_rank1_matches_split_model(chai, oracle):
+    """K = 1: C = (ah*bl + al*bh) + ah*bh with hi = bf16_rne(x), lo = bf16_rne(x - hi),
+    each product exact in FP32.  The GPU must agree with this model to one
+    FP32 rounding per accumulation (it pins the hardware conversion as round to
+    nearest even and the three-product order)."""
+    M, N = 256, 256
+    A = unif(oracle, 7, M, 1)
+    B = unif(oracle, 8, 1, N)
+    C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), mode=MODES["bf16x3"]).cpu().numpy()
+    ah = _bf16_rne(A); al = _bf16_rne(A - ah)
+    bh = _bf16_rne(B); bl = _bf16_rne(B - bh)
+    p1 = (ah @ bl).astype(np.float32)          # rank-1 products, exact in FP32
+    p2 = (al @ bh).astype(np.float32)
+    p3 = (ah @ bh).astype(np.float32)
+    model = (p1 + p2).astype(np.float32) + p3
+    ulp = np.spacing(np.abs(model).astype(np.float32))
+    dev = np.abs(C.astype(np.float64) - model.astype(np.float64)) / ulp
+    print("max deviation from the split model: %.2f ulp" % dev.max())
+    assert dev.max() <= 1.0
