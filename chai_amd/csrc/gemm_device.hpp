@@ -26,7 +26,8 @@ __device__ __forceinline__ uint16_t f32_to_bf16_rne(float f)
 
 __device__ __forceinline__ float bf16_bits_to_f32(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// a = hi + lo + r with hi = bf16(a), lo = bf16(a - hi); |r| <= 2^-17 |a|.
+// a = hi + lo + r with hi = bf16(a), lo = bf16(a - hi); |r| <= 2^-16 |a|
+// (8 significant bits each: |a - hi| <= 2^-8 |a|, |r| <= 2^-8 |a - hi|).
 __device__ __forceinline__ void split_bf16(float a, uint16_t& hi, uint16_t& lo)
 {
     hi = f32_to_bf16_rne(a);

@@ -3,7 +3,7 @@
 //
 // op(X) element (r, k) lives at src[r * ld + k] (TRANS = false) or
 // src[k * ld + r] (TRANS = true).  The output panels hi / lo are [Rp][Kp]
-// bf16, K contiguous, with X = hi + lo (+ a residual below 2^-17 |X|) and zeros
+// bf16, K contiguous, with X = hi + lo (+ a residual of at most 2^-16 |X|) and zeros
 // wherever r >= R or k >= K.  One call covers the 64 x 64 block at (r0, k0)
 // with 256 threads; reads are 16-byte vectors along the contiguous source
 // index.  A K-contiguous source is split straight from registers; a

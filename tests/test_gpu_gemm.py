@@ -3,8 +3,10 @@ int x float) against the oracle's FP64 GEMM and interpreter restatement.
 
 Tolerances (stated per element, deterministic error bounds):
   FMA     |C - C64| <= gamma_K * sum_k |a_k b_k|,  gamma_K = K u / (1 - K u), u = 2^-24
-  BF16X3  |C - C64| <= (3 * 2^-18 + gamma_3K) * sum_k |a_k b_k|   (split residual
-          |x - hi - lo| <= 2^-18 |x| per operand, dropped lo*lo term, three
+  BF16X3  |C - C64| <= (3 * 2^-16 + gamma_3K) * sum_k |a_k b_k|   (split residual
+          |x - hi - lo| <= 2^-16 |x| per operand: bf16 keeps 8 significant bits,
+          so |x - hi| <= 2^-8 |x| and |(x - hi) - lo| <= 2^-8 |x - hi|; the dropped
+          lo*lo term is below 2^-16 |a b|; three
           products accumulated per k)
   F64     |C - C64| <= 2 K 2^-53 * sum_k |a_k b_k|
 and the interpreter's own result (product rounded to f32, double sum) lies
@@ -31,7 +33,7 @@ def bound(mode, K):
     if mode == "bf16x3":
         # three products per k accumulate into one FP32 sum: 3K roundings
         g3 = 3 * K * U / (1 - 3 * K * U)
-        return 3 * 2.0 ** -18 + g3
+        return 3 * 2.0 ** -16 + g3
     return 2 * K * 2.0 ** -53
 
 

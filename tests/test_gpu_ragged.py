@@ -4,7 +4,7 @@ The persistent kernel runs the same split-BF16x3 tile arithmetic as the dense
 path, so a ragged batch must reproduce per-instance chai_hip_gemm results
 BIT-EXACTLY (same k order, same MFMAs), with the fused epilogue equal to the
 f32 ops C*scale, +shift, max(.., lo) applied one by one.  Against the FP64
-oracle the BF16X3 bound applies: |C - C64| <= (3 * 2^-18 + gamma_K) sum|a b|.
+oracle the BF16X3 bound applies: |C - C64| <= (3 * 2^-16 + gamma_3K) sum|a b|.
 """
 import threading
 
@@ -82,7 +82,8 @@ def test_ragged_vs_fp64_oracle(chai, oracle):
         absprod = np.abs(A64) @ np.abs(B64)
         g = K * U / (1 - K * U)
         err = np.abs(C.cpu().double().numpy() - C64)
-        assert np.all(err <= (3 * 2.0 ** -18 + g) * absprod), (M, N, K)
+        g3 = 3 * K * U / (1 - 3 * K * U)
+        assert np.all(err <= (3 * 2.0 ** -16 + g3) * absprod), (M, N, K)
         worst = max(worst, float((err / absprod).max()))
     print("max |C - C64| / sum|ab| =", worst)
 
