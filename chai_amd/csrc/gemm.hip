@@ -8,7 +8,7 @@
 // independent instances with optional shared A or B (sameDataA/B).
 //
 // Kernels (all gfx950):
-//   split_kernel    : f32/i32/u32 operand -> bf16 hi/lo panels, K contiguous,
+//   split_kernel    : f32/i32/u32 operand -> bf16 hi/lo panels in k blocks of 32,
 //                     zero padded to the tile grid (an HBM-bound pre-pass).
 //   gemm_bf16x3     : 256x256x32 tiles, 8 waves (2 x 4), each wave 128x64 built
 //                     from 32x32x16 bf16 MFMAs; three products hi*lo + lo*hi +
@@ -31,13 +31,14 @@ namespace chai_hip {
 
 template <typename T, bool TRANS>
 __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, uint64_t ld,
-                                                    uint64_t R, uint64_t K, uint64_t Kp,
-                                                    uint64_t src_bstride, uint64_t dst_bstride,
+                                                    uint64_t R, uint64_t K, uint64_t Rp, uint64_t Kp,
+                                                    uint64_t src_bstride,
                                                     uint16_t* __restrict__ hi,
                                                     uint16_t* __restrict__ lo)
 {
     __shared__ float tile[64][65];
-    split_tile<T, TRANS>(src + (uint64_t)blockIdx.z * src_bstride, ld, R, K, Kp,
+    const uint64_t dst_bstride = Rp * Kp;
+    split_tile<T, TRANS>(src + (uint64_t)blockIdx.z * src_bstride, ld, R, K, Kp, Rp,
                          hi + (uint64_t)blockIdx.z * dst_bstride, lo + (uint64_t)blockIdx.z * dst_bstride,
                          (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64, tile);
 }
@@ -303,9 +304,9 @@ static void launch_split(const void* src, uint64_t ld, bool trans, uint64_t R, u
 {
     dim3 grid((unsigned)(Kp / 64 + (Kp % 64 ? 1 : 0)), (unsigned)(Rp / 64), (unsigned)nb);
     if (trans)
-        split_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Kp, src_bstride, Rp * Kp, hi, lo);
+        split_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo);
     else
-        split_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Kp, src_bstride, Rp * Kp, hi, lo);
+        split_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo);
 }
 
 static void split_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
@@ -387,6 +388,7 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         prof_mark(1, st);
         GemmArgs g{};
         g.Ah = Ah; g.Al = Al; g.Bh = Bh; g.Bl = Bl;
+        g.a_kblk = Mp * 32; g.b_kblk = Np * 32;
         g.Kp = Kp; g.Np = Np;
         g.pa_bstride = d.same_a ? 0 : Mp * Kp;
         g.pb_bstride = d.same_b ? 0 : Np * Kp;

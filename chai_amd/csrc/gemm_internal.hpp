@@ -9,8 +9,10 @@
 namespace chai_hip {
 
 struct GemmArgs {
-    // split-bf16 panels, K contiguous: A [Mp][Kp], B^T [Np][Kp]
+    // split-bf16 panels in k blocks of 32: element (r, k) of a panel with Rp
+    // rows at (k / 32) * Rp * 32 + r * 32 + k % 32 (A: Rp = Mp, B^T: Rp = Np)
     const uint16_t *Ah, *Al, *Bh, *Bl;
+    uint64_t a_kblk, b_kblk;    // elements per k block: Mp * 32, Np * 32
     // packed operands for the FMA / FP64 kernels: A [Mp][Kp], B [Kp][Np]
     const float *Af, *Bf;
     const double *Ad, *Bd;

@@ -2,13 +2,15 @@
 // ragged GEMM paths.
 //
 // op(X) element (r, k) lives at src[r * ld + k] (TRANS = false) or
-// src[k * ld + r] (TRANS = true).  The output panels hi / lo are [Rp][Kp]
-// bf16, K contiguous, with X = hi + lo (+ a residual of at most 2^-16 |X|) and zeros
-// wherever r >= R or k >= K.  One call covers the 64 x 64 block at (r0, k0)
-// with 256 threads; reads are 16-byte vectors along the contiguous source
-// index.  A K-contiguous source is split straight from registers; a
-// transposed one goes through a padded LDS tile so that each thread can write
-// 16 consecutive k.
+// src[k * ld + r] (TRANS = true).  The output panels hi / lo hold Rp rows and
+// Kp columns of bf16 in k blocks of 32: element (r, k) at
+// (k / 32) * Rp * 32 + r * 32 + k % 32, so that the 32-deep slice of a row
+// range is one contiguous run (the GEMM loads it straight into LDS).  X = hi +
+// lo (+ a residual of at most 2^-16 |X|), with zeros wherever r >= R or k >= K.
+// One call covers the 64 x 64 block at (r0, k0) with 256 threads; reads are
+// 16-byte vectors along the contiguous source index.  A K-contiguous source is
+// split straight from registers; a transposed one goes through a padded LDS
+// tile so that each thread can write 16 consecutive k.
 #pragma once
 #include "gemm_device.hpp"
 
@@ -49,11 +51,17 @@ __device__ __forceinline__ void split_store16(const float* v, uint16_t* hi, uint
     if (second) { dh[1] = vh1; dl[1] = vl1; }
 }
 
+// offset of panel element (r, k), k a multiple of 16, in the k-blocked layout
+__device__ __forceinline__ uint64_t panel_off(uint64_t r, uint64_t k, uint64_t Rp)
+{
+    return (k >> 5) * (Rp * 32) + r * 32 + (k & 31);
+}
+
 template <typename T, bool TRANS>
 __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t ld, uint64_t R,
-                                           uint64_t K, uint64_t Kp, uint16_t* __restrict__ hi,
-                                           uint16_t* __restrict__ lo, uint64_t r0, uint64_t k0,
-                                           float (*tile)[65])
+                                           uint64_t K, uint64_t Kp, uint64_t Rp,
+                                           uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
+                                           uint64_t r0, uint64_t k0, float (*tile)[65])
 {
     const int t = threadIdx.x;
     // each thread finally writes 16 consecutive k of one row
@@ -75,7 +83,7 @@ __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t l
 #pragma unroll
             for (int j = 0; j < 16; j++) v[j] = (r < R && kb + j < K) ? to_f32(src[r * ld + kb + j]) : 0.0f;
         }
-        split_store16(v, hi, lo, r * Kp + kb, kb + 8 < Kp);
+        split_store16(v, hi, lo, panel_off(r, kb, Rp), kb + 8 < Kp);
         return;
     }
     // source element (r, k) at src[k * ld + r]: read 4 consecutive r per lane
@@ -104,7 +112,7 @@ __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t l
         float v[16];
 #pragma unroll
         for (int j = 0; j < 16; j++) v[j] = tile[kq + j][rr];
-        split_store16(v, hi, lo, r * Kp + k0 + kq, k0 + kq + 8 < Kp);
+        split_store16(v, hi, lo, panel_off(r, k0 + kq, Rp), k0 + kq + 8 < Kp);
     }
 }
 

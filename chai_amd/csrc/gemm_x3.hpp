@@ -1,7 +1,8 @@
 // gemm_x3.hpp -- split-BF16x3 MFMA GEMM core, templated over its tile shape.
 //
 // C = A B with A = Ah + Al and B = Bh + Bl (bf16 pairs from the split
-// pre-pass, both stored K-contiguous: A as [Mp][Kp], B^T as [Np][Kp]).
+// pre-pass; A as Mp x Kp and B^T as Np x Kp panels, both in k blocks of 32:
+// element (r, k) at (k / 32) * Rp * 32 + r * 32 + k % 32).
 // Each 32x32 output tile accumulates, per 16-deep k block, the three products
 // Ah*Bl + Al*Bh + Ah*Bh with v_mfma_f32_32x32x16_bf16 into one FP32
 // accumulator (the small terms first).
@@ -11,11 +12,13 @@
 //   WGM x WGN waves per block; each wave owns a (BM/WGM) x (BN/WGN) tile
 // LDS holds two stages of the four panels Ah, Al, Bh, Bl; rows are 16-byte
 // chunks XOR-swizzled so that the rows one MFMA operand read touches fall
-// into distinct banks.  Global loads for stage k+1 are issued into registers
-// before the MFMAs of stage k and written to LDS after them (one barrier per
-// stage); fragments of the next 16-deep k block are read from LDS while the
-// MFMAs of the current one run.  The kernel body avoids lambdas and keeps
-// every per-thread array fully unrolled so that nothing lands in scratch.
+// into distinct banks.  Stage k+1 is copied from global memory straight into
+// LDS (global_load_lds_dwordx4: lane i of a wave fills the i-th 16 bytes
+// after the wave's LDS base, so each lane fetches the source chunk that the
+// swizzle puts at its position) while the MFMAs of stage k run; one vmcnt
+// wait and one barrier close each stage.  No staging registers, no ds_write.
+// The kernel body avoids lambdas and keeps every per-thread array fully
+// unrolled so that nothing lands in scratch.
 #pragma once
 #include "gemm_device.hpp"
 #include "gemm_internal.hpp"
@@ -28,6 +31,19 @@ __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v)
     __builtin_memcpy(&r, &v, 16);
     return r;
 }
+
+typedef __attribute__((address_space(1))) void global_void_t;
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// 16 bytes per lane from global memory into LDS: lane i of the wave writes
+// bytes [16 i, 16 i + 16) after dst (dst is wave-uniform).  Completion is
+// counted by vmcnt.
+__device__ __forceinline__ void lds_dma16(const uint16_t* src, u32x4* dst)
+{
+    __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)dst, 16, 0, 0);
+}
+
+__device__ __forceinline__ void wait_lds_dma() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
 
 __device__ __forceinline__ void store_c(float* C, uint64_t idx, float v, const GemmArgs& a)
 {
@@ -45,14 +61,42 @@ struct X3Shape {
     static constexpr int SH = BK == 32 ? 2 : 1;        // rows sharing one 256-byte bank line
     static constexpr int PA = BM * CPR, PB = BN * CPR; // chunks per panel per stage
     static constexpr int STAGE = 2 * PA + 2 * PB;
-    static constexpr int NA = PA / THREADS, NB = PB / THREADS;   // load slots per panel
-    static constexpr int RA = THREADS / CPR;           // rows covered by one load slot
+    static constexpr int WAVES = THREADS / 64;
+    static constexpr int IA = PA / 64 / WAVES, IB = PB / 64 / WAVES;   // LDS-DMA issues per panel
     static constexpr int KB = BK / 16;
     static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE * 16;
     static_assert(WM % 32 == 0 && WN % 32 == 0, "wave tile must be a multiple of 32");
     static_assert(BK == 32 || BK == 64, "BK is 32 or 64");
     static_assert(PA % THREADS == 0 && PB % THREADS == 0, "panel chunks must split evenly");
+    static_assert(IA * 64 * WAVES == PA && IB * 64 * WAVES == PB, "whole wave issues per panel");
 };
+
+// Issue the LDS-DMA copies of one stage into the stage buffer st; wave w
+// copies chunks [(w * I + s) * 64, +64) of each panel.  The panel pointers
+// are wave-uniform (tile origin and stage k offset folded in); a lane adds its
+// 32-bit offset within the k block and, for a 64-deep stage, one k block when
+// bit s of its mask is set.
+template <typename S>
+__device__ __forceinline__ void x3_issue_stage(const uint16_t* pAh, const uint16_t* pAl, const uint16_t* pBh,
+                                               const uint16_t* pBl, const uint32_t (&srcA)[S::IA],
+                                               const uint32_t (&srcB)[S::IB], uint32_t hiA, uint32_t hiB,
+                                               uint64_t a_kblk, uint64_t b_kblk, u32x4* st, int wave)
+{
+#pragma unroll
+    for (int s = 0; s < S::IA; s++) {
+        const int c0 = (wave * S::IA + s) * 64;
+        const uint64_t o = srcA[s] + (S::CPR == 8 && ((hiA >> s) & 1u) ? a_kblk : 0);
+        lds_dma16(pAh + o, st + c0);
+        lds_dma16(pAl + o, st + S::PA + c0);
+    }
+#pragma unroll
+    for (int s = 0; s < S::IB; s++) {
+        const int c0 = (wave * S::IB + s) * 64;
+        const uint64_t o = srcB[s] + (S::CPR == 8 && ((hiB >> s) & 1u) ? b_kblk : 0);
+        lds_dma16(pBh + o, st + 2 * S::PA + c0);
+        lds_dma16(pBl + o, st + 2 * S::PA + S::PB + c0);
+    }
+}
 
 // One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is
 // free on entry and on exit (the last k step ends with a barrier), so a
@@ -68,27 +112,28 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
     const int wm = (wave / WGN) * S::WM;
     const int wn = (wave % WGN) * S::WN;
 
-    // this thread's load rows: row (t / CPR) + s * RA of each panel, chunk q_ld
-    const int q_ld = t % S::CPR;
-    const int r_ld = t / S::CPR;
-    const u32x4* gAh = reinterpret_cast<const u32x4*>(a.Ah + bz * a.pa_bstride + ((uint64_t)tm * BM + r_ld) * Kp) + q_ld;
-    const u32x4* gAl = reinterpret_cast<const u32x4*>(a.Al + bz * a.pa_bstride + ((uint64_t)tm * BM + r_ld) * Kp) + q_ld;
-    const u32x4* gBh = reinterpret_cast<const u32x4*>(a.Bh + bz * a.pb_bstride + ((uint64_t)tn * BN + r_ld) * Kp) + q_ld;
-    const u32x4* gBl = reinterpret_cast<const u32x4*>(a.Bl + bz * a.pb_bstride + ((uint64_t)tn * BN + r_ld) * Kp) + q_ld;
-    const uint64_t slot_stride = (uint64_t)S::RA * Kp / 8;    // in u32x4 units
-    // LDS offsets (u32x4 units, within a stage) of this thread's stores
-    int lo_st[S::NA];
+    // LDS chunk c = (wave * IA + s) * 64 + lane of a panel is row c / CPR,
+    // position c % CPR, and holds source chunk q = position ^ swizzle(row):
+    // k block q / 4 of the stage, 8 elements at (q % 4) * 8 within it.
+    uint32_t srcA[S::IA], srcB[S::IB];   // offsets within the k block, from the tile origin
+    uint32_t hiA = 0, hiB = 0;            // bit s: chunk s lies in the stage's second k block
 #pragma unroll
-    for (int s = 0; s < S::NA; s++) {
-        const int row = r_ld + s * S::RA;
-        lo_st[s] = row * S::CPR + (q_ld ^ ((row >> S::SH) & (S::CPR - 1)));
+    for (int s = 0; s < S::IA; s++) {
+        const int c = (wave * S::IA + s) * 64 + lane, row = c / S::CPR, p = c % S::CPR;
+        const int q = p ^ ((row >> S::SH) & (S::CPR - 1));
+        srcA[s] = (uint32_t)(row * 32 + (q & 3) * 8);
+        hiA |= (uint32_t)(q >> 2) << s;
     }
-    int lo_stb[S::NB];
 #pragma unroll
-    for (int s = 0; s < S::NB; s++) {
-        const int row = r_ld + s * S::RA;
-        lo_stb[s] = row * S::CPR + (q_ld ^ ((row >> S::SH) & (S::CPR - 1)));
+    for (int s = 0; s < S::IB; s++) {
+        const int c = (wave * S::IB + s) * 64 + lane, row = c / S::CPR, p = c % S::CPR;
+        const int q = p ^ ((row >> S::SH) & (S::CPR - 1));
+        srcB[s] = (uint32_t)(row * 32 + (q & 3) * 8);
+        hiB |= (uint32_t)(q >> 2) << s;
     }
+    const uint64_t a0 = bz * a.pa_bstride + (uint64_t)tm * BM * 32;   // tile origin, k block 0
+    const uint64_t b0 = bz * a.pb_bstride + (uint64_t)tn * BN * 32;
+    const uint64_t kstepA = (uint64_t)(BK / 32) * a.a_kblk, kstepB = (uint64_t)(BK / 32) * a.b_kblk;
 
     floatx16 acc[S::TI][S::TJ];
 #pragma unroll
@@ -98,18 +143,11 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
 
-    u32x4 sAh[S::NA], sAl[S::NA], sBh[S::NB], sBl[S::NB];
-
     const int nk = (int)(Kp / BK);
     // prologue: stage 0 -> LDS buffer 0
-#pragma unroll
-    for (int s = 0; s < S::NA; s++) { sAh[s] = gAh[s * slot_stride]; sAl[s] = gAl[s * slot_stride]; }
-#pragma unroll
-    for (int s = 0; s < S::NB; s++) { sBh[s] = gBh[s * slot_stride]; sBl[s] = gBl[s * slot_stride]; }
-#pragma unroll
-    for (int s = 0; s < S::NA; s++) { lds[lo_st[s]] = sAh[s]; lds[S::PA + lo_st[s]] = sAl[s]; }
-#pragma unroll
-    for (int s = 0; s < S::NB; s++) { lds[2 * S::PA + lo_stb[s]] = sBh[s]; lds[2 * S::PA + S::PB + lo_stb[s]] = sBl[s]; }
+    x3_issue_stage<S>(a.Ah + a0, a.Al + a0, a.Bh + b0, a.Bl + b0, srcA, srcB, hiA, hiB, a.a_kblk, a.b_kblk,
+                      lds, wave);
+    wait_lds_dma();
     __syncthreads();
 
     // fragment read offsets (u32x4 units) for each A / B tile row of this lane
@@ -131,13 +169,11 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
 
     for (int kt = 0; kt < nk; kt++) {
         const int cur = (kt & 1) * S::STAGE;
-        const bool more = kt + 1 < nk;
-        if (more) {
-            const uint64_t ko = (uint64_t)(kt + 1) * (BK / 8);
-#pragma unroll
-            for (int s = 0; s < S::NA; s++) { sAh[s] = gAh[s * slot_stride + ko]; sAl[s] = gAl[s * slot_stride + ko]; }
-#pragma unroll
-            for (int s = 0; s < S::NB; s++) { sBh[s] = gBh[s * slot_stride + ko]; sBl[s] = gBl[s * slot_stride + ko]; }
+        // buffer (kt + 1) & 1 was last read in step kt - 1, closed by its barrier
+        if (kt + 1 < nk) {
+            const uint64_t oa = a0 + (uint64_t)(kt + 1) * kstepA, ob = b0 + (uint64_t)(kt + 1) * kstepB;
+            x3_issue_stage<S>(a.Ah + oa, a.Al + oa, a.Bh + ob, a.Bl + ob, srcA, srcB, hiA, hiB, a.a_kblk,
+                              a.b_kblk, lds + ((kt + 1) & 1) * S::STAGE, wave);
         }
 #pragma unroll
         for (int kb = 0; kb < S::KB; kb++) {
@@ -169,17 +205,8 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
                 for (int j = 0; j < S::TJ; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
         }
-        if (more) {
-            const int nxt = ((kt + 1) & 1) * S::STAGE;
-#pragma unroll
-            for (int s = 0; s < S::NA; s++) { lds[nxt + lo_st[s]] = sAh[s]; lds[nxt + S::PA + lo_st[s]] = sAl[s]; }
-#pragma unroll
-            for (int s = 0; s < S::NB; s++) {
-                lds[nxt + 2 * S::PA + lo_stb[s]] = sBh[s];
-                lds[nxt + 2 * S::PA + S::PB + lo_stb[s]] = sBl[s];
-            }
-        }
-        __syncthreads();
+        wait_lds_dma();     // stage kt + 1 has landed (this wave's share)
+        __syncthreads();    // ... every wave's share, and stage kt is no longer read
     }
 
     // 32x32x16 output layout: lane -> column lane%32; register r -> row 8*(r/4) + 4*(lane/32) + r%4
@@ -202,7 +229,7 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
 // order: X(BM, BN, BK, waves_m, waves_n, min blocks per CU).
 #define CHAI_X3_VARIANTS(X)                                                   \
     X(256, 256, 32, 2, 4, 1)   /* 0: 8 waves of 128x64               */      \
-    X(256, 256, 32, 2, 2, 1)   /* 1: 4 waves of 128x128              */      \
+    X(256, 256, 32, 4, 2, 1)   /* 1: 8 waves of 64x128               */      \
     X(128, 128, 32, 2, 2, 2)   /* 2: 4 waves of 64x64, 2 blocks / CU */      \
     X(256, 128, 32, 2, 2, 1)   /* 3: 4 waves of 128x64               */      \
     X(128, 256, 32, 2, 2, 1)   /* 4: 4 waves of 64x128               */      \

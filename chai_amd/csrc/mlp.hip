@@ -253,8 +253,8 @@ static void gemm_features(const chai_gemm_desc& d, int variant, float* f)
     f[0] = (float)(std::log2(M) / 14);
     f[1] = (float)(std::log2(N) / 14);
     f[2] = (float)(std::log2(K) / 14);
-    f[3] = (float)(d.trans_a != 0);
-    f[4] = (float)(d.trans_b != 0);
+    f[3] = (float)(d.trans_a != 0 || d.trans_b != 0);   // (affects only the split pre-pass)
+    f[4] = (float)(std::log2((BM / p[3]) / (BN / p[4])) / 2);   // wave tile aspect log2(WM / WN) / 2
     f[5] = (float)(d.prec_a == CHAI_I32 || d.prec_a == CHAI_U32);
     f[6] = (float)(d.prec_b == CHAI_I32 || d.prec_b == CHAI_U32);
     f[7] = (float)(std::log2(BM) / 8);

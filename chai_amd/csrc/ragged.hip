@@ -6,8 +6,8 @@
 // enqueue/EnqueueMatmulAuto.cpp:57-68); a ragged batch of uneven sizes runs
 // there as one kernel (plus one epilogue kernel) per instance.  Here a batch
 // of any shapes runs as two launches:
-//   1. ragged_split_kernel: every instance's A and B^T become zero-padded,
-//      K-contiguous bf16 hi/lo panels (one flat grid of 64 x 64 blocks; a
+//   1. ragged_split_kernel: every instance's A and B^T become zero-padded
+//      bf16 hi/lo panels in k blocks of 32 (one flat grid of 64 x 64 blocks; a
 //      block finds its instance by binary search over per-instance block
 //      offsets).  Block 0 also clears the tile-queue counters, so the pair can
 //      be captured in a hipGraph and replayed without memset nodes.
@@ -41,12 +41,13 @@ struct RaggedInst {
     const float* A;
     const float* B;
     float* C;
-    uint16_t *Ah, *Al, *Bh, *Bl;   // [Mp][Kp] and [Np][Kp] panels in the workspace
+    uint16_t *Ah, *Al, *Bh, *Bl;   // Mp x Kp and Np x Kp panels (k blocks of 32) in the workspace
     uint32_t M, N, K, Kp;
     uint32_t lda, ldb, ldc;
     uint32_t a_blocks;             // split blocks of A: (Mp / 64) * (Kp / 64)
     uint32_t kblocks;              // Kp / 64
-    uint32_t pad[3];
+    uint32_t Mp, Np;               // panel rows (tile multiples)
+    uint32_t pad;
 };
 
 struct RaggedEpi {
@@ -72,13 +73,13 @@ __global__ __launch_bounds__(256) void ragged_split_kernel(const RaggedInst* __r
     uint32_t j = b - first[lo];
     if (j < I.a_blocks) {
         const uint32_t rb = j / I.kblocks, kb = j - rb * I.kblocks;
-        split_tile<float, false>(I.A, I.lda, I.M, I.K, I.Kp, I.Ah, I.Al, (uint64_t)rb * 64,
+        split_tile<float, false>(I.A, I.lda, I.M, I.K, I.Kp, I.Mp, I.Ah, I.Al, (uint64_t)rb * 64,
                                  (uint64_t)kb * 64, tile);
     } else {
         j -= I.a_blocks;
         const uint32_t rb = j / I.kblocks, kb = j - rb * I.kblocks;
         // B^T panel element (n, k) = B[k * ldb + n]: transposed read
-        split_tile<float, true>(I.B, I.ldb, I.N, I.K, I.Kp, I.Bh, I.Bl, (uint64_t)rb * 64,
+        split_tile<float, true>(I.B, I.ldb, I.N, I.K, I.Kp, I.Np, I.Bh, I.Bl, (uint64_t)rb * 64,
                                 (uint64_t)kb * 64, tile);
     }
 }
@@ -106,6 +107,7 @@ __global__ __launch_bounds__(WGM * WGN * 64, MINB) void ragged_x3_kernel(
             const RaggedInst& I = inst[w.x];
             GemmArgs g{};
             g.Ah = I.Ah; g.Al = I.Al; g.Bh = I.Bh; g.Bl = I.Bl;
+            g.a_kblk = (uint64_t)I.Mp * 32; g.b_kblk = (uint64_t)I.Np * 32;
             g.Kp = I.Kp;
             g.C = I.C; g.ldc = I.ldc;
             g.M = I.M; g.N = I.N;
@@ -260,6 +262,7 @@ static int ragged_build(const chai_ragged_item* items, size_t count, const chai_
         I.M = (uint32_t)it.M; I.N = (uint32_t)it.N; I.K = (uint32_t)it.K; I.Kp = (uint32_t)s.Kp;
         I.lda = (uint32_t)it.lda; I.ldb = (uint32_t)it.ldb; I.ldc = (uint32_t)it.ldc;
         I.kblocks = (uint32_t)(s.Kp / 64);
+        I.Mp = (uint32_t)s.Mp; I.Np = (uint32_t)s.Np;
         I.a_blocks = (uint32_t)(s.Mp / 64) * I.kblocks;
         hfirst[j] = acc;
         acc += I.a_blocks + (uint32_t)(s.Np / 64) * I.kblocks;

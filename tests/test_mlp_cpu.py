@@ -13,7 +13,8 @@ def test_features_are_scaled_and_distinguish_variants():
     assert np.all(np.isfinite(F)) and np.all(np.abs(F) <= 4)
     assert len({tuple(r) for r in F}) == 8           # every variant gets its own input
     np.testing.assert_allclose(F[:, :3], [[12 / 14, 10 / 14, np.log2(777) / 14]] * 8, rtol=1e-6)
-    assert F[0, 4] == 1 and F[0, 3] == 0 and F[0, 5] == 0
+    # f3: any transposed operand; f4: wave-tile aspect log2(WM / WN) / 2 (variant 0: 128 x 64)
+    assert F[0, 3] == 1 and F[0, 4] == 0.5 and F[1, 4] == -0.5 and F[2, 4] == 0 and F[0, 5] == 0
 
 
 def test_features_reject_bad_variant():
