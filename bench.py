@@ -485,6 +485,18 @@ def run_mc(args):
     line["mc_mean"] = float(out.item())
     line["philox_normals_2^24_ms"] = round(ms_norm, 4)
     line["philox_normals_GBs"] = round(4.0 * n / (ms_norm * 1e-3) / 1e9, 1)
+    # steady state of the raw stream: 2^28 f32 (1 GiB written)
+    big = torch.empty(1 << 28, device="cuda")
+    ms_u = timed(lambda: chai.rng_fill(big, seed=7), args.steps, args.warmup)
+    ms_n = timed(lambda: chai.rng_fill(big, seed=7, normal=True), args.steps, args.warmup)
+    line["philox_uniform_2^28"] = {"ms": round(ms_u, 4), "GBs": round(4.0 * 2 ** 28 / (ms_u * 1e-3) / 1e9, 1),
+                                   "frac_hbm": round(4.0 * 2 ** 28 / (ms_u * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)}
+    line["philox_normals_2^28"] = {"ms": round(ms_n, 4), "GBs": round(4.0 * 2 ** 28 / (ms_n * 1e-3) / 1e9, 1),
+                                   "frac_hbm": round(4.0 * 2 ** 28 / (ms_n * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)}
+    # the same 1 GiB written by torch's fill kernel: the write bandwidth this part reaches
+    ms_fill = timed(lambda: big.fill_(1.0), args.steps, args.warmup)
+    line["torch_fill_2^28_GBs"] = round(4.0 * 2 ** 28 / (ms_fill * 1e-3) / 1e9, 1)
+    del big
     line["gemv_ms"] = round(ms_gemv, 4)
     line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "gemv_rows"}
@@ -523,7 +535,18 @@ def run_dense(args, mixed):
         dtype, peak, bound = "f64 (FP64 MFMA)", PEAK_FP64_TFLOPS, "mfma"
     ms = timed(fn, args.steps, args.warmup)
     tf = 2.0 * n ** 3 / (ms * 1e-3) / 1e12
-    ach = tf * (3 if mixed else 1)
+    # core kernel duration (hipEvents inside the library on the launch stream)
+    chai.profile(True)
+    core = []
+    for _ in range(5):
+        fn()
+        core.append(chai.gemm_last_timing()[1])
+    chai.profile(False)
+    core_ms = sum(core) / len(core)
+    # MFMA work per f32 multiply-add: the ints in [-128, 127] are exact in bf16,
+    # so their lo panel is zero and the tiles run two products (ah*bl, ah*bh)
+    products = 2 if mixed else 1
+    ach = products * 2.0 * n ** 3 / (core_ms * 1e-3) / 1e12
     rows = np.random.default_rng(0).choice(n, 8, replace=False)
     ridx = torch.from_numpy(rows).cuda()
     ref = A[ridx].cpu().numpy().astype(np.float64) @ B.cpu().numpy().astype(np.float64)
@@ -531,7 +554,8 @@ def run_dense(args, mixed):
     line = base_line(args, round(tf, 2), "TFLOP/s", round(ms, 4), dtype,
                      {"workload": ("mixed i32[-128,127] x f32 " if mixed else "dgemm f64 ") + "4096^3"})
     line["roofline"] = {"bound": bound, "achieved": round(ach, 1), "peak": peak, "unit": "TFLOP/s",
-                        "frac": round(ach / peak, 4), "traffic": None}
+                        "frac": round(ach / peak, 4), "traffic": None, "kernel_ms": round(core_ms, 4),
+                        "mfma_products_per_mad": products}
     line["max_abs_err_vs_fp64"] = err
     emit(line)
 

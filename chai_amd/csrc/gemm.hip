@@ -30,17 +30,24 @@ namespace chai_hip {
 // =====================================================================
 
 template <typename T, bool TRANS>
+// Block (kb, rb, z) splits rows [64 rb, +64) x k [64 kb, +64) of instance z
+// and writes nz[(z * gridDim.y + rb) * gridDim.x + kb] = 1 if any of its lo
+// values is nonzero (the GEMM tile skips products with all-zero lo panels).
 __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, uint64_t ld,
                                                     uint64_t R, uint64_t K, uint64_t Rp, uint64_t Kp,
                                                     uint64_t src_bstride,
                                                     uint16_t* __restrict__ hi,
-                                                    uint16_t* __restrict__ lo)
+                                                    uint16_t* __restrict__ lo, uint8_t* __restrict__ nz)
 {
     __shared__ float tile[64][65];
     const uint64_t dst_bstride = Rp * Kp;
-    split_tile<T, TRANS>(src + (uint64_t)blockIdx.z * src_bstride, ld, R, K, Kp, Rp,
-                         hi + (uint64_t)blockIdx.z * dst_bstride, lo + (uint64_t)blockIdx.z * dst_bstride,
-                         (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64, tile);
+    const uint32_t lo_or =
+        split_tile<T, TRANS>(src + (uint64_t)blockIdx.z * src_bstride, ld, R, K, Kp, Rp,
+                             hi + (uint64_t)blockIdx.z * dst_bstride, lo + (uint64_t)blockIdx.z * dst_bstride,
+                             (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64, tile);
+    const int any = __syncthreads_or(lo_or != 0u);
+    if (threadIdx.x == 0)
+        nz[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (uint8_t)(any != 0);
 }
 
 // Pack op(X) into a dense zero-padded [Rp][Cp] array of TO (element (r, c) of
@@ -297,26 +304,28 @@ static void prof_mark(int i, hipStream_t st)
     t_prof.valid = (i == 2) ? true : (i == 0 ? false : t_prof.valid);
 }
 
+// Kp and Rp are multiples of 64: the grid is exactly (Kp / 64, Rp / 64, nb)
+// and nz receives (Rp / 64) * (Kp / 64) flags per instance.
 template <typename T>
 static void launch_split(const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
                          uint64_t Rp, uint64_t Kp, uint64_t src_bstride, uint64_t nb,
-                         uint16_t* hi, uint16_t* lo, hipStream_t st)
+                         uint16_t* hi, uint16_t* lo, uint8_t* nz, hipStream_t st)
 {
-    dim3 grid((unsigned)(Kp / 64 + (Kp % 64 ? 1 : 0)), (unsigned)(Rp / 64), (unsigned)nb);
+    dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64), (unsigned)nb);
     if (trans)
-        split_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo);
+        split_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo, nz);
     else
-        split_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo);
+        split_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo, nz);
 }
 
 static void split_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
                       uint64_t Rp, uint64_t Kp, uint64_t src_bstride, uint64_t nb, uint16_t* hi,
-                      uint16_t* lo, hipStream_t st)
+                      uint16_t* lo, uint8_t* nz, hipStream_t st)
 {
     switch (prec) {
-        case CHAI_F32: launch_split<float>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, st); break;
-        case CHAI_I32: launch_split<int32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, st); break;
-        default: launch_split<uint32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, st); break;
+        case CHAI_F32: launch_split<float>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, nz, st); break;
+        case CHAI_I32: launch_split<int32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, nz, st); break;
+        default: launch_split<uint32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, nb, hi, lo, nz, st); break;
     }
 }
 
@@ -373,22 +382,29 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         const X3Variant& v = kX3Variants[vid];
         const uint64_t Mp = round_up(M, v.BM), Np = round_up(N, v.BN), Kp = round_up(K, 64);
         const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
-        const size_t bytes = 2 * sizeof(uint16_t) * (Mp * Kp * na + Np * Kp * nbb);
+        const size_t panel_bytes = 2 * sizeof(uint16_t) * (Mp * Kp * na + Np * Kp * nbb);
+        const uint64_t fa = (Mp / 64) * (Kp / 64), fb = (Np / 64) * (Kp / 64);   // split flags per instance
+        const size_t bytes = panel_bytes + fa * na + fb * nbb;
         uint16_t* ws = nullptr;
         CHAI_HIP_TRY(hipMallocAsync((void**)&ws, bytes, st));
         uint16_t* Ah = ws;
         uint16_t* Al = Ah + Mp * Kp * na;
         uint16_t* Bh = Al + Mp * Kp * na;
         uint16_t* Bl = Bh + Np * Kp * nbb;
+        uint8_t* nz_a = (uint8_t*)ws + panel_bytes;
+        uint8_t* nz_b = nz_a + fa * na;
         prof_mark(0, st);
         // op(A): element (m,k) at A[m*lda+k] or, transposed, A[k*lda+m]
-        split_any(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, Ah, Al, st);
+        split_any(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, Ah, Al, nz_a, st);
         // B^T panels: element (n,k) = op(B)[k][n] at B[k*ldb+n] (transposed read) or B[n*ldb+k]
-        split_any(d.prec_b, B, d.ldb, d.trans_b == 0, N, K, Np, Kp, d.stride_b, nbb, Bh, Bl, st);
+        split_any(d.prec_b, B, d.ldb, d.trans_b == 0, N, K, Np, Kp, d.stride_b, nbb, Bh, Bl, nz_b, st);
         prof_mark(1, st);
         GemmArgs g{};
         g.Ah = Ah; g.Al = Al; g.Bh = Bh; g.Bl = Bl;
         g.a_kblk = Mp * 32; g.b_kblk = Np * 32;
+        g.a_nz = nz_a; g.b_nz = nz_b;
+        g.a_nz_bstride = d.same_a ? 0 : fa;
+        g.b_nz_bstride = d.same_b ? 0 : fb;
         g.Kp = Kp; g.Np = Np;
         g.pa_bstride = d.same_a ? 0 : Mp * Kp;
         g.pb_bstride = d.same_b ? 0 : Np * Kp;

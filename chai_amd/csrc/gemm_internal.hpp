@@ -13,6 +13,11 @@ struct GemmArgs {
     // rows at (k / 32) * Rp * 32 + r * 32 + k % 32 (A: Rp = Mp, B^T: Rp = Np)
     const uint16_t *Ah, *Al, *Bh, *Bl;
     uint64_t a_kblk, b_kblk;    // elements per k block: Mp * 32, Np * 32
+    // optional, per 64 x 64 split block (row block major, Kp / 64 per row
+    // block): 1 if any lo value of the block is nonzero.  A tile whose whole
+    // Al (Bl) panel is zero skips the products with it (null: never skip).
+    const uint8_t *a_nz, *b_nz;
+    uint64_t a_nz_bstride, b_nz_bstride;
     // packed operands for the FMA / FP64 kernels: A [Mp][Kp], B [Kp][Np]
     const float *Af, *Bf;
     const double *Ad, *Bd;

@@ -31,9 +31,10 @@ __device__ __forceinline__ f32x4 load4_f32(const T* p)
     }
 }
 
-// 16 consecutive k of one row -> 2 x 16 bytes of each panel
-__device__ __forceinline__ void split_store16(const float* v, uint16_t* hi, uint16_t* lo, uint64_t off,
-                                             bool second)
+// 16 consecutive k of one row -> 2 x 16 bytes of each panel; returns the OR
+// of the lo words stored (zero iff every value stored was exact in bf16)
+__device__ __forceinline__ uint32_t split_store16(const float* v, uint16_t* hi, uint16_t* lo, uint64_t off,
+                                                 bool second)
 {
     u32x4 vh0, vh1, vl0, vl1;
 #pragma unroll
@@ -48,7 +49,12 @@ __device__ __forceinline__ void split_store16(const float* v, uint16_t* hi, uint
     u32x4* dl = reinterpret_cast<u32x4*>(lo + off);
     dh[0] = vh0;
     dl[0] = vl0;
-    if (second) { dh[1] = vh1; dl[1] = vl1; }
+    uint32_t lo_or = vl0[0] | vl0[1] | vl0[2] | vl0[3];
+    if (second) {
+        dh[1] = vh1; dl[1] = vl1;
+        lo_or |= vl1[0] | vl1[1] | vl1[2] | vl1[3];
+    }
+    return lo_or;
 }
 
 // offset of panel element (r, k), k a multiple of 16, in the k-blocked layout
@@ -57,8 +63,9 @@ __device__ __forceinline__ uint64_t panel_off(uint64_t r, uint64_t k, uint64_t R
     return (k >> 5) * (Rp * 32) + r * 32 + (k & 31);
 }
 
+// Returns the OR of this thread's lo words (see split_store16).
 template <typename T, bool TRANS>
-__device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t ld, uint64_t R,
+__device__ __forceinline__ uint32_t split_tile(const T* __restrict__ src, uint64_t ld, uint64_t R,
                                            uint64_t K, uint64_t Kp, uint64_t Rp,
                                            uint16_t* __restrict__ hi, uint16_t* __restrict__ lo,
                                            uint64_t r0, uint64_t k0, float (*tile)[65])
@@ -70,7 +77,7 @@ __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t l
     const bool vec = ((ld & 3) == 0) && (((uintptr_t)src & 15) == 0);
     if (!TRANS) {
         // K-contiguous source: no transpose needed, straight from registers
-        if (k0 + kq >= Kp) return;
+        if (k0 + kq >= Kp) return 0u;
         float v[16];
         const uint64_t kb = k0 + kq;
         if (vec && r < R && kb + 16 <= K) {
@@ -83,8 +90,7 @@ __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t l
 #pragma unroll
             for (int j = 0; j < 16; j++) v[j] = (r < R && kb + j < K) ? to_f32(src[r * ld + kb + j]) : 0.0f;
         }
-        split_store16(v, hi, lo, panel_off(r, kb, Rp), kb + 8 < Kp);
-        return;
+        return split_store16(v, hi, lo, panel_off(r, kb, Rp), kb + 8 < Kp);
     }
     // source element (r, k) at src[k * ld + r]: read 4 consecutive r per lane
     // (16 lanes per 64-wide row), transpose through the padded LDS tile
@@ -108,12 +114,11 @@ __device__ __forceinline__ void split_tile(const T* __restrict__ src, uint64_t l
         }
     }
     __syncthreads();
-    if (k0 + kq < Kp) {
-        float v[16];
+    if (k0 + kq >= Kp) return 0u;
+    float v[16];
 #pragma unroll
-        for (int j = 0; j < 16; j++) v[j] = tile[kq + j][rr];
-        split_store16(v, hi, lo, panel_off(r, k0 + kq, Rp), k0 + kq + 8 < Kp);
-    }
+    for (int j = 0; j < 16; j++) v[j] = tile[kq + j][rr];
+    return split_store16(v, hi, lo, panel_off(r, k0 + kq, Rp), k0 + kq + 8 < Kp);
 }
 
 }  // namespace chai_hip

@@ -76,7 +76,7 @@ struct X3Shape {
 // are wave-uniform (tile origin and stage k offset folded in); a lane adds its
 // 32-bit offset within the k block and, for a 64-deep stage, one k block when
 // bit s of its mask is set.
-template <typename S>
+template <typename S, int SKIP>
 __device__ __forceinline__ void x3_issue_stage(const uint16_t* pAh, const uint16_t* pAl, const uint16_t* pBh,
                                                const uint16_t* pBl, const uint32_t (&srcA)[S::IA],
                                                const uint32_t (&srcB)[S::IB], uint32_t hiA, uint32_t hiB,
@@ -87,22 +87,22 @@ __device__ __forceinline__ void x3_issue_stage(const uint16_t* pAh, const uint16
         const int c0 = (wave * S::IA + s) * 64;
         const uint64_t o = srcA[s] + (S::CPR == 8 && ((hiA >> s) & 1u) ? a_kblk : 0);
         lds_dma16(pAh + o, st + c0);
-        lds_dma16(pAl + o, st + S::PA + c0);
+        if (!(SKIP & 1)) lds_dma16(pAl + o, st + S::PA + c0);
     }
 #pragma unroll
     for (int s = 0; s < S::IB; s++) {
         const int c0 = (wave * S::IB + s) * 64;
         const uint64_t o = srcB[s] + (S::CPR == 8 && ((hiB >> s) & 1u) ? b_kblk : 0);
         lds_dma16(pBh + o, st + 2 * S::PA + c0);
-        lds_dma16(pBl + o, st + 2 * S::PA + S::PB + c0);
+        if (!(SKIP & 2)) lds_dma16(pBl + o, st + 2 * S::PA + S::PB + c0);
     }
 }
 
-// One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is
-// free on entry and on exit (the last k step ends with a barrier), so a
-// persistent kernel may call it for tile after tile.
-template <int BM, int BN, int BK, int WGM, int WGN>
-__device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, uint64_t bz, u32x4* lds)
+// The tile body.  SKIP bit 0: the tile's Al panel is zero, bit 1: its Bl
+// panel is zero; the products with a zero panel (exact zeros) are neither
+// loaded nor multiplied.
+template <int BM, int BN, int BK, int WGM, int WGN, int SKIP>
+__device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uint32_t tn, uint64_t bz, u32x4* lds)
 {
     using S = X3Shape<BM, BN, BK, WGM, WGN>;
     const uint64_t Kp = a.Kp;
@@ -145,8 +145,8 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
 
     const int nk = (int)(Kp / BK);
     // prologue: stage 0 -> LDS buffer 0
-    x3_issue_stage<S>(a.Ah + a0, a.Al + a0, a.Bh + b0, a.Bl + b0, srcA, srcB, hiA, hiB, a.a_kblk, a.b_kblk,
-                      lds, wave);
+    x3_issue_stage<S, SKIP>(a.Ah + a0, a.Al + a0, a.Bh + b0, a.Bl + b0, srcA, srcB, hiA, hiB, a.a_kblk,
+                            a.b_kblk, lds, wave);
     wait_lds_dma();
     __syncthreads();
 
@@ -172,8 +172,8 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
         // buffer (kt + 1) & 1 was last read in step kt - 1, closed by its barrier
         if (kt + 1 < nk) {
             const uint64_t oa = a0 + (uint64_t)(kt + 1) * kstepA, ob = b0 + (uint64_t)(kt + 1) * kstepB;
-            x3_issue_stage<S>(a.Ah + oa, a.Al + oa, a.Bh + ob, a.Bl + ob, srcA, srcB, hiA, hiB, a.a_kblk,
-                              a.b_kblk, lds + ((kt + 1) & 1) * S::STAGE, wave);
+            x3_issue_stage<S, SKIP>(a.Ah + oa, a.Al + oa, a.Bh + ob, a.Bl + ob, srcA, srcB, hiA, hiB,
+                                    a.a_kblk, a.b_kblk, lds + ((kt + 1) & 1) * S::STAGE, wave);
         }
 #pragma unroll
         for (int kb = 0; kb < S::KB; kb++) {
@@ -181,24 +181,28 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
 #pragma unroll
             for (int i = 0; i < S::TI; i++) {
                 ah[i] = as_bf16x8(lds[cur + offA[i][kb]]);
-                al[i] = as_bf16x8(lds[cur + S::PA + offA[i][kb]]);
+                if (!(SKIP & 1)) al[i] = as_bf16x8(lds[cur + S::PA + offA[i][kb]]);
             }
 #pragma unroll
             for (int j = 0; j < S::TJ; j++) {
                 bh[j] = as_bf16x8(lds[cur + offB[j][kb]]);
-                bl[j] = as_bf16x8(lds[cur + S::PB + offB[j][kb]]);
+                if (!(SKIP & 2)) bl[j] = as_bf16x8(lds[cur + S::PB + offB[j][kb]]);
             }
             // three passes over the tiles: consecutive MFMAs write different accumulators
+            if (!(SKIP & 2)) {
 #pragma unroll
-            for (int i = 0; i < S::TI; i++)
+                for (int i = 0; i < S::TI; i++)
 #pragma unroll
-                for (int j = 0; j < S::TJ; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < S::TJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+            }
+            if (!(SKIP & 1)) {
 #pragma unroll
-            for (int i = 0; i < S::TI; i++)
+                for (int i = 0; i < S::TI; i++)
 #pragma unroll
-                for (int j = 0; j < S::TJ; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < S::TJ; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+            }
 #pragma unroll
             for (int i = 0; i < S::TI; i++)
 #pragma unroll
@@ -223,6 +227,33 @@ __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t
                 if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
             }
         }
+}
+
+// One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is
+// free on entry and on exit (the last k step ends with a barrier), so a
+// persistent kernel may call it for tile after tile.  With split flags the
+// block first ORs the flags of its A and B panels (a block-uniform decision).
+template <int BM, int BN, int BK, int WGM, int WGN>
+__device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, uint64_t bz, u32x4* lds)
+{
+    int skip = 0;
+    if (a.a_nz) {
+        constexpr int THREADS = WGM * WGN * 64;
+        const uint64_t kb64 = a.Kp / 64;
+        const uint64_t na = (uint64_t)(BM / 64) * kb64, nb = (uint64_t)(BN / 64) * kb64;
+        const uint8_t* fa = a.a_nz + bz * a.a_nz_bstride + (uint64_t)tm * na;
+        const uint8_t* fb = a.b_nz + bz * a.b_nz_bstride + (uint64_t)tn * nb;
+        int nz_a = 0, nz_b = 0;
+        for (uint64_t i = threadIdx.x; i < na; i += THREADS) nz_a |= fa[i];
+        for (uint64_t i = threadIdx.x; i < nb; i += THREADS) nz_b |= fb[i];
+        skip = (__syncthreads_or(nz_a) ? 0 : 1) | (__syncthreads_or(nz_b) ? 0 : 2);
+    }
+    switch (skip) {
+        case 0: x3_tile_body<BM, BN, BK, WGM, WGN, 0>(a, tm, tn, bz, lds); break;
+        case 1: x3_tile_body<BM, BN, BK, WGM, WGN, 1>(a, tm, tn, bz, lds); break;
+        case 2: x3_tile_body<BM, BN, BK, WGM, WGN, 2>(a, tm, tn, bz, lds); break;
+        default: x3_tile_body<BM, BN, BK, WGM, WGN, 3>(a, tm, tn, bz, lds); break;
+    }
 }
 
 // The autotuner's search space for the split-BF16x3 family, in variant-id

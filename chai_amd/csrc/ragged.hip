@@ -112,7 +112,7 @@ __global__ __launch_bounds__(WGM * WGN * 64, MINB) void ragged_x3_kernel(
             g.C = I.C; g.ldc = I.ldc;
             g.M = I.M; g.N = I.N;
             g.epi = epi.on; g.epi_scale = epi.scale; g.epi_shift = epi.shift; g.epi_lo = epi.lo;
-            x3_tile<BM, BN, BK, WGM, WGN>(g, w.y >> 16, w.y & 0xffffu, 0, lds);
+            x3_tile_body<BM, BN, BK, WGM, WGN, 0>(g, w.y >> 16, w.y & 0xffffu, 0, lds);   // f32 operands: no zero-panel flags
         }
     }
 }

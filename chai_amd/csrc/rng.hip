@@ -1,8 +1,9 @@
 // rng.hip -- chai_hip_rng_fill: materialise Chai's RNG stream into HBM.
 //
-// One thread per generated vector (4 x 32-bit or 2 x 64-bit elements), so a
-// wavefront writes 64 contiguous 16-byte vectors = 1 KiB per store instruction
-// when the row width is a multiple of the vector length (fully coalesced).
+// A thread generates one vector (4 x 32-bit or 2 x 64-bit elements) per step
+// of a grid-stride loop, so a wavefront writes 64 contiguous 16-byte vectors =
+// 1 KiB per store instruction when the row width is a multiple of the vector
+// length (fully coalesced).
 // vidx follows the width-padded layout of kernel/Subscript.cpp:359-380 so the
 // stream is independent of how the kernel is tiled.
 #include "common.hpp"
@@ -22,11 +23,10 @@ struct RngGeom {
     int has_limits, lo_nonzero;
 };
 
+// vector v of the stream -> its elements in `out`
 template <typename T, int VARIANT, int VL, int NORMAL>
-__global__ __launch_bounds__(256) void rng_fill_32(RngGeom g, T* __restrict__ out)
+__device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out, uint64_t v)
 {
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= g.nvec) return;
     // rows without padding (W a multiple of VL) are contiguous in the vector
     // index: element offset v * VL, no division
     uint64_t row = 0, x0 = v * VL, n = VL;
@@ -65,11 +65,19 @@ __global__ __launch_bounds__(256) void rng_fill_32(RngGeom g, T* __restrict__ ou
     }
 }
 
-template <int VARIANT, int VL, int NORMAL>
-__global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict__ out)
+// Grid-stride over the vectors: a bounded grid keeps every wave busy for many
+// vectors (one 1 KiB store per wave per step) instead of one vector per thread.
+template <typename T, int VARIANT, int VL, int NORMAL>
+__global__ __launch_bounds__(256) void rng_fill_32(RngGeom g, T* __restrict__ out)
 {
-    const uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= g.nvec) return;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nvec; v += stride)
+        rng_vec_32<T, VARIANT, VL, NORMAL>(g, out, v);
+}
+
+template <int VARIANT, int VL, int NORMAL>
+__device__ __forceinline__ void rng_vec_64(const RngGeom& g, double* __restrict__ out, uint64_t v)
+{
     uint64_t row = 0, x0 = v * VL, n = VL;
     if (g.W != g.vpr * VL) {
         row = v / g.vpr;
@@ -96,10 +104,26 @@ __global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict
     for (uint64_t i = 0; i < n; i++) dst[i] = val[i];
 }
 
+template <int VARIANT, int VL, int NORMAL>
+__global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict__ out)
+{
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nvec; v += stride)
+        rng_vec_64<VARIANT, VL, NORMAL>(g, out, v);
+}
+
+// blocks for n vectors: one per 256, at most two rounds of full residency
+// (256 CUs x 8 blocks of 256 threads)
+static unsigned rng_blocks(uint64_t nvec)
+{
+    const uint64_t b = (nvec + 255) / 256;
+    return (unsigned)(b < 4096 ? b : 4096);
+}
+
 template <typename T, int VARIANT, int NORMAL>
 static void launch32(int vl, const RngGeom& g, void* out, hipStream_t st)
 {
-    const unsigned blocks = ceil_div_u(g.nvec, 256);
+    const unsigned blocks = rng_blocks(g.nvec);
     if (vl == 4) rng_fill_32<T, VARIANT, 4, NORMAL><<<blocks, 256, 0, st>>>(g, (T*)out);
     else if (vl == 2) rng_fill_32<T, VARIANT, 2, NORMAL><<<blocks, 256, 0, st>>>(g, (T*)out);
     else rng_fill_32<T, VARIANT, 1, NORMAL><<<blocks, 256, 0, st>>>(g, (T*)out);
@@ -116,7 +140,7 @@ static int dispatch(int prec, int vl, int normal, const RngGeom& g, void* out, h
             else launch32<float, VARIANT, 0>(vl, g, out, st);
             break;
         case CHAI_F64: {
-            const unsigned blocks = ceil_div_u(g.nvec, 256);
+            const unsigned blocks = rng_blocks(g.nvec);
             if (vl == 2) {
                 if (normal) rng_fill_64<VARIANT, 2, 1><<<blocks, 256, 0, st>>>(g, (double*)out);
                 else rng_fill_64<VARIANT, 2, 0><<<blocks, 256, 0, st>>>(g, (double*)out);

@@ -114,8 +114,54 @@ def test_gemm_mixed_int_float(chai, oracle):
                   out_dtype=torch.float32).cpu().numpy()
     C64 = Ai.astype(np.float64) @ B.astype(np.float64)
     absprod = np.abs(Ai.astype(np.float64)) @ np.abs(B.astype(np.float64))
-    # the int operand is exact in bf16 (|a| < 256): only B's split residual and f32 accumulation
-    assert np.all(np.abs(C - C64) <= (2.0 ** -17 + K * U) * absprod)
+    # the int operand is exact in bf16 (|a| <= 256, so its lo panel is zero and
+    # the al*bh products are skipped): only B's split residual (2^-16) and the
+    # f32 accumulation of 2K products remain
+    g2 = 2 * K * U / (1 - 2 * K * U)
+    assert np.all(np.abs(C - C64) <= (2.0 ** -16 + g2) * absprod)
+
+
+def test_gemm_int_not_exact_in_bf16(chai, oracle):
+    """i32 values with more than 8 significant bits: lo is nonzero, all three
+    products run, the general split bound holds."""
+    rng = np.random.default_rng(8)
+    M, N, K = 300, 200, 400
+    Ai = rng.integers(-(1 << 20), 1 << 20, size=(M, K)).astype(np.int32)
+    B = unif(oracle, 4, K, N)
+    C = chai.gemm(torch.from_numpy(Ai).cuda(), torch.from_numpy(B).cuda(), mode=2,
+                  out_dtype=torch.float32).cpu().numpy()
+    A64 = Ai.astype(np.float64)
+    C64 = A64 @ B.astype(np.float64)
+    absprod = np.abs(A64) @ np.abs(B.astype(np.float64))
+    assert np.all(np.abs(C - C64) <= bound("bf16x3", K) * absprod)
+
+
+@pytest.mark.parametrize("M,N,K", [(256, 256, 1024), (300, 200, 77), (1000, 1100, 600)])
+def test_gemm_bf16_exact_operands_are_exact(chai, M, N, K):
+    """Integer-valued f32 operands in [-8, 8] are exact in bf16: both lo panels
+    are zero, every tile runs the hi*hi product alone, and with |C| < 2^24 every
+    partial sum is exact, so C equals the FP64 product bit for bit."""
+    rng = np.random.default_rng(M + N + K)
+    A = rng.integers(-8, 9, size=(M, K)).astype(np.float32)
+    B = rng.integers(-8, 9, size=(K, N)).astype(np.float32)
+    C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), mode=2).cpu().numpy()
+    C64 = A.astype(np.float64) @ B.astype(np.float64)
+    assert np.array_equal(C.astype(np.float64), C64)
+
+
+def test_gemm_zero_lo_panels_per_tile(chai, oracle):
+    """Tiles decide separately: A is bf16-exact except in rows 300..309 (one
+    tile row), B is general.  Tiles of exact A rows skip the al*bh products;
+    every tile meets the split bound against FP64."""
+    rng = np.random.default_rng(5)
+    M, N, K = 1024, 512, 700
+    A = rng.integers(-100, 101, size=(M, K)).astype(np.float32)
+    A[300:310] = unif(oracle, 9, 10, K)
+    B = unif(oracle, 10, K, N)
+    C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), mode=2).cpu().numpy()
+    A64, B64 = A.astype(np.float64), B.astype(np.float64)
+    err = np.abs(C - A64 @ B64)
+    assert np.all(err <= bound("bf16x3", K) * (np.abs(A64) @ np.abs(B64)))
 
 
 @pytest.mark.parametrize("M,N,K", [(128, 128, 128), (65, 97, 33), (512, 384, 640)])
