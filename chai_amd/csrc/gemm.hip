@@ -14,13 +14,17 @@
 //                     from 32x32x16 bf16 MFMAs; three products hi*lo + lo*hi +
 //                     hi*hi accumulate in FP32; double-buffered XOR-swizzled LDS.
 //   sgemm_fma       : plain FP32 VALU FMA, 128x128x16 tiles, 8x8 outputs/thread.
-//   dgemm_mfma      : FP64 with 16x16x4 f64 MFMAs, 128x128x16 tiles, 4 waves.
+//   dgemm_mfma      : FP64 with 16x16x4 f64 MFMAs, 64 x 64 per wave, three tile
+//                     shapes (128x128, 256x128, 128x256), stages copied into LDS directly.
 #include "common.hpp"
 #include "gemm_device.hpp"
 #include "gemm_internal.hpp"
 #include "gemm_split.hpp"
 #include "gemm_x3.hpp"
 
+#include <algorithm>
+#include <cmath>
+#include <mutex>
 #include <type_traits>
 
 namespace chai_hip {
@@ -191,22 +195,34 @@ __global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
 // FP64 MFMA GEMM
 // =====================================================================
 
-constexpr int D_BM = 128, D_BN = 128, D_BK = 8;
+constexpr int D_BK = 8;      // k rows per LDS stage
+constexpr int D_PAD = 16;    // doubles after each LDS row: rows k and k + 1 start 128 B apart mod 256 B
 
-// A packed [Mp][Kp] f64, B packed [Kp][Np] f64.
-__global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
+// A^T packed [Kp][Mp] f64 and B packed [Kp][Np] f64 (k-major, zero padded).
+// Block tile BM x BN, WGM x WGN waves of 64 x 64 (4 x 4 v_mfma_f64_16x16x4f64
+// accumulators).  Each stage is 8 k rows of both panels, copied straight into
+// LDS one 1 KiB row piece per wave issue (global_load_lds_dwordx4); the padded
+// row pitch makes the fragment reads (16 lanes per row, 4 rows) conflict free.
+template <int BM, int BN, int WGM, int WGN, int MINB>
+__global__ __launch_bounds__(WGM * WGN * 64, MINB) void dgemm_mfma(GemmArgs a)
 {
-    __shared__ double As[2][D_BK][D_BM + 2];   // k-major (A transposed on the way in)
-    __shared__ double Bs[2][D_BK][D_BN + 2];
+    constexpr int WAVES = WGM * WGN;
+    constexpr int PA = BM + D_PAD, PB = BN + D_PAD;          // LDS row pitch (doubles)
+    constexpr int STAGE = D_BK * (PA + PB);                   // doubles per stage
+    constexpr int RA = BM / 128, RB = BN / 128;               // 1 KiB pieces per k row
+    constexpr int NI = D_BK * (RA + RB), IW = NI / WAVES;     // issues per stage, per wave
+    static_assert(BM % 128 == 0 && BN % 128 == 0 && NI % WAVES == 0, "whole issues per wave");
+    static_assert(BM / WGM == 64 && BN / WGN == 64, "64 x 64 per wave");
+    extern __shared__ double dlds[];
     uint32_t tm, tn;
     tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
     const uint64_t bz = blockIdx.z;
-    const uint64_t Kp = a.Kp, Np = a.Np;
-    const double* A = a.Ad + bz * a.pa_bstride + (uint64_t)tm * D_BM * Kp;
-    const double* B = a.Bd + bz * a.pb_bstride + (uint64_t)tn * D_BN;
+    const uint64_t Kp = a.Kp, Np = a.Np, Mp = a.Mp;
+    const double* At = a.Ad + bz * a.pa_bstride + (uint64_t)tm * BM;
+    const double* Bp = a.Bd + bz * a.pb_bstride + (uint64_t)tn * BN;
     double* C = a.Cd + bz * a.c_bstride;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int wm = (wave / WGN) * 64, wn = (wave % WGN) * 64;
 
     doublex4 acc[4][4];
 #pragma unroll
@@ -214,60 +230,56 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = (doublex4){0.0, 0.0, 0.0, 0.0};
 
-    // A tile 128 x 8 doubles: thread loads row t>>1, 4 consecutive k at (t&1)*4
-    // B tile 8 x 128 doubles: thread loads k row t>>5, 4 consecutive n at (t&31)*4
-    f64x2 ra[2], rb[2];
-    auto gload = [&](int kt) {
-        const uint64_t k0 = (uint64_t)kt * D_BK;
-        const f64x2* pa = reinterpret_cast<const f64x2*>(A + (uint64_t)(t >> 1) * Kp + k0 + (t & 1) * 4);
-        const f64x2* pb = reinterpret_cast<const f64x2*>(B + (k0 + (t >> 5)) * Np + (t & 31) * 4);
-#pragma unroll
-        for (int h = 0; h < 2; h++) { ra[h] = pa[h]; rb[h] = pb[h]; }
-    };
-    auto lstore = [&](int buf) {
-        const int row = t >> 1, kc = (t & 1) * 4;
-#pragma unroll
-        for (int h = 0; h < 2; h++) {
-            As[buf][kc + 2 * h][row] = ra[h].x;
-            As[buf][kc + 2 * h + 1][row] = ra[h].y;
-            Bs[buf][t >> 5][(t & 31) * 4 + 2 * h] = rb[h].x;
-            Bs[buf][t >> 5][(t & 31) * 4 + 2 * h + 1] = rb[h].y;
-        }
-    };
     const int nk = (int)(Kp / D_BK);
-    gload(0);
-    lstore(0);
+#define DGEMM_ISSUE(KT, ST)                                                                          \
+    {                                                                                                \
+        _Pragma("unroll") for (int s = 0; s < IW; s++) {                                             \
+            const int j = wave * IW + s;   /* wave-uniform */                                        \
+            if (j < D_BK * RA) {                                                                     \
+                const int k = j / RA, h = j % RA;                                                    \
+                lds_dma16(At + ((uint64_t)(KT) * D_BK + k) * Mp + h * 128 + lane * 2,                \
+                          (ST) + k * PA + h * 128);                                                  \
+            } else {                                                                                 \
+                const int k = (j - D_BK * RA) / RB, h = (j - D_BK * RA) % RB;                        \
+                lds_dma16(Bp + ((uint64_t)(KT) * D_BK + k) * Np + h * 128 + lane * 2,                \
+                          (ST) + D_BK * PA + k * PB + h * 128);                                      \
+            }                                                                                        \
+        }                                                                                            \
+    }
+    DGEMM_ISSUE(0, dlds)
+    wait_lds_dma();
     __syncthreads();
     for (int kt = 0; kt < nk; kt++) {
-        const int buf = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
+        const double* cur = dlds + (kt & 1) * STAGE;
+        if (kt + 1 < nk) DGEMM_ISSUE(kt + 1, dlds + ((kt + 1) & 1) * STAGE)
 #pragma unroll
         for (int ks = 0; ks < D_BK / 4; ks++) {
             const int k = ks * 4 + (lane >> 4);
             double av[4], bv[4];
 #pragma unroll
-            for (int i = 0; i < 4; i++) av[i] = As[buf][k][wm + i * 16 + (lane & 15)];
+            for (int i = 0; i < 4; i++) av[i] = cur[k * PA + wm + i * 16 + (lane & 15)];
 #pragma unroll
-            for (int j = 0; j < 4; j++) bv[j] = Bs[buf][k][wn + j * 16 + (lane & 15)];
+            for (int j = 0; j < 4; j++) bv[j] = cur[D_BK * PA + k * PB + wn + j * 16 + (lane & 15)];
 #pragma unroll
             for (int i = 0; i < 4; i++)
 #pragma unroll
                 for (int j = 0; j < 4; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
-        if (kt + 1 < nk) lstore(buf ^ 1);
+        wait_lds_dma();
         __syncthreads();
     }
+#undef DGEMM_ISSUE
     // f64 16x16x4 output layout (measured on gfx950): lane -> column lane%16;
     // register r -> row 4*r + lane/16 (A row i supplied by lane i + 16k).
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const uint64_t col = (uint64_t)tn * D_BN + wn + j * 16 + (lane & 15);
+            const uint64_t col = (uint64_t)tn * BN + wn + j * 16 + (lane & 15);
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const uint64_t row = (uint64_t)tm * D_BM + wm + i * 16 + 4 * r + (lane >> 4);
+                const uint64_t row = (uint64_t)tm * BM + wm + i * 16 + 4 * r + (lane >> 4);
                 if (row < a.M && col < a.N) {
                     const uint64_t idx = row * a.ldc + col;
                     const double v = acc[i][j][r];
@@ -276,6 +288,19 @@ __global__ __launch_bounds__(256) void dgemm_mfma(GemmArgs a)
             }
         }
 }
+
+struct DVariant {
+    int BM, BN, WGM, WGN;
+    size_t lds;
+    void (*kernel)(GemmArgs);
+};
+#define CHAI_DV(BM, BN, WGM, WGN, MINB) \
+    {BM, BN, WGM, WGN, 2 * (size_t)D_BK * (BM + D_PAD + BN + D_PAD) * 8, dgemm_mfma<BM, BN, WGM, WGN, MINB>}
+// FP64 tile variants (the tuner's search space for mode F64), with the
+// resident blocks per CU the register budget is sized for
+static const DVariant kDVariants[] = {CHAI_DV(128, 128, 2, 2, 2), CHAI_DV(256, 128, 4, 2, 1),
+                                      CHAI_DV(128, 256, 2, 4, 1)};
+static constexpr int kNumD = sizeof(kDVariants) / sizeof(kDVariants[0]);
 
 // =====================================================================
 // host side
@@ -351,6 +376,37 @@ static void pack_any(int prec, const void* src, uint64_t ld, bool trans, uint64_
         case CHAI_I32: launch_pack<int32_t, TO>(src, ld, trans, R, Cn, Rp, Cp, src_bstride, nb, dst, st); break;
         default: launch_pack<uint32_t, TO>(src, ld, trans, R, Cn, Rp, Cp, src_bstride, nb, dst, st); break;
     }
+}
+
+// FP64 tile without tuning: the fewest tile-rounds x tile area, i.e. the least
+// MFMA time once the last partial round of resident blocks is counted.
+static int f64_default_variant(uint64_t M, uint64_t N, uint64_t nb)
+{
+    static int slots[kNumD] = {};
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lk(mu);
+    int best = 0;
+    double best_cost = 0;
+    for (int v = 0; v < kNumD; v++) {
+        const DVariant& d = kDVariants[v];
+        if (!slots[v]) {
+            int dev = 0, cus = 0, per_cu = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+                hipFuncSetAttribute((const void*)d.kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)d.lds) !=
+                    hipSuccess ||
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, d.kernel, d.WGM * d.WGN * 64, d.lds) !=
+                    hipSuccess) {
+                set_error("gemm: f64 occupancy query failed");
+                return -1;
+            }
+            slots[v] = std::max(1, cus * std::max(per_cu, 1));
+        }
+        const double tiles = (double)((M + d.BM - 1) / d.BM) * (double)((N + d.BN - 1) / d.BN) * (double)nb;
+        const double cost = std::ceil(tiles / slots[v]) * d.BM * d.BN;
+        if (v == 0 || cost < best_cost) { best = v; best_cost = cost; }
+    }
+    return best;
 }
 
 static void set_epilogue(GemmArgs& g, const chai_epilogue* epi)
@@ -429,7 +485,14 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
     }
     if (mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_F64) {
         const bool f64 = mode == CHAI_GEMM_F64;
-        const uint64_t BMx = f64 ? D_BM : F_BM, BNx = f64 ? D_BN : F_BN, BKx = f64 ? D_BK : F_BK;
+        int dv = 0;
+        if (f64) {
+            if (variant >= kNumD) { set_error("gemm: f64 variant %d out of range", variant); return CHAI_ERR_ARG; }
+            dv = variant >= 0 ? variant : f64_default_variant(M, N, nb);
+            if (dv < 0) return CHAI_ERR_HIP;
+        }
+        const uint64_t BMx = f64 ? kDVariants[dv].BM : F_BM, BNx = f64 ? kDVariants[dv].BN : F_BN;
+        const uint64_t BKx = f64 ? D_BK : F_BK;
         const uint64_t Mp = round_up(M, BMx), Np = round_up(N, BNx), Kp = round_up(K, 32);
         const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
         const size_t es = f64 ? 8 : 4;
@@ -438,17 +501,18 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         CHAI_HIP_TRY(hipMallocAsync(&ws, bytes, st));
         char* pa = (char*)ws;
         char* pb = pa + es * Mp * Kp * na;
-        // A packed [Mp][Kp] (op(A) row-major), B packed [Kp][Np] (op(B) row-major)
+        // FMA: A packed [Mp][Kp] (op(A) row-major); FP64: A^T packed [Kp][Mp]
+        // (element (k, m) = op(A)[m][k]); B packed [Kp][Np] (op(B) row-major)
         prof_mark(0, st);
         if (f64) {
-            pack_any<double>(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, (double*)pa, st);
+            pack_any<double>(d.prec_a, A, d.lda, d.trans_a == 0, K, M, Kp, Mp, d.stride_a, na, (double*)pa, st);
             pack_any<double>(d.prec_b, B, d.ldb, d.trans_b != 0, K, N, Kp, Np, d.stride_b, nbb, (double*)pb, st);
         } else {
             pack_any<float>(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, na, (float*)pa, st);
             pack_any<float>(d.prec_b, B, d.ldb, d.trans_b != 0, K, N, Kp, Np, d.stride_b, nbb, (float*)pb, st);
         }
         GemmArgs g{};
-        g.Kp = Kp; g.Np = Np;
+        g.Kp = Kp; g.Np = Np; g.Mp = Mp;
         g.pa_bstride = d.same_a ? 0 : Mp * Kp;
         g.pb_bstride = d.same_b ? 0 : Kp * Np;
         g.c_bstride = d.stride_c; g.ldc = d.ldc;
@@ -461,7 +525,14 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         prof_mark(1, st);
         if (f64) {
             g.Ad = (const double*)pa; g.Bd = (const double*)pb; g.Cd = (double*)C;
-            dgemm_mfma<<<grid, 256, 0, st>>>(g);
+            const DVariant& v = kDVariants[dv];
+            static bool d_attr[kNumD] = {};
+            if (!d_attr[dv]) {
+                CHAI_HIP_TRY(hipFuncSetAttribute((const void*)v.kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)v.lds));
+                d_attr[dv] = true;
+            }
+            hipLaunchKernelGGL(v.kernel, grid, dim3(v.WGM * v.WGN * 64), v.lds, st, g);
         } else {
             g.Af = (const float*)pa; g.Bf = (const float*)pb; g.C = (float*)C;
             sgemm_fma<<<grid, 256, 0, st>>>(g);
@@ -482,10 +553,15 @@ using namespace chai_hip;
 
 namespace chai_hip {
 
-int gemm_num_variants(int mode) { return mode == CHAI_GEMM_BF16X3 ? kNumX3 : 1; }
+int gemm_num_variants(int mode) { return mode == CHAI_GEMM_BF16X3 ? kNumX3 : mode == CHAI_GEMM_F64 ? kNumD : 1; }
 
 int gemm_variant_params(int mode, int vid, int* p)
 {
+    if (mode == CHAI_GEMM_F64 && vid >= 0 && vid < kNumD) {
+        const DVariant& v = kDVariants[vid];
+        p[0] = v.BM; p[1] = v.BN; p[2] = D_BK; p[3] = v.WGM; p[4] = v.WGN;
+        return CHAI_OK;
+    }
     if (mode != CHAI_GEMM_BF16X3 || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;
     const X3Variant& v = kX3Variants[vid];
     p[0] = v.BM; p[1] = v.BN; p[2] = v.BK; p[3] = v.WGM; p[4] = v.WGN;

@@ -51,6 +51,19 @@ __device__ __forceinline__ void split_bf16x2(float a0, float a1, uint32_t& hi, u
     lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(r, bf16x2_t));
 }
 
+typedef __attribute__((address_space(1))) void global_void_t;
+typedef __attribute__((address_space(3))) void lds_void_t;
+
+// 16 bytes per lane from global memory into LDS (global_load_lds_dwordx4):
+// lane i of the wave writes bytes [16 i, 16 i + 16) after dst (dst is
+// wave-uniform).  Completion is counted by vmcnt.
+__device__ __forceinline__ void lds_dma16(const void* src, void* dst)
+{
+    __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)dst, 16, 0, 0);
+}
+
+__device__ __forceinline__ void wait_lds_dma() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
+
 // XCD-aware tile order: the dispatcher hands consecutive workgroups to the 8
 // XCDs round-robin; remap so that the tiles one XCD runs together are
 // neighbours (shared A row panels / B column panels stay in that XCD's L2),

@@ -18,12 +18,13 @@ struct GemmArgs {
     // Al (Bl) panel is zero skips the products with it (null: never skip).
     const uint8_t *a_nz, *b_nz;
     uint64_t a_nz_bstride, b_nz_bstride;
-    // packed operands for the FMA / FP64 kernels: A [Mp][Kp], B [Kp][Np]
+    // packed operands: FMA kernel A [Mp][Kp], B [Kp][Np]; FP64 kernel A^T
+    // [Kp][Mp], B [Kp][Np] (both k-major)
     const float *Af, *Bf;
     const double *Ad, *Bd;
     float* C;
     double* Cd;
-    uint64_t Kp, Np;
+    uint64_t Kp, Np, Mp;
     uint64_t pa_bstride, pb_bstride, c_bstride;   // elements between batch instances
     uint64_t ldc, M, N;
     uint32_t tiles_m, tiles_n;

@@ -32,19 +32,6 @@ __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v)
     return r;
 }
 
-typedef __attribute__((address_space(1))) void global_void_t;
-typedef __attribute__((address_space(3))) void lds_void_t;
-
-// 16 bytes per lane from global memory into LDS: lane i of the wave writes
-// bytes [16 i, 16 i + 16) after dst (dst is wave-uniform).  Completion is
-// counted by vmcnt.
-__device__ __forceinline__ void lds_dma16(const uint16_t* src, u32x4* dst)
-{
-    __builtin_amdgcn_global_load_lds((global_void_t*)src, (lds_void_t*)dst, 16, 0, 0);
-}
-
-__device__ __forceinline__ void wait_lds_dma() { asm volatile("s_waitcnt vmcnt(0)" ::: "memory"); }
-
 __device__ __forceinline__ void store_c(float* C, uint64_t idx, float v, const GemmArgs& a)
 {
     if (a.general) v = (float)(a.alpha * (double)v + a.beta * (double)C[idx]);

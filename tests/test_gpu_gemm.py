@@ -48,7 +48,7 @@ def check(mode, C, A, B, oracle, ta=False, tb=False):
     return err
 
 
-MODES = {"fma": 1, "bf16x3": 2}
+MODES = {"fma": 1, "bf16x3": 2, "f64": 3}
 
 
 @pytest.mark.parametrize("mode", ["fma", "bf16x3"])
@@ -171,6 +171,24 @@ def test_dgemm(chai, oracle, M, N, K):
     check("f64", C, A, B, oracle)
     ref = oracle.gemm_f64ref(A, B, nthreads=4)
     assert np.all(np.abs(C - ref) <= 4 * K * 2.0 ** -53 * (np.abs(A) @ np.abs(B)))
+
+
+def test_dgemm_variants_agree(chai, oracle):
+    """Every FP64 tile shape accumulates each output over k in the same order
+    (4-deep MFMA steps), so all variants give the same bits; the default choice
+    is one of them."""
+    M, N, K = 300, 520, 200
+    A = unif(oracle, 11, M, K, np.float64); B = unif(oracle, 12, K, N, np.float64)
+    At, Bt = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    outs = [chai.gemm(At, Bt, mode=MODES["f64"], variant=v) for v in range(len(chai.gemm_variants(MODES["f64"])))]
+    assert len(outs) == 3
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    assert torch.equal(chai.gemm(At, Bt, mode=MODES["f64"]), outs[0])
+    check("f64", outs[0].cpu().numpy(), A, B, oracle)
+    # transposed operands (packed k-major on the way in) give the same bits
+    Ct = chai.gemm(At.t().contiguous(), Bt.t().contiguous(), trans_a=True, trans_b=True, mode=MODES["f64"], variant=1)
+    assert torch.equal(Ct, outs[0])
 
 
 def test_gemm_8192_sampled_rows(chai, oracle):
