@@ -55,33 +55,87 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, u
 }
 
 // Pack op(X) into a dense zero-padded [Rp][Cp] array of TO (element (r, c) of
-// op(X) at src[r*ld + c] or, transposed, src[c*ld + r]).
+// op(X) at src[r*ld + c] or, transposed, src[c*ld + r]).  A block covers 64 x 64
+// elements; each thread writes 16 consecutive c of one row as 4-element
+// vectors.  A row-major source is copied straight through registers (4-element
+// vector loads when aligned); a transposed one goes through a padded LDS tile,
+// read 4 consecutive r per lane.
+template <typename T, int N>
+__device__ __forceinline__ void load_vec(const T* p, T* v)
+{
+    typedef T TN __attribute__((ext_vector_type(N)));
+    const TN x = *reinterpret_cast<const TN*>(p);
+#pragma unroll
+    for (int i = 0; i < N; i++) v[i] = x[i];
+}
+
+template <typename TO>
+__device__ __forceinline__ void store16(TO* p, const TO* v, uint64_t n)
+{
+    typedef TO TO4 __attribute__((ext_vector_type(4)));
+    if (n == 16 && ((uintptr_t)p & (4 * sizeof(TO) - 1)) == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            TO4 x = {v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+            reinterpret_cast<TO4*>(p)[q] = x;
+        }
+    } else {
+        for (uint64_t j = 0; j < n; j++) p[j] = v[j];
+    }
+}
+
 template <typename T, typename TO, bool TRANS>
 __global__ __launch_bounds__(256) void pack_kernel(const T* __restrict__ src, uint64_t ld, uint64_t R,
-                                                   uint64_t Cn, uint64_t Cp, uint64_t src_bstride,
+                                                   uint64_t Cn, uint64_t Rp, uint64_t Cp, uint64_t src_bstride,
                                                    uint64_t dst_bstride, TO* __restrict__ dst)
 {
-    __shared__ TO tile[32][33];
-    const uint64_t r0 = (uint64_t)blockIdx.y * 32, c0 = (uint64_t)blockIdx.x * 32;
+    __shared__ TO tile[64][65];
+    const uint64_t r0 = (uint64_t)blockIdx.y * 64, c0 = (uint64_t)blockIdx.x * 64;
     src += (uint64_t)blockIdx.z * src_bstride;
     dst += (uint64_t)blockIdx.z * dst_bstride;
     const int t = threadIdx.x;
-    for (int i = 0; i < 4; i++) {
-        const int a = i * 8 + (t >> 5), b = t & 31;
-        if (TRANS) {   // read along r (contiguous in src)
-            const uint64_t r = r0 + b, c = c0 + a;
-            tile[a][b] = (r < R && c < Cn) ? (TO)src[c * ld + r] : (TO)0;
+    const bool vec = ((ld & 3) == 0) && (((uintptr_t)src & (4 * sizeof(T) - 1)) == 0);
+    const int rr = t >> 2, cq = (t & 3) * 16;     // this thread's output row and first column
+    const uint64_t r = r0 + rr, cb = c0 + cq;
+    TO v[16];
+    if (!TRANS) {
+        if (vec && r < R && cb + 16 <= Cn) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                T x[4];
+                load_vec<T, 4>(src + r * ld + cb + 4 * q, x);
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[4 * q + i] = (TO)x[i];
+            }
         } else {
-            const uint64_t r = r0 + a, c = c0 + b;
-            tile[b][a] = (r < R && c < Cn) ? (TO)src[r * ld + c] : (TO)0;
+#pragma unroll
+            for (int j = 0; j < 16; j++) v[j] = (r < R && cb + j < Cn) ? (TO)src[r * ld + cb + j] : (TO)0;
         }
+    } else {
+        // source (r, c) at src[c * ld + r]: 16 lanes read 64 consecutive r of one c
+        const int r4 = (t & 15) * 4, cc0 = t >> 4;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int cc = cc0 + 16 * i;
+            const uint64_t c = c0 + cc;
+            if (vec && r0 + r4 + 4 <= R && c < Cn) {
+                T x[4];
+                load_vec<T, 4>(src + c * ld + r0 + r4, x);
+#pragma unroll
+                for (int j = 0; j < 4; j++) tile[cc][r4 + j] = (TO)x[j];
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint64_t rj = r0 + r4 + j;
+                    tile[cc][r4 + j] = (rj < R && c < Cn) ? (TO)src[c * ld + rj] : (TO)0;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 16; j++) v[j] = tile[cq + j][rr];
     }
-    __syncthreads();
-    for (int i = 0; i < 4; i++) {
-        const int a = i * 8 + (t >> 5), b = t & 31;   // write row r0+a, col c0+b
-        const uint64_t c = c0 + b;
-        if (c < Cp) dst[(r0 + a) * Cp + c] = tile[b][a];
-    }
+    if (r < Rp && cb < Cp) store16(dst + r * Cp + cb, v, cb + 16 <= Cp ? 16 : Cp - cb);
 }
 
 // =====================================================================
@@ -359,11 +413,11 @@ static void launch_pack(const void* src, uint64_t ld, bool trans, uint64_t R, ui
                         uint64_t Rp, uint64_t Cp, uint64_t src_bstride, uint64_t nb, TO* dst,
                         hipStream_t st)
 {
-    dim3 grid((unsigned)(Cp / 32), (unsigned)(Rp / 32), (unsigned)nb);
+    dim3 grid((unsigned)((Cp + 63) / 64), (unsigned)((Rp + 63) / 64), (unsigned)nb);
     if (trans)
-        pack_kernel<T, TO, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, Cn, Cp, src_bstride, Rp * Cp, dst);
+        pack_kernel<T, TO, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, Cn, Rp, Cp, src_bstride, Rp * Cp, dst);
     else
-        pack_kernel<T, TO, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, Cn, Cp, src_bstride, Rp * Cp, dst);
+        pack_kernel<T, TO, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, Cn, Rp, Cp, src_bstride, Rp * Cp, dst);
 }
 
 template <typename TO>
