@@ -71,6 +71,8 @@ def lib():
         L.chai_hip_profile_enable.argtypes = [i]
         L.chai_hip_gemm_last_timing.argtypes = [ctypes.POINTER(ctypes.c_float),
                                                 ctypes.POINTER(ctypes.c_float)]
+        L.chai_hip_gemm_profile_mean.argtypes = [i, ctypes.POINTER(ctypes.c_float),
+                                                 ctypes.POINTER(ctypes.c_float)]
         _lib = L
     return _lib
 
@@ -324,6 +326,14 @@ def gemm_last_timing():
     """(pre-pass ms, core kernel ms) of this thread's last profiled GEMM."""
     a, b = ctypes.c_float(), ctypes.c_float()
     check(lib().chai_hip_gemm_last_timing(ctypes.byref(a), ctypes.byref(b)), "gemm_last_timing")
+    return a.value, b.value
+
+
+def gemm_profile_mean(count):
+    """(mean pre-pass ms, mean core kernel ms) over this thread's last `count`
+    profiled GEMMs (at most 64 kept since profiling was enabled)."""
+    a, b = ctypes.c_float(), ctypes.c_float()
+    check(lib().chai_hip_gemm_profile_mean(int(count), ctypes.byref(a), ctypes.byref(b)), "gemm_profile_mean")
     return a.value, b.value
 
 

@@ -362,25 +362,33 @@ static constexpr int kNumD = sizeof(kDVariants) / sizeof(kDVariants[0]);
 
 static uint64_t round_up(uint64_t x, uint64_t m) { return (x + m - 1) / m * m; }
 
-// ---- optional per-kernel timing of the calling thread's last GEMM
+// ---- optional per-kernel timing of the calling thread's GEMMs: a ring of
+// event triples (before the pre-pass, before the core kernel, after it), one
+// slot per call, so back-to-back calls are timed without a host wait between
 static volatile int g_profile = 0;
-struct ProfEvents {
-    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
-    bool valid = false;
-    ~ProfEvents()
+constexpr int kProfSlots = 64;
+struct ProfRing {
+    hipEvent_t e[kProfSlots][3] = {};
+    int cur = -1;              // slot of the call in progress / last call
+    uint64_t done = 0;         // calls whose three events were recorded
+    ~ProfRing()
     {
-        for (auto& x : e)
-            if (x) (void)hipEventDestroy(x);
+        for (auto& s : e)
+            for (auto& x : s)
+                if (x) (void)hipEventDestroy(x);
     }
 };
-static thread_local ProfEvents t_prof;
+static thread_local ProfRing t_prof;
 
 static void prof_mark(int i, hipStream_t st)
 {
     if (!g_profile) return;
-    if (!t_prof.e[i] && hipEventCreate(&t_prof.e[i]) != hipSuccess) { t_prof.e[i] = nullptr; return; }
-    (void)hipEventRecord(t_prof.e[i], st);
-    t_prof.valid = (i == 2) ? true : (i == 0 ? false : t_prof.valid);
+    if (i == 0) t_prof.cur = (t_prof.cur + 1) % kProfSlots;
+    if (t_prof.cur < 0) return;
+    hipEvent_t& ev = t_prof.e[t_prof.cur][i];
+    if (!ev && hipEventCreate(&ev) != hipSuccess) { ev = nullptr; return; }
+    (void)hipEventRecord(ev, st);
+    if (i == 2) t_prof.done++;
 }
 
 // Kp and Rp are multiples of 64: the grid is exactly (Kp / 64, Rp / 64, nb)
@@ -632,20 +640,50 @@ void gemm_set_default_variant(int mode, int vid)
 extern "C" int chai_hip_profile_enable(int on)
 {
     g_profile = on ? 1 : 0;
+    if (on) t_prof.done = 0;   // the calling thread's ring starts over
+    return CHAI_OK;
+}
+
+// durations of ring slot k (waits for its last event)
+static int prof_slot(int k, float* prep_ms, float* core_ms)
+{
+    hipEvent_t* e = t_prof.e[k];
+    if (!e[0] || !e[1] || !e[2]) { set_error("gemm timing: incomplete events"); return CHAI_ERR_STATE; }
+    CHAI_HIP_TRY(hipEventSynchronize(e[2]));
+    CHAI_HIP_TRY(hipEventElapsedTime(prep_ms, e[0], e[1]));
+    CHAI_HIP_TRY(hipEventElapsedTime(core_ms, e[1], e[2]));
     return CHAI_OK;
 }
 
 extern "C" int chai_hip_gemm_last_timing(float* prep_ms, float* core_ms)
 {
-    if (!t_prof.valid || !t_prof.e[0] || !t_prof.e[1] || !t_prof.e[2]) {
+    if (t_prof.done == 0 || t_prof.cur < 0) {
         set_error("gemm_last_timing: no profiled GEMM on this thread (enable profiling first)");
         return CHAI_ERR_STATE;
     }
-    CHAI_HIP_TRY(hipEventSynchronize(t_prof.e[2]));
     float a = 0.f, b = 0.f;
-    CHAI_HIP_TRY(hipEventElapsedTime(&a, t_prof.e[0], t_prof.e[1]));
-    CHAI_HIP_TRY(hipEventElapsedTime(&b, t_prof.e[1], t_prof.e[2]));
+    const int rc = prof_slot(t_prof.cur, &a, &b);
+    if (rc != CHAI_OK) return rc;
     if (prep_ms) *prep_ms = a;
     if (core_ms) *core_ms = b;
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_gemm_profile_mean(int count, float* prep_ms, float* core_ms)
+{
+    if (count < 1 || count > kProfSlots || (uint64_t)count > t_prof.done || t_prof.cur < 0) {
+        set_error("gemm_profile_mean: %d calls asked, %llu profiled on this thread (at most %d kept)", count,
+                  (unsigned long long)t_prof.done, kProfSlots);
+        return CHAI_ERR_ARG;
+    }
+    double sa = 0, sb = 0;
+    for (int i = 0; i < count; i++) {
+        float a = 0.f, b = 0.f;
+        const int rc = prof_slot((t_prof.cur - i + kProfSlots) % kProfSlots, &a, &b);
+        if (rc != CHAI_OK) return rc;
+        sa += a; sb += b;
+    }
+    if (prep_ms) *prep_ms = (float)(sa / count);
+    if (core_ms) *core_ms = (float)(sb / count);
     return CHAI_OK;
 }

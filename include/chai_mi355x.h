@@ -357,9 +357,13 @@ int chai_hip_gemv(int prec_a, int prec_x, int prec_y, int trans_a, size_t M, siz
  * autotune/AutoTuneBench.cpp:116-123): when enabled, chai_hip_gemm records
  * hipEvents on its launch stream around the operand pre-pass (split / pack)
  * and around the core GEMM kernel; chai_hip_gemm_last_timing waits for the
- * calling thread's most recent profiled call and returns both durations. */
+ * calling thread's most recent profiled call and returns both durations.
+ * chai_hip_gemm_profile_mean returns their means over the thread's last
+ * `count` profiled calls (at most 64 are kept; enabling starts over), so a
+ * run of back-to-back calls is timed without host waits in between. */
 int chai_hip_profile_enable(int on);
 int chai_hip_gemm_last_timing(float* prep_ms, float* core_ms);
+int chai_hip_gemm_profile_mean(int count, float* prep_ms, float* core_ms);
 
 #ifdef __cplusplus
 }
