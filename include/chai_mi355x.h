@@ -58,7 +58,9 @@ enum { CHAI_OP_DOT = 130, CHAI_OP_MEAN = 131, CHAI_OP_SUM = 132 };
 int chai_hip_init(int device_ordinal);
 int chai_hip_shutdown(void);
 const char* chai_hip_last_error(void);
-/* Device properties: CU count, LDS bytes per CU, clock (kHz), arch name. */
+/* Device properties: CU count, LDS bytes per CU, clock (kHz), arch name.
+ * Replaces OCLdevice::maxWorkGroupSize and the OpenCL device queries behind
+ * it (vendor/OCLdevice.hpp:544, vendor/OCLdevice.cpp:892-895). */
 int chai_hip_device_info(int* cu_count, int* lds_per_cu, int* clock_khz,
                          char* arch_name, size_t arch_len);
 
@@ -329,7 +331,9 @@ int chai_hip_trace_compile(const chai_trace* trace, void** handle);
 /* buffers[v] = device pointer of variable v (INPUT and OUTPUT required;
  * TEMP may be NULL, the library allocates one when it must be stored). */
 int chai_hip_trace_launch(void* handle, void* const* buffers, void* stream);
-/* number of kernels the trace compiled into, and of VM instructions */
+/* number of kernels the trace compiled into, and of VM instructions (the
+ * statement / kernel split the reference makes in WorkTrace::together,
+ * jitkrnl/WorkTrace.cpp:62-370) */
 int chai_hip_trace_info(void* handle, int* num_kernels, int* num_instructions);
 int chai_hip_trace_release(void* handle);
 /* Fused kernels run either on the pre-compiled trace VM kernel or as
@@ -337,9 +341,12 @@ int chai_hip_trace_release(void* handle);
  * time with hipRTC (default; env CHAI_TRACE_JIT=0 or set_jit(0) selects the
  * VM).  A failed JIT compile falls back to the VM. */
 int chai_hip_trace_set_jit(int on);
-/* Generated source (or the compile log when the JIT failed) of kernel k. */
+/* Generated source (or the compile log when the JIT failed) of kernel k;
+ * replaces the kernel text the reference prints to its journal
+ * (AutoTuneMemo::printSource, autotune/AutoTuneMemo.cpp:248). */
 int chai_hip_trace_kernel_source(void* handle, int kernel, char* buf, size_t len, int* jitted);
-/* Front end only (no device): the JIT source kernel k of a trace would get. */
+/* Front end only (no device): the JIT source kernel k of a trace would get
+ * (the same text as chai_hip_trace_kernel_source, produced without a GPU). */
 int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, char* buf, size_t len);
 
 /* ------------------------------------------------------------------- GEMV */
