@@ -54,7 +54,9 @@ def parse():
     p.add_argument("--test-shapes", type=int, default=500)
     p.add_argument("--save-model", default=None, help="autotune: write the trained model here")
     p.add_argument("--profile-run", action="store_true",
-                   help="sgemm: 3 GEMMs plus the calibration kernels, for rocprofv3")
+                   help="short program for rocprofv3: sgemm (3 GEMMs plus the calibration kernels) or, "
+                        "with --workload ragged, the ragged batch from two operand buffers")
+    p.add_argument("--ragged-variant", type=int, default=2, help="tile variant of the ragged profile run")
     p.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
                    help="summarise two rocprofv3 --pmc runs of --profile-run (no GPU needed)")
     p.add_argument("--pmc-table", nargs="+", metavar="DIR",
@@ -146,6 +148,33 @@ def cold_start(A, B, C):
     if "model_s" in out:
         out["speedup"] = round(out["exhaustive_s"] / out["model_s"], 1)
     return out
+
+
+def profile_run_ragged(args):
+    """Short program for rocprofv3 counter runs of the ragged batch: the 4096
+    instances as views of two large operand buffers (two fills instead of
+    8192), one plan, three runs."""
+    import torch
+    import chai_amd as chai
+    from chai_amd import ragged
+    from chai_amd.workloads import RAGGED_EPILOGUE, ragged_sizes
+    chai.init(0)
+    sizes = [int(k) for k in ragged_sizes()]
+    tot = sum(k * k for k in sizes)
+    bufA = torch.empty(tot, device="cuda"); bufB = torch.empty(tot, device="cuda")
+    bufC = torch.empty(tot, device="cuda")
+    chai.rng_fill(bufA, seed=42, limits=(-1.0, 1.0))
+    chai.rng_fill(bufB, seed=43, limits=(-1.0, 1.0))
+    items, off = [], 0
+    for k in sizes:
+        v = lambda b: b[off:off + k * k].view(k, k)   # noqa: E731
+        items.append(ragged.item(v(bufA), v(bufB), v(bufC)))
+        off += k * k
+    plan = ragged.Plan(items, epi=chai.epilogue(*RAGGED_EPILOGUE), variant=args.ragged_variant)
+    for _ in range(3):
+        plan.run()
+    torch.cuda.synchronize()
+    print("ragged profile run done", plan.info(), flush=True)
 
 
 def profile_run(args):
@@ -366,7 +395,8 @@ def run_ragged(args):
     ms = timed(lambda: plan.run(graph=True), args.steps, args.warmup)
     flops = info["useful_flops"]
     # per-instance launches of the same kernels (fused epilogue), as the reference runs them
-    ms_pi = timed(lambda: [chai.gemm(A, B, C, mode=chai.GEMM_BF16X3, epi=epi, variant=v) for A, B, C in batch],
+    dv = v if v < len(chai.gemm_variants()) else 2     # the f32-fed ragged tiles have no dense twin
+    ms_pi = timed(lambda: [chai.gemm(A, B, C, mode=chai.GEMM_BF16X3, epi=epi, variant=dv) for A, B, C in batch],
                   1, 1)
     line = base_line(args, round(flops / (ms * 1e-3) / 1e12, 2), "TFLOP/s", round(ms, 4),
                      "f32 (split-bf16x3 MFMA, f32 accumulate)",
@@ -379,6 +409,8 @@ def run_ragged(args):
                         "traffic": None, "kernel": "ragged_x3_kernel", "kernel_ms": round(core_ms, 4)}
     line["split_prepass_ms"] = round(split_ms, 4)
     line["tile_variant"] = v
+    line["tile_variant_kind"] = "f32-fed tiles (no split pre-pass)" if info["split_blocks"] == 0 else \
+        "split pre-pass + panel tiles"
     line["tune_s"] = round(tune_s, 2)
     line["plan"] = info
     line["per_instance_ms"] = round(ms_pi, 3)
@@ -581,7 +613,9 @@ def main():
               "(the GEMM would shard by 2-D output tiles across GPUs, all-gathering B panels over xGMI)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
-    if args.profile_run:
+    if args.profile_run and args.workload == "ragged":
+        profile_run_ragged(args)
+    elif args.profile_run:
         profile_run(args)
     elif args.workload == "sgemm":
         run_sgemm(args)

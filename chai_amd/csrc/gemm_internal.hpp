@@ -25,6 +25,7 @@ struct GemmArgs {
     float* C;
     double* Cd;
     uint64_t Kp, Np, Mp;
+    uint64_t lda, ldb, K;       // f32 A / B row strides and depth, for the tile fed straight from them
     uint64_t pa_bstride, pb_bstride, c_bstride;   // elements between batch instances
     uint64_t ldc, M, N;
     uint32_t tiles_m, tiles_n;

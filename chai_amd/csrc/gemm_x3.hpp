@@ -85,6 +85,89 @@ __device__ __forceinline__ void x3_issue_stage(const uint16_t* pAh, const uint16
     }
 }
 
+// Fragment read offsets (u32x4 units within a stage) of this lane's A and B
+// tile rows for each 16-deep k block.
+template <typename S>
+__device__ __forceinline__ void x3_frag_offsets(int wm, int wn, int lane, int (&offA)[S::TI][S::KB],
+                                                int (&offB)[S::TJ][S::KB])
+{
+#pragma unroll
+    for (int kb = 0; kb < S::KB; kb++) {
+        const int q = kb * 2 + (lane >> 5);
+#pragma unroll
+        for (int i = 0; i < S::TI; i++) {
+            const int row = wm + i * 32 + (lane & 31);
+            offA[i][kb] = row * S::CPR + (q ^ ((row >> S::SH) & (S::CPR - 1)));
+        }
+#pragma unroll
+        for (int j = 0; j < S::TJ; j++) {
+            const int row = wn + j * 32 + (lane & 31);
+            offB[j][kb] = 2 * S::PA + row * S::CPR + (q ^ ((row >> S::SH) & (S::CPR - 1)));
+        }
+    }
+}
+
+// The MFMAs of one LDS stage (at chunk offset cur): per 16-deep k block the
+// three products ah*bl, al*bh, ah*bh, minus those with a zero panel (SKIP).
+template <typename S, int SKIP>
+__device__ __forceinline__ void x3_stage_mfma(const u32x4* lds, int cur, const int (&offA)[S::TI][S::KB],
+                                              const int (&offB)[S::TJ][S::KB], floatx16 (&acc)[S::TI][S::TJ])
+{
+#pragma unroll
+    for (int kb = 0; kb < S::KB; kb++) {
+        bf16x8 ah[S::TI], al[S::TI], bh[S::TJ], bl[S::TJ];
+#pragma unroll
+        for (int i = 0; i < S::TI; i++) {
+            ah[i] = as_bf16x8(lds[cur + offA[i][kb]]);
+            if (!(SKIP & 1)) al[i] = as_bf16x8(lds[cur + S::PA + offA[i][kb]]);
+        }
+#pragma unroll
+        for (int j = 0; j < S::TJ; j++) {
+            bh[j] = as_bf16x8(lds[cur + offB[j][kb]]);
+            if (!(SKIP & 2)) bl[j] = as_bf16x8(lds[cur + S::PB + offB[j][kb]]);
+        }
+        // three passes over the tiles: consecutive MFMAs write different accumulators
+        if (!(SKIP & 2)) {
+#pragma unroll
+            for (int i = 0; i < S::TI; i++)
+#pragma unroll
+                for (int j = 0; j < S::TJ; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        }
+        if (!(SKIP & 1)) {
+#pragma unroll
+            for (int i = 0; i < S::TI; i++)
+#pragma unroll
+                for (int j = 0; j < S::TJ; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < S::TI; i++)
+#pragma unroll
+            for (int j = 0; j < S::TJ; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+    }
+}
+
+// Store the wave's accumulators (32x32x16 output layout: lane -> column
+// lane % 32; register r -> row 8 (r / 4) + 4 (lane / 32) + r % 4).
+template <typename S>
+__device__ __forceinline__ void x3_store(const GemmArgs& a, float* C, uint64_t row_base, uint64_t col_base, int lane,
+                                         const floatx16 (&acc)[S::TI][S::TJ])
+{
+#pragma unroll
+    for (int i = 0; i < S::TI; i++)
+#pragma unroll
+        for (int j = 0; j < S::TJ; j++) {
+            const uint64_t col = col_base + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint64_t row = row_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
+            }
+        }
+}
+
 // The tile body.  SKIP bit 0: the tile's Al panel is zero, bit 1: its Bl
 // panel is zero; the products with a zero panel (exact zeros) are neither
 // loaded nor multiplied.
@@ -137,22 +220,8 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
     wait_lds_dma();
     __syncthreads();
 
-    // fragment read offsets (u32x4 units) for each A / B tile row of this lane
     int offA[S::TI][S::KB], offB[S::TJ][S::KB];
-#pragma unroll
-    for (int kb = 0; kb < S::KB; kb++) {
-        const int q = kb * 2 + (lane >> 5);
-#pragma unroll
-        for (int i = 0; i < S::TI; i++) {
-            const int row = wm + i * 32 + (lane & 31);
-            offA[i][kb] = row * S::CPR + (q ^ ((row >> S::SH) & (S::CPR - 1)));
-        }
-#pragma unroll
-        for (int j = 0; j < S::TJ; j++) {
-            const int row = wn + j * 32 + (lane & 31);
-            offB[j][kb] = 2 * S::PA + row * S::CPR + (q ^ ((row >> S::SH) & (S::CPR - 1)));
-        }
-    }
+    x3_frag_offsets<S>(wm, wn, lane, offA, offB);
 
     for (int kt = 0; kt < nk; kt++) {
         const int cur = (kt & 1) * S::STAGE;
@@ -162,58 +231,12 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
             x3_issue_stage<S, SKIP>(a.Ah + oa, a.Al + oa, a.Bh + ob, a.Bl + ob, srcA, srcB, hiA, hiB,
                                     a.a_kblk, a.b_kblk, lds + ((kt + 1) & 1) * S::STAGE, wave);
         }
-#pragma unroll
-        for (int kb = 0; kb < S::KB; kb++) {
-            bf16x8 ah[S::TI], al[S::TI], bh[S::TJ], bl[S::TJ];
-#pragma unroll
-            for (int i = 0; i < S::TI; i++) {
-                ah[i] = as_bf16x8(lds[cur + offA[i][kb]]);
-                if (!(SKIP & 1)) al[i] = as_bf16x8(lds[cur + S::PA + offA[i][kb]]);
-            }
-#pragma unroll
-            for (int j = 0; j < S::TJ; j++) {
-                bh[j] = as_bf16x8(lds[cur + offB[j][kb]]);
-                if (!(SKIP & 2)) bl[j] = as_bf16x8(lds[cur + S::PB + offB[j][kb]]);
-            }
-            // three passes over the tiles: consecutive MFMAs write different accumulators
-            if (!(SKIP & 2)) {
-#pragma unroll
-                for (int i = 0; i < S::TI; i++)
-#pragma unroll
-                    for (int j = 0; j < S::TJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-            }
-            if (!(SKIP & 1)) {
-#pragma unroll
-                for (int i = 0; i < S::TI; i++)
-#pragma unroll
-                    for (int j = 0; j < S::TJ; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-            }
-#pragma unroll
-            for (int i = 0; i < S::TI; i++)
-#pragma unroll
-                for (int j = 0; j < S::TJ; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
-        }
+        x3_stage_mfma<S, SKIP>(lds, cur, offA, offB, acc);
         wait_lds_dma();     // stage kt + 1 has landed (this wave's share)
         __syncthreads();    // ... every wave's share, and stage kt is no longer read
     }
 
-    // 32x32x16 output layout: lane -> column lane%32; register r -> row 8*(r/4) + 4*(lane/32) + r%4
-    const uint64_t row_base = (uint64_t)tm * BM + wm;
-    const uint64_t col_base = (uint64_t)tn * BN + wn;
-#pragma unroll
-    for (int i = 0; i < S::TI; i++)
-#pragma unroll
-        for (int j = 0; j < S::TJ; j++) {
-            const uint64_t col = col_base + j * 32 + (lane & 31);
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const uint64_t row = row_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
-            }
-        }
+    x3_store<S>(a, C, (uint64_t)tm * BM + wm, (uint64_t)tn * BN + wn, lane, acc);
 }
 
 // One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is

@@ -23,6 +23,7 @@
 #include "gemm_internal.hpp"
 #include "gemm_split.hpp"
 #include "gemm_x3.hpp"
+#include "gemm_x3_f32.hpp"
 
 #include <algorithm>
 #include <chrono>
@@ -117,18 +118,62 @@ __global__ __launch_bounds__(WGM * WGN * 64, MINB) void ragged_x3_kernel(
     }
 }
 
+// The f32-fed variants: tiles read A and B directly (gemm_x3_f32.hpp), so the
+// plan needs no split pre-pass and no panels; a one-block kernel clears the
+// queue counters instead.
+template <int BM, int BN, int WGM, int WGN, int MINB>
+__global__ __launch_bounds__(WGM * WGN * 64, MINB) void ragged_x3f_kernel(
+    const RaggedInst* __restrict__ inst, const uint2* __restrict__ tiles,
+    const uint32_t* __restrict__ qmeta, uint32_t* counters, RaggedEpi epi)
+{
+    extern __shared__ u32x4 lds[];
+    __shared__ uint32_t s_next;
+    const uint32_t home = blockIdx.x % kQueues;
+    for (uint32_t qi = 0; qi < kQueues; qi++) {
+        const uint32_t q = __builtin_amdgcn_readfirstlane((home + qi) % kQueues);
+        const uint32_t qlen = qmeta[q], qoff = qmeta[kQueues + q];
+        for (;;) {
+            if (threadIdx.x == 0) s_next = atomicAdd(&counters[q], 1u);
+            __syncthreads();
+            const uint32_t t = __builtin_amdgcn_readfirstlane(s_next);
+            __syncthreads();   // every thread has read s_next before thread 0 may rewrite it
+            if (t >= qlen) break;
+            const uint2 w = tiles[qoff + t];
+            const RaggedInst& I = inst[w.x];
+            GemmArgs g{};
+            g.Af = I.A; g.Bf = I.B; g.lda = I.lda; g.ldb = I.ldb;
+            g.K = I.K; g.Kp = I.Kp;
+            g.C = I.C; g.ldc = I.ldc;
+            g.M = I.M; g.N = I.N;
+            g.epi = epi.on; g.epi_scale = epi.scale; g.epi_shift = epi.shift; g.epi_lo = epi.lo;
+            x3f_tile<BM, BN, WGM, WGN>(g, w.y >> 16, w.y & 0xffffu, lds);
+        }
+    }
+}
+
+__global__ void ragged_clear_counters(uint32_t* counters)
+{
+    if (threadIdx.x < kQueues) counters[threadIdx.x] = 0u;
+}
+
 struct RaggedVariant {
     int BM, BN, BK;
     size_t lds;
     int threads;
     void (*kernel)(const RaggedInst*, const uint2*, const uint32_t*, uint32_t*, RaggedEpi);
+    bool fused;    // f32-fed tiles: no split pre-pass
 };
 
 #define CHAI_RAGGEDV(BM, BN, BK, WGM, WGN, MINB)                                              \
     {BM, BN, BK, X3Shape<BM, BN, BK, WGM, WGN>::LDS_BYTES, WGM * WGN * 64,                    \
-     ragged_x3_kernel<BM, BN, BK, WGM, WGN, MINB>},
+     ragged_x3_kernel<BM, BN, BK, WGM, WGN, MINB>, false},
+#define CHAI_RAGGEDF(BM, BN, WGM, WGN, MINB)                                                  \
+    {BM, BN, 32, X3Shape<BM, BN, 32, WGM, WGN>::LDS_BYTES, WGM * WGN * 64,                    \
+     ragged_x3f_kernel<BM, BN, WGM, WGN, MINB>, true},
 
-static const RaggedVariant kRaggedVariants[] = {CHAI_X3_VARIANTS(CHAI_RAGGEDV)};
+// variants 0..7: the panel tiles of the dense path (split pre-pass first);
+// 8..9: the f32-fed tiles
+static const RaggedVariant kRaggedVariants[] = {CHAI_X3_VARIANTS(CHAI_RAGGEDV) CHAI_X3F_VARIANTS(CHAI_RAGGEDF)};
 static constexpr int kNumRagged = sizeof(kRaggedVariants) / sizeof(kRaggedVariants[0]);
 static int g_ragged_tuned = -1;
 static std::mutex g_ragged_mu;
@@ -245,8 +290,9 @@ static int ragged_build(const chai_ragged_item* items, size_t count, const chai_
     const size_t o_tiles = take(ntiles * sizeof(uint2), 256);
     const size_t o_qmeta = take(2 * kQueues * sizeof(uint32_t), 256);
     const size_t o_cnt = take(kQueues * sizeof(uint32_t), 256);
-    std::vector<size_t> o_panel(n);
-    for (size_t j = 0; j < n; j++) o_panel[j] = take(2 * sizeof(uint16_t) * (sh[j].Mp + sh[j].Np) * sh[j].Kp, 256);
+    std::vector<size_t> o_panel(n, 0);
+    if (!v.fused)
+        for (size_t j = 0; j < n; j++) o_panel[j] = take(2 * sizeof(uint16_t) * (sh[j].Mp + sh[j].Np) * sh[j].Kp, 256);
     P->mem_bytes = off;
 
     // host images
@@ -270,7 +316,7 @@ static int ragged_build(const chai_ragged_item* items, size_t count, const chai_
         P->padded_flops += 2ull * s.tm * s.tn * v.BM * v.BN * s.Kp;
     }
     hfirst[n] = acc;
-    P->split_blocks = acc;
+    P->split_blocks = v.fused ? 0 : acc;
 
     // greedy LPT assignment of instances to the per-XCD queues
     std::vector<std::vector<uint2>> q(kQueues);
@@ -297,6 +343,7 @@ static int ragged_build(const chai_ragged_item* items, size_t count, const chai_
     if (e != hipSuccess) { P->mem = nullptr; delete P; return hip_fail(e, "ragged_plan: workspace"); }
     char* base = (char*)P->mem;
     for (size_t j = 0; j < n; j++) {
+        if (v.fused) continue;   // no panels
         uint16_t* p = (uint16_t*)(base + o_panel[j]);
         const uint64_t a = sh[j].Mp * sh[j].Kp, b = sh[j].Np * sh[j].Kp;
         hinst[j].Ah = p; hinst[j].Al = p + a; hinst[j].Bh = p + 2 * a; hinst[j].Bl = p + 2 * a + b;
@@ -336,8 +383,13 @@ static int ragged_launch(RaggedPlan* P, hipStream_t st, bool timed)
     if (P->count == 0) return CHAI_OK;
     const RaggedVariant& v = kRaggedVariants[P->variant];
     if (timed) CHAI_HIP_TRY(hipEventRecord(P->ev[0], st));
-    ragged_split_kernel<<<P->split_blocks, 256, 0, st>>>(P->d_inst, P->d_first, P->count, P->d_counters);
-    CHAI_LAUNCH_CHECK("ragged_split_kernel");
+    if (v.fused) {
+        ragged_clear_counters<<<1, 64, 0, st>>>(P->d_counters);
+        CHAI_LAUNCH_CHECK("ragged_clear_counters");
+    } else {
+        ragged_split_kernel<<<P->split_blocks, 256, 0, st>>>(P->d_inst, P->d_first, P->count, P->d_counters);
+        CHAI_LAUNCH_CHECK("ragged_split_kernel");
+    }
     if (timed) CHAI_HIP_TRY(hipEventRecord(P->ev[1], st));
     hipLaunchKernelGGL(v.kernel, dim3(P->grid), dim3(v.threads), v.lds, st, (const RaggedInst*)P->d_inst,
                        (const uint2*)P->d_tiles, (const uint32_t*)P->d_qmeta, P->d_counters, P->epi);

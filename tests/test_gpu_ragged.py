@@ -43,9 +43,13 @@ SHAPES = [(1, 1, 1), (17, 33, 65), (300, 129, 1000), (64, 64, 64), (1024, 96, 48
           (256, 256, 256), (129, 257, 513), (16, 1024, 16), (500, 500, 500)]
 
 
-@pytest.mark.parametrize("variant", [0, 2, 5, 7])
+@pytest.mark.parametrize("variant", [0, 2, 5, 7, 8, 9])
 @pytest.mark.parametrize("with_epi", [False, True])
 def test_ragged_matches_dense_bit_exact(chai, oracle, variant, with_epi):
+    """Variants 0..7 run the dense path's panel tiles after a split pre-pass;
+    8 and 9 read the f32 operands directly and split inside the tile.  Every
+    tile shape accumulates each output in the same k order with the same
+    MFMAs, so all of them equal the dense GEMM bit for bit."""
     from chai_amd import ragged
     print("case ragged variant", variant, "epi", with_epi, flush=True)
     batch = make_batch(oracle, SHAPES)
@@ -53,16 +57,44 @@ def test_ragged_matches_dense_bit_exact(chai, oracle, variant, with_epi):
     plan = ragged.Plan([ragged.item(A, B, C) for A, B, C in batch], epi=epi, variant=variant)
     info = plan.info()
     assert info["variant"] == variant and info["tiles"] > 0
+    assert (info["split_blocks"] == 0) == (variant >= 8)
     plan.run()
     torch.cuda.synchronize()
+    dv = variant if variant < 8 else 2
     for (A, B, C), (M, N, K) in zip(batch, SHAPES):
-        D = chai.gemm(A, B, mode=chai.GEMM_BF16X3, variant=variant)
+        D = chai.gemm(A, B, mode=chai.GEMM_BF16X3, variant=dv)
         if with_epi:
             assert torch.equal(C, epi_ref(D)), (M, N, K)
-            E = chai.gemm(A, B, mode=chai.GEMM_BF16X3, variant=variant, epi=epi)
+            E = chai.gemm(A, B, mode=chai.GEMM_BF16X3, variant=dv, epi=epi)
             assert torch.equal(C, E), (M, N, K)
         else:
             assert torch.equal(C, D), (M, N, K)
+    plan.release()
+
+
+@pytest.mark.parametrize("variant", [8, 9])
+def test_ragged_f32_fed_strided_and_graph(chai, oracle, variant):
+    """The f32-fed tiles on sub-matrix views (unaligned rows, odd strides) and
+    through graph replays, against the dense GEMM of contiguous copies."""
+    from chai_amd import ragged
+    big = torch.from_numpy(unif(oracle, 77, 700, 900)).cuda()
+    views = [(big[3:300, 5:262], big[1:258, 400:517]),      # odd offsets: scalar loads
+             (big[100:164, 0:64], big[200:264, 512:704]),   # aligned rows
+             (big[10:11, 7:8], big[20:21, 30:31])]          # 1 x 1 x 1
+    items, outs = [], []
+    for A, B in views:
+        C = torch.full((A.shape[0], B.shape[1]), float("nan"), device="cuda")
+        outs.append(C)
+        items.append(ragged.item(A, B, C))
+    plan = ragged.Plan(items, variant=variant)
+    for graph in (False, True, True):
+        for C in outs:
+            C.fill_(float("nan"))
+        torch.cuda.synchronize()
+        plan.run(graph=graph)
+        torch.cuda.synchronize()
+        for (A, B), C in zip(views, outs):
+            assert torch.equal(C, chai.gemm(A.contiguous(), B.contiguous(), mode=chai.GEMM_BF16X3)), (A.shape, graph)
     plan.release()
 
 
