@@ -54,6 +54,21 @@ __device__ __forceinline__ void split_bf16x2(float a0, float a1, uint32_t& hi, u
 typedef __attribute__((address_space(1))) void global_void_t;
 typedef __attribute__((address_space(3))) void lds_void_t;
 
+// Global-memory accesses through pointers the compiler cannot place (read
+// from descriptors in memory): without the address space they become flat
+// instructions, which also count against lgkmcnt and so make LDS waits wait
+// for them.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v)
+{
+    *(__attribute__((address_space(1))) T*)p = v;
+}
+
 // 16 bytes per lane from global memory into LDS (global_load_lds_dwordx4):
 // lane i of the wave writes bytes [16 i, 16 i + 16) after dst (dst is
 // wave-uniform).  Completion is counted by vmcnt.

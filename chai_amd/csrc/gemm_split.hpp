@@ -23,10 +23,10 @@ template <typename T>
 __device__ __forceinline__ f32x4 load4_f32(const T* p)
 {
     if constexpr (sizeof(T) == 4 && (T)0.5 != 0) {
-        return *reinterpret_cast<const f32x4*>(p);
+        return gload(reinterpret_cast<const f32x4*>(p));
     } else {
         typedef T t4 __attribute__((ext_vector_type(4)));
-        const t4 v = *reinterpret_cast<const t4*>(p);
+        const t4 v = gload(reinterpret_cast<const t4*>(p));
         return f32x4{(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
     }
 }
@@ -47,11 +47,11 @@ __device__ __forceinline__ uint32_t split_store16(const float* v, uint16_t* hi, 
     }
     u32x4* dh = reinterpret_cast<u32x4*>(hi + off);
     u32x4* dl = reinterpret_cast<u32x4*>(lo + off);
-    dh[0] = vh0;
-    dl[0] = vl0;
+    gstore(dh, vh0);
+    gstore(dl, vl0);
     uint32_t lo_or = vl0[0] | vl0[1] | vl0[2] | vl0[3];
     if (second) {
-        dh[1] = vh1; dl[1] = vl1;
+        gstore(dh + 1, vh1); gstore(dl + 1, vl1);
         lo_or |= vl1[0] | vl1[1] | vl1[2] | vl1[3];
     }
     return lo_or;
@@ -88,7 +88,7 @@ __device__ __forceinline__ uint32_t split_tile(const T* __restrict__ src, uint64
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 16; j++) v[j] = (r < R && kb + j < K) ? to_f32(src[r * ld + kb + j]) : 0.0f;
+            for (int j = 0; j < 16; j++) v[j] = (r < R && kb + j < K) ? to_f32(gload(src + r * ld + kb + j)) : 0.0f;
         }
         return split_store16(v, hi, lo, panel_off(r, kb, Rp), kb + 8 < Kp);
     }
@@ -108,7 +108,7 @@ __device__ __forceinline__ uint32_t split_tile(const T* __restrict__ src, uint64
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const uint64_t rj = r0 + r4 + j;
-                    tile[kk][r4 + j] = (rj < R && k < K) ? to_f32(src[k * ld + rj]) : 0.0f;
+                    tile[kk][r4 + j] = (rj < R && k < K) ? to_f32(gload(src + k * ld + rj)) : 0.0f;
                 }
             }
         }

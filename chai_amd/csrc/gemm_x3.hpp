@@ -34,9 +34,9 @@ __device__ __forceinline__ bf16x8 as_bf16x8(u32x4 v)
 
 __device__ __forceinline__ void store_c(float* C, uint64_t idx, float v, const GemmArgs& a)
 {
-    if (a.general) v = (float)(a.alpha * (double)v + a.beta * (double)C[idx]);
+    if (a.general) v = (float)(a.alpha * (double)v + a.beta * (double)gload(C + idx));
     if (a.epi) v = fmaxf(v * a.epi_scale + a.epi_shift, a.epi_lo);   // no contraction (-ffp-contract=off)
-    C[idx] = v;
+    gstore(C + idx, v);
 }
 
 template <int BM, int BN, int BK, int WGM, int WGN>

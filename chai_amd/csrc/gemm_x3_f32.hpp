@@ -62,12 +62,12 @@ __device__ __forceinline__ void x3f_load(const GemmArgs& a, uint64_t r0, uint64_
         if (avec && row < a.M && kh + 16 <= a.K) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
-                const f32x4 x = reinterpret_cast<const f32x4*>(src)[q];
+                const f32x4 x = gload(reinterpret_cast<const f32x4*>(src) + q);
                 fa[p][4 * q] = x[0]; fa[p][4 * q + 1] = x[1]; fa[p][4 * q + 2] = x[2]; fa[p][4 * q + 3] = x[3];
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 16; j++) fa[p][j] = (row < a.M && kh + j < a.K) ? src[j] : 0.0f;
+            for (int j = 0; j < 16; j++) fa[p][j] = (row < a.M && kh + j < a.K) ? gload(src + j) : 0.0f;
         }
     }
     const uint64_t kb = k0 + (uint64_t)(t / F::RP) * 16;       // B: 16 k of column t % RP
@@ -76,7 +76,7 @@ __device__ __forceinline__ void x3f_load(const GemmArgs& a, uint64_t r0, uint64_
         const uint64_t col = c0 + (uint64_t)(t % F::RP) + (uint64_t)p * F::RP;
         const float* src = a.Bf + kb * a.ldb + col;
 #pragma unroll
-        for (int j = 0; j < 16; j++) fb[p][j] = (col < a.N && kb + j < a.K) ? src[(uint64_t)j * a.ldb] : 0.0f;
+        for (int j = 0; j < 16; j++) fb[p][j] = (col < a.N && kb + j < a.K) ? gload(src + (uint64_t)j * a.ldb) : 0.0f;
     }
 }
 
