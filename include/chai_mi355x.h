@@ -165,8 +165,11 @@ int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A, const void
 /* The replacement for gathering identical traces from many host threads into
  * one VectorTrace (runtime/Scheduler.cpp:48-307, common/VectorTrace.cpp:9-100,
  * enqueue/EnqueueMatmulAuto.cpp:57-68 setBatching): instances of
- * C = A B + epilogue with DIFFERENT shapes run as one split pre-pass launch and
- * one persistent split-BF16x3 launch.  The persistent kernel pulls output
+ * C = A B + epilogue with DIFFERENT shapes run as one persistent
+ * split-BF16x3 launch: tile variants 0..7 (the dense path's shapes) after one
+ * split pre-pass launch, variants 8..9 reading the f32 operands directly with
+ * no pre-pass (the split happens inside each tile).  Results are identical
+ * for every variant.  The persistent kernel pulls output
  * tiles from eight per-XCD queues, each filled largest-instance-first
  * (LPT by M N K) and balanced by work; a block drains its own XCD's queue, then
  * steals from the others.  f32 operands, row-major, no transposes. */
@@ -178,19 +181,22 @@ typedef struct chai_ragged_item {
     size_t lda, ldb, ldc;
 } chai_ragged_item;
 
-/* Build a plan: validates the items, allocates the split workspace and the
- * tile queues (the plan owns them), picks the tile variant (-1 = the one the
- * ragged tuner chose last, else variant 0). */
+/* Build a plan: validates the items, allocates the tile queues and, for
+ * variants 0..7, the split workspace (the plan owns them), picks the tile
+ * variant (-1 = the one the ragged tuner chose last, else variant 0). */
 int chai_hip_ragged_plan(const chai_ragged_item* items, size_t count, const chai_epilogue* epi,
                          int variant, void** plan);
-/* Run the plan on `stream` (asynchronous).  use_graph = 1 captures the two
- * launches into a hipGraph on the first call and replays it afterwards. */
+/* Run the plan on `stream` (asynchronous).  use_graph = 1 captures the
+ * launches (pre-pass or counter reset, then the persistent kernel) into a
+ * hipGraph on the first call and replays it afterwards. */
 int chai_hip_ragged_run(void* plan, int use_graph, void* stream);
 /* info[0..6]: output tiles, persistent blocks, workspace bytes, useful flops
- * (sum 2MNK), padded flops (tiles x BM x BN x 2Kp), variant, split blocks. */
+ * (sum 2MNK), padded flops (tiles x BM x BN x 2Kp), variant, split blocks
+ * (0 for the f32-fed variants). */
 int chai_hip_ragged_info(void* plan, uint64_t* info);
-/* hipEvent durations (ms) of the last run's split pre-pass and persistent
- * kernel (not available for graph replays: returns CHAI_ERR_STATE). */
+/* hipEvent durations (ms) of the last run's split pre-pass (or counter
+ * reset) and persistent kernel (not available for graph replays: returns
+ * CHAI_ERR_STATE). */
 int chai_hip_ragged_last_timing(void* plan, float* split_ms, float* core_ms);
 int chai_hip_ragged_release(void* plan);
 /* Time every tile variant on this batch (one warm-up and `trials` runs each)
