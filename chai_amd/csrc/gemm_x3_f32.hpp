@@ -54,16 +54,25 @@ __device__ __forceinline__ void x3f_load(const GemmArgs& a, uint64_t r0, uint64_
 {
     using F = X3F32Shape<BM, BN, WGM, WGN>;
     const int t = threadIdx.x;
+    // A stage whose 32 k all lie below K (every stage but the last of a
+    // K % 32 != 0 operand) needs one row / column test per lane instead of
+    // one per element; kfull is block-uniform.
+    const bool kfull = k0 + 32 <= a.K;
     const uint64_t kh = k0 + (uint64_t)(t & 1) * 16;          // A: 16 k of row t / 2
 #pragma unroll
     for (int p = 0; p < F::NPA; p++) {
         const uint64_t row = r0 + (uint64_t)(t >> 1) + (uint64_t)p * F::RP;
         const float* src = a.Af + row * a.lda + kh;
-        if (avec && row < a.M && kh + 16 <= a.K) {
+        if (kfull && avec) {
+            if (row < a.M) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const f32x4 x = gload(reinterpret_cast<const f32x4*>(src) + q);
-                fa[p][4 * q] = x[0]; fa[p][4 * q + 1] = x[1]; fa[p][4 * q + 2] = x[2]; fa[p][4 * q + 3] = x[3];
+                for (int q = 0; q < 4; q++) {
+                    const f32x4 x = gload(reinterpret_cast<const f32x4*>(src) + q);
+                    fa[p][4 * q] = x[0]; fa[p][4 * q + 1] = x[1]; fa[p][4 * q + 2] = x[2]; fa[p][4 * q + 3] = x[3];
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; j++) fa[p][j] = 0.0f;
             }
         } else {
 #pragma unroll
@@ -75,8 +84,19 @@ __device__ __forceinline__ void x3f_load(const GemmArgs& a, uint64_t r0, uint64_
     for (int p = 0; p < F::NPB; p++) {
         const uint64_t col = c0 + (uint64_t)(t % F::RP) + (uint64_t)p * F::RP;
         const float* src = a.Bf + kb * a.ldb + col;
+        if (kfull) {
+            if (col < a.N) {
 #pragma unroll
-        for (int j = 0; j < 16; j++) fb[p][j] = (col < a.N && kb + j < a.K) ? gload(src + (uint64_t)j * a.ldb) : 0.0f;
+                for (int j = 0; j < 16; j++) fb[p][j] = gload(src + (uint64_t)j * a.ldb);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 16; j++) fb[p][j] = 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < 16; j++)
+                fb[p][j] = (col < a.N && kb + j < a.K) ? gload(src + (uint64_t)j * a.ldb) : 0.0f;
+        }
     }
 }
 
