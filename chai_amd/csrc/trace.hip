@@ -11,8 +11,10 @@
 // elementwise statements over one index space fuse; a reduction is fused into
 // the kernel that computes its operand (the reference instead emits a separate
 // serial reduction kernel); a statement reading a reduction result of the
-// current group starts a new group.  Within a group, TEMP variables live in
-// VM registers; a variable crossing groups is stored.
+// current group starts a new group, and so does a statement gathering
+// (gather1_floor / gather2_floor) from a variable the current group computes.
+// Within a group, TEMP variables live in VM registers; a variable crossing
+// groups, or gathered from, is stored.
 #include "common.hpp"
 #include "gemm_internal.hpp"
 #include "trace_vm.hpp"
@@ -116,6 +118,10 @@ __global__ __launch_bounds__(VM_THREADS) void trace_vm_kernel(VmLaunchArgs L)
                     c = v;
                     for (int i = 0; i < 4; i++) v.l[i] = as_double(I.p2, a.l[i]) != 0.0 ? b.l[i] : c.l[i];
                     break;
+                case VM_GATHER:
+                    switch (I.s2) { VM_SLOTS(VM_RD_CASE) }
+                    v = vm_gather(var_at(I.a), chain_at(I.p2), slot, y, I.e == 2, I.b, a, I.c, v, nval);
+                    break;
                 case VM_STORE:
                     vm_store(var_at(I.a), a, slot, y, x0, nval, L.W, L.H);
                     wr = false;
@@ -183,6 +189,7 @@ bool is_rng(uint32_t c) { return c >= 110 && c <= 115; }
 bool is_index(uint32_t c) { return c >= 42 && c <= 49; }
 bool is_red(uint32_t c) { return c == 130 || c == 131 || c == 132; }
 bool is_mm(uint32_t c) { return c == 120 || c == 121; }
+bool is_gather(uint32_t c) { return c == 40 || c == 41; }
 
 int prec_of_suffix(int k) { return k == 0 ? CHAI_U32 : k == 1 ? CHAI_I32 : k == 2 ? CHAI_F32 : CHAI_F64; }
 
@@ -338,6 +345,8 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
             arity = 1;
         } else if (n.code == 130) {
             arity = 2;
+        } else if (is_gather(n.code)) {
+            arity = n.code == 40 ? 2 : 3;
         } else if (n.code == 120) {
             arity = 2;
         } else if (n.code == 121) {
@@ -349,7 +358,16 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
         e.kids.assign(stack.end() - arity, stack.end());
         stack.resize(stack.size() - arity);
         // typing
-        if (arity == 1 && !is_convert(n.code)) {
+        if (is_gather(n.code)) {
+            // shape and type of the data array, which must be a variable in
+            // memory (the reference's JIT gathers from a data variable too,
+            // jitstmt/MakeStmt.cpp:656-673); subscripts keep their own types
+            const ENode& a = s.nodes[e.kids[0]];
+            if (a.code != CHAI_TRACE_VAR) return fail("gather data must be a variable");
+            if (a.src->flags & 1) return fail("gather from a transposed variable is not supported");
+            e.prec = a.prec;
+            e.d = a.d;
+        } else if (arity == 1 && !is_convert(n.code)) {
             const ENode& k = s.nodes[e.kids[0]];
             e.prec = k.prec;
             e.d = k.d;
@@ -574,6 +592,32 @@ struct Emitter {
             G.prog.push_back(I);
             return tgt;
         }
+        if (is_gather(e.code)) {
+            // subscripts are evaluated at the gather's own coordinate (its dims
+            // join the broadcast path) in their own types; the data variable
+            // is addressed, never loaded as a value
+            const int chain_g = chain_for(e.d, false);
+            if (chain_g < 0) { err = fail("broadcast nesting too deep for the VM"); return 0; }
+            path.push_back(e.d);
+            const int sx = emit(s, e.kids[1], tgt, err);
+            if (err) return 0;
+            int sy = sx;
+            if (e.code == 41) {
+                sy = emit(s, e.kids[2], sx == tgt ? tgt + 1 : tgt, err);
+                if (err) return 0;
+            }
+            path.pop_back();
+            VmIns I = ins(VM_GATHER, e.prec, tgt);
+            I.p2 = (uint8_t)chain_g;
+            I.a = var_index(s.nodes[e.kids[0]].src->arg);
+            I.s1 = (uint8_t)sx;
+            I.s2 = (uint8_t)sy;
+            I.b = (uint32_t)s.nodes[e.kids[1]].prec;
+            I.c = e.code == 41 ? (uint32_t)s.nodes[e.kids[2]].prec : 0u;
+            I.e = e.code == 41 ? 2u : 1u;
+            G.prog.push_back(I);
+            return tgt;
+        }
         // operators: evaluate the operands into consecutive slots, converting to
         // the operation's precision where it differs
         int want[3] = {e.prec, e.prec, e.prec};
@@ -642,12 +686,19 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
     // group statements
     std::vector<int> group_of_stmt(t.num_stmts, -1);
     std::vector<int> red_group_of_var(t.num_vars, -1);
+    std::vector<int> group_of_var(t.num_vars, -1);
     for (uint32_t i = 0; i < t.num_stmts; i++) {
         const Stmt& s = P.stmts[i];
         bool reads_red = false;
         for (const auto& e : s.nodes)
             if (e.code == CHAI_TRACE_VAR && !P.groups.empty() &&
                 red_group_of_var[e.src->arg] == (int)P.groups.size() - 1)
+                reads_red = true;
+        // a gather reads its data at arbitrary subscripts: data computed in the
+        // current kernel must be complete first, so the statement starts a new one
+        for (const auto& e : s.nodes)
+            if (is_gather(e.code) && !P.groups.empty() &&
+                group_of_var[s.nodes[e.kids[0]].src->arg] == (int)P.groups.size() - 1)
                 reads_red = true;
         const bool join = s.kind != 2 && !P.groups.empty() && P.groups.back().kind == 0 &&
                           P.groups.back().space == s.space && !reads_red &&
@@ -661,6 +712,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
         }
         P.groups.back().stmts.push_back((int)i);
         group_of_stmt[i] = (int)P.groups.size() - 1;
+        group_of_var[s.lhs] = (int)P.groups.size() - 1;
         if (s.kind == 1) red_group_of_var[s.lhs] = (int)P.groups.size() - 1;
     }
     // which TEMP variables cross groups (or feed matmul) and need memory
@@ -676,6 +728,8 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
             if (def_group[v] < 0) return fail("temp variable %u read before it is assigned", v);
             if (def_group[v] != group_of_stmt[i] || s.kind == 2) P.needs_mem[v] = true;
         }
+        for (const auto& e : s.nodes)
+            if (is_gather(e.code)) P.needs_mem[s.nodes[e.kids[0]].src->arg] = true;
         if (s.kind == 1 && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
         if (s.kind == 2 && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
     }
@@ -967,6 +1021,11 @@ static std::string jit_source(const TraceProgram& P, const Group& G)
                 snprintf(b, sizeof b,
                          "        { V4 c = R%d, p = R%d, q = R%d; for (int i = 0; i < 4; i++) R%d.l[i] = as_double(%u, c.l[i]) != 0.0 ? p.l[i] : q.l[i]; }\n",
                          I.s1, I.s2, I.s3, I.dst, (unsigned)I.p2);
+                break;
+            case VM_GATHER:
+                snprintf(b, sizeof b, "        R%d = vm_gather(%s, %s, slot, y, %d, %uu, R%d, %uu, R%d, nval);\n",
+                         I.dst, var_lit(I.a, true).c_str(), chain_lit(I.p2).c_str(), I.e == 2 ? 1 : 0, I.b, I.s1,
+                         I.c, I.s2);
                 break;
             case VM_STORE:
                 snprintf(b, sizeof b, "        vm_store(%s, R%d, slot, y, x0, nval, %lluull, %lluull);\n",

@@ -38,6 +38,7 @@ enum VmOp : uint16_t {
     VM_STORE,      // var a <- R[s1]
     VM_ACC,        // acc[a] += (double) R[s1]
     VM_MOV,        // R[dst] <- R[s1]
+    VM_GATHER,     // R[dst] <- var a at subscripts R[s1] (prec b), R[s2] (prec c, e == 2); p2 = chain
 };
 
 struct VmIns {
@@ -617,6 +618,39 @@ __device__ __forceinline__ V4 vm_load(const VmVar& V, uint64_t slot, uint64_t y,
     }
     const uint64_t rowoff = ((slot % V.S) * V.H + (y % V.H)) * V.W;
     for (int i = 0; i < 4; i++) if (i < nval) r.l[i] = vm_load_elem(V.in, V.prec, rowoff + (x0 + i) % V.W);
+    return r;
+}
+
+// Gather subscript (interp/InterpGather.hpp:24-26, misc/SimpleFun1.cpp:196-216):
+// floor the element (identity for integers) and convert it to int as the
+// reference's x86 build does (NaN and out-of-range values give INT_MIN, uint32
+// reinterprets its bits); then "xf < 0 ? xf + W : xf", evaluated in size_t and
+// stored to an int, and "% W" on that int sign-extended to 64 bits.  The result
+// is always below W.
+__device__ __forceinline__ uint64_t gather_sub(uint32_t prec, uint64_t b, uint64_t W)
+{
+    int xf;
+    if (prec == CHAI_U32 || prec == CHAI_I32) xf = i_(b);
+    else if (prec == CHAI_F32) xf = x86_i((double)floorf(f_(b)));
+    else xf = x86_i(floor(d_(b)));
+    if (xf >= 0) return (uint64_t)xf < W ? (uint64_t)xf : (uint64_t)xf % W;
+    const int m = (int)(uint32_t)((uint64_t)(int64_t)xf + W);
+    return (uint64_t)(int64_t)m % W;
+}
+
+// gather1_floor / gather2_floor: the output has the data array's shape; C
+// reduces the caller's (y, slot) to the gather's own coordinate
+__device__ __forceinline__ V4 vm_gather(const VmVar& V, const VmChain& C, uint64_t slot, uint64_t y, int two,
+                                        uint32_t px, const V4& ix, uint32_t py, const V4& iy, int nval)
+{
+    V4 r = {{0, 0, 0, 0}};
+    const uint64_t sr = chain_mod(slot, C.S, C.nS) % V.S, yr = chain_mod(y, C.H, C.nH) % V.H;
+    for (int i = 0; i < 4; i++)
+        if (i < nval) {
+            const uint64_t xs = gather_sub(px, ix.l[i], V.W);
+            const uint64_t ys = two ? gather_sub(py, iy.l[i], V.H) : yr;
+            r.l[i] = vm_load_elem(V.in, V.prec, (sr * V.H + ys) * V.W + xs);
+        }
     return r;
 }
 

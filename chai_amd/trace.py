@@ -145,6 +145,25 @@ def smoothstep(e0, e1, x): return Expr(307, [lit(e0), lit(e1), lit(x)])
 def mad24(a, b, c): return Expr(406, [lit(a), lit(b), lit(c)])
 def mad_hi(a, b, c): return Expr(407, [lit(a), lit(b), lit(c)])
 def mad_sat(a, b, c): return Expr(408, [lit(a), lit(b), lit(c)])
+def _gather_data(a):
+    if not isinstance(a, Expr) or a.code != VAR:
+        raise TypeError("gather data must be a trace variable (assign the expression to a temp first)")
+    return a
+
+
+def gather1_floor(a, i):
+    """out(x, y) = a(floor(i(x, y)) wrapped into [0, W), y), shape and type of a
+    (api/chai.h:158, interp/InterpGather.hpp:20-30): a negative subscript gets W
+    added once, the result is reduced modulo W."""
+    return Expr(40, [_gather_data(a), lit(i)])
+
+
+def gather2_floor(a, i, j):
+    """out(x, y) = a(floor(i(x, y)) wrapped into [0, W), floor(j(x, y)) wrapped
+    into [0, H)) (api/chai.h:165, interp/InterpGather.hpp:34-48)."""
+    return Expr(41, [_gather_data(a), lit(i), lit(j)])
+
+
 def matmul(a, b): return Expr(120, [a, b])
 def matmul_g(a, b, c, alpha=1.0, beta=1.0): return Expr(121, [a, b, c], s0=alpha, s1=beta)
 

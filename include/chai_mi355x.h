@@ -293,7 +293,11 @@ int chai_hip_gemm_predict_variant(const chai_gemm_desc* desc, void* mlp, int* va
  * jitkrnl/GroupTogether.cpp:178-201): elementwise statements over one index
  * space fuse into one launch of the trace VM kernel; a sum/mean/dot fuses
  * into the kernel producing its operand; matmul runs the GEMM/GEMV family;
- * temporaries used inside one fused kernel never touch memory.  A trace the
+ * temporaries used inside one fused kernel never touch memory; gather1_floor /
+ * gather2_floor (bytecodes 40 / 41, node order data, x subscript[, y
+ * subscript]) read a variable at computed subscripts with the interpreter's
+ * wrap (interp/InterpGather.hpp:20-52), and a statement gathering from a
+ * variable the current kernel computes starts a new kernel.  A trace the
  * compiler cannot place on the GPU returns CHAI_ERR_ARG, so the caller falls
  * back to the interpreter (runtime/Scheduler.cpp:221-254). */
 enum { CHAI_TRACE_VAR = 0x10000 };

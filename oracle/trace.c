@@ -20,7 +20,9 @@
  *   - sum / mean / dot in a double accumulator (interp/InterpAccum.cpp,
  *     interp/InterpDotprod.cpp); mean divides by W*H;
  *   - matmul / matmulG with interpreter GEMM semantics (linalg.c);
- *   - rng_*_make with the canonical stream (rng.c).
+ *   - rng_*_make with the canonical stream (rng.c);
+ *   - gather1_floor / gather2_floor with the interpreter's subscript
+ *     arithmetic (interp/InterpGather.hpp:20-52), x86 float -> int conversion.
  * Floating point transcendentals are glibc's (as the interpreter's); the GPU
  * uses the device math library, so those results are compared with a stated
  * ulp tolerance, not bit for bit.
@@ -28,10 +30,15 @@
  * Pinned by: the integer scalar ops against the reference's own
  * misc/SimpleFun2.cpp / SimpleFun1.cpp compiled into oracle/_ref
  * (tests/test_oracle_ref.py); closed forms (MD5 of known messages as an
- * integer trace, sums of index ramps) in tests/test_oracle_trace.py.
+ * integer trace, sums of index ramps) in tests/test_oracle_trace.py; the
+ * gathers against a periodic 5-point stencil (np.roll, the reference's
+ * sample/filter.cpp workload), a binomial-lattice backward step (numpy
+ * slicing, sample/finance/american_option.cpp) and the subscript rule's closed
+ * form per range of the floored index, in tests/test_oracle_trace.py.
  */
 #define _GNU_SOURCE
 #include <math.h>
+#include <stddef.h>
 #include <stdlib.h>
 #include <string.h>
 #include "chai_oracle.h"
@@ -419,6 +426,32 @@ static int un_int(uint32_t op, int is_signed, uint32_t a, uint32_t* r)
 
 static size_t maxz(size_t a, size_t b) { return a > b ? a : b; }
 
+/* gather subscript, step 1 (interp/InterpGather.hpp:24, misc/SimpleFun1.cpp:196-216):
+ * floor the element (identity for the integer types) and convert to int the
+ * way the reference's x86 build does (cvttss2si / cvttsd2si: NaN and values
+ * outside int give INT_MIN; uint32 reinterprets its bits). */
+static int gather_floor(const Arr* a, size_t i)
+{
+    double v;
+    switch (a->prec) {
+        case 0: case 1: return (int)(int32_t)getu(a, i);
+        case 4: v = (double)floorf(((const float*)a->d)[i]); break;
+        default: v = floor(((const double*)a->d)[i]); break;
+    }
+    return (v != v || v >= 2147483648.0 || v < -2147483648.0) ? INT32_MIN : (int)v;
+}
+
+/* gather subscript, step 2 (interp/InterpGather.hpp:25-26): the interpreter's
+ * "(xf < 0) ? xf + W : xf" mixes int and size_t, so it is computed in size_t
+ * and stored back to an int; "xfM % W" then converts that int to size_t
+ * (sign-extending) before an unsigned 64-bit modulo. */
+static size_t gather_wrap(int xf, size_t W)
+{
+    const size_t m = xf < 0 ? (size_t)(ptrdiff_t)xf + W : (size_t)(ptrdiff_t)xf;
+    const int xfM = (int)(uint32_t)m;
+    return (size_t)(ptrdiff_t)xfM % W;
+}
+
 int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
 {
     Arr stk[MAXSTK];
@@ -524,6 +557,26 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                                               B.W, r.d, N, general, n->s0, n->s1)) rc = -1;
                 }
                 free(A.d); free(B.d);
+            } else if (c == 40 || c == 41) {
+                /* gather1_floor / gather2_floor (interp/InterpGather.cpp:20-60,
+                 * InterpGather.hpp:20-52): shape and type of the data array;
+                 * subscripts read by modulo broadcast at (x, y, slot) */
+                Arr iy = {0, 0, 0, 0, NULL}, ix, a;
+                if (c == 41) iy = stk[--sp];
+                ix = stk[--sp];
+                a = stk[--sp];
+                r = arr_new(a.prec, a.W, a.H, a.S);
+                const size_t e = esz(a.prec);
+                for (size_t s = 0; s < a.S; s++)
+                    for (size_t y = 0; y < a.H; y++)
+                        for (size_t x = 0; x < a.W; x++) {
+                            const size_t xr = gather_wrap(gather_floor(&ix, bidx(&ix, x, y, s)), a.W);
+                            const size_t yr = c == 41 ? gather_wrap(gather_floor(&iy, bidx(&iy, x, y, s)), a.H) : y;
+                            memcpy((char*)r.d + ((s * a.H + y) * a.W + x) * e,
+                                   (const char*)a.d + ((s * a.H + yr) * a.W + xr) * e, e);
+                        }
+                free(a.d); free(ix.d);
+                if (c == 41) free(iy.d);
             } else if (c == 23) {                          /* cond */
                 Arr b2 = stk[--sp], b1 = stk[--sp], cc = stk[--sp];
                 const int p = b1.prec > b2.prec ? b1.prec : b2.prec;

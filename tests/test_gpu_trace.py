@@ -280,3 +280,62 @@ def test_integer_ternary_ops_bit_exact(chai, oracle, engine, prec):
     g, _ = run_gpu(t, data)
     for v in outs:
         assert np.array_equal(g[v.index], o[v.index]), v.index
+
+
+# ------------------------------------------------------------------- gathers
+@pytest.mark.parametrize("N", [12, 2048])
+def test_gather_filter_stencil_bit_exact(chai, oracle, engine, N):
+    """sample/filter.cpp's five gather2_floor reads; N = 12 against the
+    oracle, both sizes against numpy's periodic rolls (exact in f32)"""
+    from tests.trace_cases import filter_expected, filter_trace
+    t, A, B = filter_trace(N)
+    a = np.random.default_rng(N).standard_normal((1, N, N)).astype(np.float32)
+    r, prog = run_gpu(t, {A.index: a})
+    assert np.array_equal(r[B.index], filter_expected(a))
+    if N <= 64:
+        assert np.array_equal(r[B.index], oracle.trace_eval(t, {A.index: a})[B.index])
+    assert prog.info()[0] == 1
+
+
+def test_gather_lattice_one_kernel_per_step(chai, oracle, engine):
+    """american_option.cpp's backward induction: each step gathers the
+    previous step's temp, so the trace splits into one kernel per step;
+    f64 values bit-exact against the oracle and numpy's recurrence"""
+    from tests.trace_cases import LATTICE, lattice_expected, lattice_trace
+    N, steps = 101, 6
+    t, (P, O), (Pn, On) = lattice_trace(N, steps, **LATTICE)
+    p0 = 100.0 * LATTICE["up"] ** (2.0 * np.arange(N) - (N - 1))
+    o0 = np.maximum(p0 - LATTICE["strike"], 0.0)
+    r, prog = run_gpu(t, {P.index: p0, O.index: o0})
+    ref = oracle.trace_eval(t, {P.index: p0, O.index: o0})
+    ep, eo = lattice_expected(p0, o0, steps, **LATTICE)
+    for v, e in ((Pn, ep), (On, eo)):
+        assert np.array_equal(r[v.index], ref[v.index])
+        assert np.array_equal(r[v.index].ravel(), e)
+    assert prog.info()[0] == steps
+
+
+@pytest.mark.parametrize("prec", [U32, I32, F32, F64])
+@pytest.mark.parametrize("two", [False, True])
+def test_gather_subscript_rule_bit_exact(chai, oracle, engine, prec, two):
+    """every range of the subscript rule (negative above / below -W, past W,
+    NaN, inf, out of int range), subscripts with more slots than the data"""
+    from tests.trace_cases import gather_rule_trace
+    t, data, o, expect = gather_rule_trace(prec, two, seed=int(prec) * 2 + two)
+    r, _ = run_gpu(t, data)
+    assert np.array_equal(r[o.index], oracle.trace_eval(t, data)[o.index])
+    assert np.array_equal(r[o.index], expect)
+
+
+def test_gather_broadcast_reduction_and_second_kernel(chai, oracle, engine):
+    """gathers smaller than their statement (modulo broadcast), subscript
+    expressions, a gather under a sum reduction and a gather from a temp of
+    the same trace"""
+    from tests.trace_cases import gather_bcast_trace
+    t, data, outs, expect = gather_bcast_trace()
+    r, prog = run_gpu(t, data)
+    ref = oracle.trace_eval(t, data)
+    for o, e in zip(outs, expect):
+        assert np.array_equal(r[o.index], ref[o.index])
+        assert np.array_equal(r[o.index].ravel(), e.ravel())
+    assert prog.info()[0] >= 3

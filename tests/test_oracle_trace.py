@@ -135,3 +135,68 @@ def test_random_traces_run_on_oracle(oracle, seed):
         t, data, outs, red = random_trace(seed, kind)
         r = oracle.trace_eval(t, data)
         assert r[red.index].size == 2
+
+
+# ------------------------------------------------------------------- gathers
+def test_gather_filter_sample_is_a_periodic_stencil(oracle):
+    """sample/filter.cpp (N = 12, as the sample): five gather2_floor reads at
+    (X +- 1, Y +- 1) equal numpy's periodic rolls, summed in the same order"""
+    from tests.trace_cases import filter_expected, filter_trace
+    t, A, B = filter_trace(12)
+    a = np.random.default_rng(1).standard_normal((1, 12, 12)).astype(np.float32)
+    r = oracle.trace_eval(t, {A.index: a})
+    assert np.array_equal(r[B.index], filter_expected(a))
+
+
+def test_gather_lattice_sample_is_a_binomial_step(oracle):
+    """sample/finance/american_option.cpp: gather1_floor(option, IDX) and
+    (option, IDX + 1) are the option array and its left shift (IDX + 1 = N
+    wraps to 0); six backward steps equal numpy's recurrence bit for bit"""
+    from tests.trace_cases import LATTICE, lattice_expected, lattice_trace
+    N, steps = 101, 6
+    t, (P, O), (Pn, On) = lattice_trace(N, steps, **LATTICE)
+    p0 = 100.0 * LATTICE["up"] ** (2.0 * np.arange(N) - (N - 1))
+    o0 = np.maximum(p0 - LATTICE["strike"], 0.0)
+    r = oracle.trace_eval(t, {P.index: p0, O.index: o0})
+    ep, eo = lattice_expected(p0, o0, steps, **LATTICE)
+    assert np.array_equal(r[Pn.index].ravel(), ep) and np.array_equal(r[On.index].ravel(), eo)
+
+
+@pytest.mark.parametrize("prec", [U32, I32, F32, F64])
+@pytest.mark.parametrize("two", [False, True])
+def test_gather_subscript_rule_closed_form(oracle, prec, two):
+    """every range of the floored subscript (inside, negative above / below
+    -W, past W, NaN / inf / out of int range) on a data array of non power
+    of two width whose elements hold their own flat index; subscripts with
+    more slots than the data"""
+    from tests.trace_cases import gather_rule_trace
+    t, data, o, expect = gather_rule_trace(prec, two, seed=int(prec) * 2 + two)
+    r = oracle.trace_eval(t, data)
+    assert np.array_equal(r[o.index], expect)
+
+
+def test_gather_subscript_closed_form_cases():
+    """the closed form itself on hand-worked cases (W = 12): -1 -> 11,
+    -12 -> 0, -13 -> (2^64 - 1) mod 12 = 3, 25 -> 1; INT_MIN for NaN"""
+    from tests.trace_cases import subscript_closed_form as f
+    assert [f(v, I32, 12) for v in (-1, -12, -13, 25, 0, 11)] == [11, 0, 3, 1, 0, 11]
+    assert (2 ** 64 - 1) % 12 == 3
+    assert f(np.float32(-0.5), F32, 12) == 11 and f(np.float64(np.nan), F64, 12) == (2 ** 64 - 2 ** 31 + 12) % 12
+    assert f(np.uint32(2 ** 32 - 1), U32, 8) == 7        # bits of -1; power of two width: true modulo
+
+
+def test_gather_broadcast_reduction_and_second_kernel(oracle):
+    from tests.trace_cases import gather_bcast_trace
+    t, data, outs, expect = gather_bcast_trace()
+    r = oracle.trace_eval(t, data)
+    assert np.array_equal(r[outs[0].index], expect[0])
+    assert np.array_equal(r[outs[1].index].ravel(), expect[1])
+    assert np.array_equal(r[outs[2].index], expect[2])
+
+
+def test_gather_data_must_be_a_variable():
+    from chai_amd.trace import gather1_floor
+    t = Trace()
+    a = t.input(F32, 8)
+    with pytest.raises(TypeError):
+        gather1_floor(a + 1, a)

@@ -14,7 +14,10 @@ import subprocess
 import numpy as np
 import pytest
 
-from tests.trace_cases import bcast_trace, md5_expected, md5_hex, md5_inputs, md5_trace, MSGS, random_trace
+from chai_amd import F32, F64, I32, U32
+
+from tests.trace_cases import (bcast_trace, filter_expected, filter_trace, gather_bcast_trace, gather_rule_trace,
+                               md5_expected, md5_hex, md5_inputs, md5_trace, MSGS, random_trace)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(os.path.dirname(HERE), "chai_amd", "csrc")
@@ -36,7 +39,7 @@ extern "C" void emu_run(void** bufs, const uint32_t* slots, int nslots, unsigned
 '''
 
 
-def emulate(t, data, tmp_path, kernel=0):
+def emulate(t, data, tmp_path, kernel=0, slots_space=None):
     from chai_amd import trace as tr
     from oracle import oracle as o
     src = tr.emit_source(t, kernel)
@@ -57,7 +60,7 @@ def emulate(t, data, tmp_path, kernel=0):
                    else np.zeros((S, H, W), dtype=dt))
     bufs = (ctypes.c_void_p * len(arrs))(*[arrs[i].ctypes.data for i in range(len(arrs))])
     sl = (ctypes.c_uint32 * len(slots))(*slots)
-    S = max(v[4] for v in t.vars)
+    S = slots_space or max(v[4] for v in t.vars)     # slots of the kernel's index space
     part = np.zeros(1 << 16)
     lib.emu_run(bufs, sl, len(slots), S, max(1, (vps + 255) // 256), part.ctypes.data_as(ctypes.c_void_p))
     return arrs, o.trace_eval(t, data)
@@ -78,6 +81,34 @@ def test_emitted_random_traces_match_oracle(kind, seed, tmp_path):
     got, ref = emulate(t, data, tmp_path)
     for v in outs:
         np.testing.assert_array_equal(got[v.index], ref[v.index], err_msg="%s seed %d var %d" % (kind, seed, v.index))
+
+
+@pytest.mark.parametrize("N", [12, 37])
+def test_emitted_filter_stencil_matches_oracle(N, tmp_path):
+    """sample/filter.cpp's gather2_floor stencil through the emitted kernel"""
+    t, A, B = filter_trace(N)
+    a = np.random.default_rng(N).standard_normal((1, N, N)).astype(np.float32)
+    got, ref = emulate(t, {A.index: a}, tmp_path)
+    np.testing.assert_array_equal(got[B.index], ref[B.index])
+    np.testing.assert_array_equal(got[B.index], filter_expected(a))
+
+
+@pytest.mark.parametrize("prec", [U32, I32, F32, F64])
+@pytest.mark.parametrize("two", [False, True])
+def test_emitted_gather_subscripts_match_oracle(prec, two, tmp_path):
+    """every range of the subscript rule, subscript slots broadcast by modulo"""
+    t, data, o, expect = gather_rule_trace(prec, two, seed=int(prec) * 2 + two)
+    got, ref = emulate(t, data, tmp_path, slots_space=2)
+    np.testing.assert_array_equal(got[o.index], ref[o.index])
+    np.testing.assert_array_equal(got[o.index], expect)
+
+
+def test_emitted_gather_under_broadcast_matches_oracle(tmp_path):
+    """gathers smaller than the statement (nested modulo chains) and
+    subscript expressions mixing inputs with index arrays (first kernel)"""
+    t, data, outs, expect = gather_bcast_trace()
+    got, ref = emulate(t, data, tmp_path, kernel=0, slots_space=4)
+    np.testing.assert_array_equal(got[outs[0].index], ref[outs[0].index])
 
 
 def test_emitted_md5_matches_hashlib(tmp_path):
@@ -118,7 +149,6 @@ def _ops_trace(codes, prec, arity, n, seed):
     return t, data, outs
 
 
-from chai_amd import F32, F64, I32, U32  # noqa: E402
 from chai_amd.trace import Trace  # noqa: E402
 
 

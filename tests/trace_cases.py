@@ -8,7 +8,7 @@ import numpy as np
 from chai_amd import F32, F64, I32, U32
 from chai_amd.trace import (Trace, cond, exp, f32, fabs, fma, hadd, log, lor, maxe, mean, mine,
                             mul_hi, rotate, sqrt, sum_, u32, i32, add_sat, sub_sat, rhadd,
-                            abs_diff, mul24, clamp, dot)
+                            abs_diff, mul24, clamp, dot, gather1_floor, gather2_floor)
 
 # ---------------------------------------------------------------- MD5 (RFC 1321)
 MD5_S = [7, 12, 17, 22] * 4 + [5, 9, 14, 20] * 4 + [4, 11, 16, 23] * 4 + [6, 10, 15, 21] * 4
@@ -169,3 +169,150 @@ def bcast_trace(seed, W=37, H=5, slots=2, depth=4, n_stmt=3):
         full = t.rng_uniform(U32, W, H, slots, seed=500 + i)
         outs.append(t.assign(t.output(U32, W, H, slots), e + (full & u32(0))))
     return t, data, outs
+
+
+# ------------------------------------------------------------------- gathers
+def filter_trace(N):
+    """The reference's sample/filter.cpp: a 5-point stencil of gather2_floor
+    with index_f32 subscripts shifted by -1 / +1, periodic through the
+    subscript wrap (x = -1 -> W - 1, x = W -> 0)."""
+    t = Trace()
+    A = t.input(F32, N, N)
+    X = Trace.index(F32, 0, N, N)
+    Y = Trace.index(F32, 1, N, N)
+    B = t.assign(t.output(F32, N, N),
+                 gather2_floor(A, X, Y) + gather2_floor(A, X - 1, Y) + gather2_floor(A, X + 1, Y)
+                 + gather2_floor(A, X, Y - 1) + gather2_floor(A, X, Y + 1))
+    return t, A, B
+
+
+def filter_expected(a):
+    """[S][H][W] -> the same stencil with numpy rolls, summed in the trace's order"""
+    r = lambda k, ax: np.roll(a, k, axis=ax)  # noqa: E731
+    return (((a + r(1, -1)) + r(-1, -1)) + r(1, -2)) + r(-1, -2)
+
+
+LATTICE = dict(up=1.0207, pr_up=0.5102, pr_down=0.4898, disc=1.0005, strike=100.0)
+
+
+def lattice_trace(N, steps, up, pr_up, pr_down, disc, strike):
+    """sample/finance/american_option.cpp's backward induction, `steps` steps
+    in one trace: each step gathers the previous step's option values at IDX
+    and IDX + 1, a temp computed in the same trace, so every step is a kernel
+    of its own."""
+    t = Trace()
+    prices, option = t.input(F64, N), t.input(F64, N)
+    ins = (prices, option)
+    IDX = Trace.index(F64, 0, N)
+    for k in range(steps):
+        mk = t.output if k == steps - 1 else t.temp
+        no_ex = (pr_down * gather1_floor(option, IDX) + pr_up * gather1_floor(option, IDX + 1)) / disc
+        prices = t.assign(mk(F64, N), up * prices)
+        option = t.assign(mk(F64, N), maxe(no_ex, prices - strike))
+    return t, ins, (prices, option)
+
+
+def lattice_expected(prices, option, steps, up, pr_up, pr_down, disc, strike):
+    for _ in range(steps):
+        no_ex = (pr_down * option + pr_up * np.roll(option, -1)) / disc
+        prices = up * prices
+        ex = prices - strike
+        option = np.where(no_ex < ex, ex, no_ex)       # max(a, b) = a < b ? b : a
+    return prices, option
+
+
+def gather_subscripts(prec, shape, span, rng):
+    """subscript arrays of one type covering every range of the wrap rule:
+    in [0, span), negative above and below -span, past span, and the
+    conversion's special values"""
+    n = int(np.prod(shape))
+    if prec in (U32, I32):
+        v = rng.integers(-3 * span - 2, 3 * span + 3, size=n).astype(np.int64)
+        sp = [-2 ** 31, 2 ** 31 - 1, -2 ** 31 + 7, -span, -span - 1, span, 0, -1]
+        v[:min(n, len(sp))] = sp[:n]
+        return (v.astype(np.int32).view(np.uint32) if prec == U32 else v.astype(np.int32)).reshape(shape)
+    v = rng.uniform(-3 * span - 2, 3 * span + 3, size=n)
+    sp = [np.nan, np.inf, -np.inf, 3e9, -3e9, -0.0, -0.5, 1e-30, -1e-30, float(span) - 0.25,
+          -float(span), 2.0 ** 31, -2.0 ** 31, float(-span) - 0.5]
+    v[:min(n, len(sp))] = sp[:n]
+    return v.astype(np.float32 if prec == F32 else np.float64).reshape(shape)
+
+
+def subscript_closed_form(v, prec, W):
+    """the wrap rule per range of the floored subscript xf (C's int / size_t
+    conversions): xf >= 0 -> xf mod W; -W <= xf < 0 -> xf + W (0 at -W);
+    xf < -W -> the int xf + W sign-extended to 64 bits, (2^64 + xf + W) mod W"""
+    if prec == U32:
+        xf = int(np.uint32(v).view(np.int32))
+    elif prec == I32:
+        xf = int(v)
+    else:
+        f = math.floor(float(v)) if math.isfinite(float(v)) else None
+        xf = f if f is not None and -2 ** 31 <= f < 2 ** 31 else -2 ** 31
+    if xf >= 0:
+        return xf % W
+    if xf >= -W:
+        return (xf + W) % W
+    return (2 ** 64 + xf + W) % W
+
+
+def gather_rule_trace(prec_i, two, W=12, H=10, S=2, Si=3, seed=0):
+    """data element (s, y, x) holds its own flat index; gathered through
+    subscripts of type prec_i with more slots than the data (the output keeps
+    the data's slots, the subscripts are read at slot % Si)."""
+    rng = np.random.default_rng(seed)
+    t = Trace()
+    a = t.input(I32, W, H, S)
+    i = t.input(prec_i, W, H, Si)
+    j = t.input(prec_i, W, 2, 1) if two else None
+    o = t.assign(t.output(I32, W, H, S), gather2_floor(a, i, j) if two else gather1_floor(a, i))
+    data = {a.index: np.arange(S * H * W, dtype=np.int32).reshape(S, H, W),
+            i.index: gather_subscripts(prec_i, (Si, H, W), W, rng)}
+    if two:
+        data[j.index] = gather_subscripts(prec_i, (1, 2, W), H, rng)
+    expect = np.zeros((S, H, W), dtype=np.int32)
+    for s_ in range(S):
+        for y in range(H):
+            for x in range(W):
+                xr = subscript_closed_form(data[i.index][s_ % Si, y, x], prec_i, W)
+                yr = subscript_closed_form(data[j.index][0, y % 2, x], prec_i, H) if two else y
+                expect[s_, y, x] = (s_ * H + yr) * W + xr
+    return t, data, o, expect
+
+
+def gather_bcast_trace(seed=0):
+    """gathers whose results are smaller than the statement (broadcast by
+    modulo), a subscript expression mixing an input and an index array, a
+    gather under a sum reduction, and a gather from a variable computed in
+    the same trace (a second kernel)."""
+    rng = np.random.default_rng(seed)
+    t = Trace()
+    a = t.input(F32, 5, 3, 2)
+    i = t.input(I32, 5, 1, 1)
+    b = t.input(F32, 10, 6, 4)
+    yi = Trace.index(I32, 1, 5, 3)
+    o1 = t.assign(t.output(F32, 10, 6, 4), gather1_floor(a, i) * b + gather2_floor(a, i * 3 - 7, yi + i))
+    s1 = t.assign(t.output(F32, 1, 1, 2), sum_(gather1_floor(a, i + 1)))
+    tmp = t.assign(t.temp(F32, 5, 3, 2), a * f32(2.0) + f32(1.0))
+    o2 = t.assign(t.output(F32, 5, 3, 2), gather2_floor(tmp, Trace.index(F32, 0, 5) * f32(-1.5), yi - 1))
+    A = rng.standard_normal((2, 3, 5)).astype(np.float32)
+    I = rng.integers(-20, 20, size=(1, 1, 5)).astype(np.int32)
+    B = rng.standard_normal((4, 6, 10)).astype(np.float32)
+    wrap = lambda v, W: subscript_closed_form(v, I32, W)  # noqa: E731
+    g1 = A[:, :, [wrap(v, 5) for v in I.ravel()]]                       # (2, 3, 5)
+    g2 = np.empty_like(A)
+    for y in range(3):
+        for x in range(5):
+            g2[:, y, x] = A[:, wrap(y + I[0, 0, x], 3), wrap(I[0, 0, x] * 3 - 7, 5)]
+    tile = lambda g: g[np.arange(4)[:, None, None] % g.shape[0], np.arange(6)[None, :, None] % g.shape[1],  # noqa: E731
+                       np.arange(10)[None, None, :] % g.shape[2]]
+    e1 = tile(g1) * B + tile(g2)
+    g3 = A[:, :, [wrap(v + 1, 5) for v in I.ravel()]]
+    e_s1 = g3.astype(np.float64).reshape(2, -1).sum(1).astype(np.float32)   # exact in double
+    T = A * np.float32(2.0) + np.float32(1.0)
+    e2 = np.empty_like(A)
+    for y in range(3):
+        for x in range(5):
+            xf = np.float32(x) * np.float32(-1.5)
+            e2[:, y, x] = T[:, wrap(y - 1, 3), subscript_closed_form(xf, F32, 5)]
+    return t, {a.index: A, i.index: I, b.index: B}, (o1, s1, o2), (e1, e_s1, e2)
