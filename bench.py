@@ -11,7 +11,7 @@ as api/RngFun.cpp:30 does).
 Other workloads (--workload): ragged (4096-instance ragged GEMM batch + fused
 epilogue), autotune (learned tile model: regret and cold start), mc (fused
 Monte Carlo trace, Philox normals, GEMV 16384^2), dgemm (FP64 MFMA 4096^3),
-mixed (i32 x f32 4096^3).  --profile-run is the short program profiled with
+mixed (i32 x f32 4096^3), gather (gather2_floor stencil 8192^2).  --profile-run is the short program profiled with
 rocprofv3; --pmc-summary turns two counter runs into the traffic summary.
 
 --gpus N > 1 is refused: this build measures one GPU.
@@ -34,6 +34,7 @@ PEAK_FP64_TFLOPS = 78.6        # MI355X FP64 MFMA
 PEAK_HBM_GBS = 8000.0
 METRIC = "GEMM TFLOP/s (bf16x3 MFMA vs FP32) at FP32-level error; autotune cold-start s"
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_bf16x3_8192_pmc.txt")
+PMC_SUMMARY_GATHER = os.path.join(ROOT, "profiles", "gather_stencil_8192_pmc.txt")
 MODEL_FILE = os.path.join(ROOT, "chai_amd", "data", "tile_mlp.npz")
 
 
@@ -43,7 +44,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="sgemm",
-                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed"])
+                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed", "gather"])
     p.add_argument("--size", type=int, default=8192)
     p.add_argument("--fp32-steps", type=int, default=3)
     p.add_argument("--cpu-size", type=int, default=1024,
@@ -116,10 +117,10 @@ def cpu_baseline_gemm(n):
                       % (n, dt)}
 
 
-def read_traffic():
-    """HBM bytes per launch of the bf16x3 GEMM kernel from the committed counter summary."""
+def read_traffic(path=PMC_SUMMARY):
+    """HBM bytes per launch of the measured kernel from a committed counter summary."""
     try:
-        with open(PMC_SUMMARY) as f:
+        with open(path) as f:
             m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
@@ -187,11 +188,35 @@ def profile_run(args):
     big = torch.empty(1 << 28, dtype=torch.float32, device="cuda")     # 1 GiB
     chai.rng_fill(big, seed=1)            # writes exactly 2^30 bytes
     chai.reduce(big, "sum")               # reads exactly 2^30 bytes
+    torch.empty_like(big).copy_(big)      # plain device copy: reads and writes 2^30 bytes
     A, B = uniform((n, n), 42), uniform((n, n), 42 + n * n)
     C = torch.empty(n, n, device="cuda")
     chai.gemm_tune(A, B, C, trials=1)     # the variant the benchmark runs
     for _ in range(3):
         chai.gemm(A, B, C=C, mode=chai.GEMM_BF16X3)
+    torch.cuda.synchronize()
+    print("profile run done", flush=True)
+
+
+def profile_run_gather(args):
+    """Short program for rocprofv3: the calibration kernels, then three JIT
+    launches of the 8192^2 gather stencil."""
+    import torch
+    import chai_amd as chai
+    from chai_amd.workloads import filter_trace
+    chai.init(0)
+    big = torch.empty(1 << 28, dtype=torch.float32, device="cuda")
+    chai.rng_fill(big, seed=1)
+    chai.reduce(big, "sum")
+    torch.empty_like(big).copy_(big)      # plain device copy: reads and writes 2^30 bytes
+    del big
+    N = 8192
+    t, A, B = filter_trace(N)
+    a = uniform((N, N), 42).view(-1)
+    b = torch.empty(N * N, device="cuda")
+    prog = t.compile()
+    for _ in range(3):
+        prog.run({A: a, B: b})
     torch.cuda.synchronize()
     print("profile run done", flush=True)
 
@@ -215,11 +240,15 @@ def _counter_rows(d, counter):
     return rows
 
 
-def pmc_summary(fetch_dir, write_dir):
+def pmc_summary(fetch_dir, write_dir, workload="sgemm"):
     """Per-kernel FETCH_SIZE / WRITE_SIZE (KiB per dispatch) from two rocprofv3 runs of
-    --profile-run, scaled by the calibration kernels of known traffic (rng_fill
-    writes 2^30 B, the reduction's partial kernel reads 2^30 B)."""
+    --profile-run, each scaled by what the same counter reads for a plain copy
+    kernel of known size (the 2^30-byte device copy: the copy dispatch with
+    the largest count); the readings of rng_fill (writes 2^30 B) and of the
+    reduction's partial kernel (reads 2^30 B) are listed beside it."""
     fe, wr = _counter_rows(fetch_dir, "FETCH_SIZE"), _counter_rows(write_dir, "WRITE_SIZE")
+    f_copy = max([v for k, r in fe.items() if "copy" in k.lower() for v in r] or [0.0]) or None
+    w_copy = max([v for k, r in wr.items() if "copy" in k.lower() for v in r] or [0.0]) or None
 
     def pick(rows, pat):
         # the matching kernel with the most dispatches (the tile search runs
@@ -231,26 +260,34 @@ def pmc_summary(fetch_dir, write_dir):
         return (best, sum(rows[best]) / len(rows[best])) if best else (None, None)
     _, f_cal = pick(fe, r"reduce_partial")
     _, w_cal = pick(wr, r"rng_fill_32")
-    kf, f_gemm = pick(fe, r"gemm_x3_kernel")
-    kw, w_gemm = pick(wr, r"gemm_x3_kernel")
-    fetch_scale = (2 ** 30) / (f_cal * 1024) if f_cal else None
-    write_scale = (2 ** 30) / (w_cal * 1024) if w_cal else None
-    lines = ["# bf16x3 GEMM 8192^3: HBM traffic per launch from rocprofv3 counters",
+    pat = r"chai_trace_kernel" if workload == "gather" else r"gemm_x3_kernel"
+    kf, f_gemm = pick(fe, pat)
+    kw, w_gemm = pick(wr, pat)
+    fetch_scale = (2 ** 30) / (f_copy * 1024) if f_copy else None
+    write_scale = (2 ** 30) / (w_copy * 1024) if w_copy else None
+    wl = " --workload gather" if workload == "gather" else ""
+    lines = ["# %s: HBM traffic per launch from rocprofv3 counters" % (
+                 "gather2_floor 5-point stencil 8192^2 f32 (JIT kernel)" if workload == "gather"
+                 else "bf16x3 GEMM 8192^3"),
              "# commands (from the repository root):",
-             "#   rocprofv3 --pmc FETCH_SIZE -d <fetch_dir> -o run -- python3 bench.py --profile-run",
-             "#   rocprofv3 --pmc WRITE_SIZE -d <write_dir> -o run -- python3 bench.py --profile-run",
-             "#   python3 bench.py --pmc-summary <fetch_dir> <write_dir>",
-             "calibration_fetch_KiB(reduce_partial, 2^30 B read) = %s" % f_cal,
-             "calibration_write_KiB(rng_fill_32, 2^30 B written) = %s" % w_cal,
+             "#   rocprofv3 --pmc FETCH_SIZE -d <fetch_dir> -o run -- python3 bench.py --profile-run" + wl,
+             "#   rocprofv3 --pmc WRITE_SIZE -d <write_dir> -o run -- python3 bench.py --profile-run" + wl,
+             "#   python3 bench.py --pmc-summary <fetch_dir> <write_dir>" + wl,
+             "calibration_fetch_KiB(device copy, 2^30 B read) = %s" % f_copy,
+             "calibration_write_KiB(device copy, 2^30 B written) = %s" % w_copy,
+             "reading_fetch_KiB(reduce_partial, 2^30 B read) = %s" % f_cal,
+             "reading_write_KiB(rng_fill_32, 2^30 B written) = %s" % w_cal,
              "fetch_scale = %s" % fetch_scale, "write_scale = %s" % write_scale,
              "kernel = %s" % (kf or kw),
              "fetch_KiB_per_dispatch = %s" % f_gemm, "write_KiB_per_dispatch = %s" % w_gemm]
     if f_gemm and w_gemm and fetch_scale and write_scale:
         t = f_gemm * 1024 * fetch_scale + w_gemm * 1024 * write_scale
         n = 8192
-        alg = 2 * 2 * 2 * n * n + 4 * n * n
+        alg = 8 * n * n if workload == "gather" else 2 * 2 * 2 * n * n + 4 * n * n
         lines.append("traffic_bytes_per_launch = %.0f" % t)
-        lines.append("algorithmic_bytes_per_launch = %d  # bf16 hi+lo panels of A and B^T read once, C written" % alg)
+        lines.append("algorithmic_bytes_per_launch = %d  # %s" % (alg, (
+            "A read once, B written once (f32)" if workload == "gather"
+            else "bf16 hi+lo panels of A and B^T read once, C written")))
         lines.append("traffic_over_algorithmic = %.2f" % (t / alg))
     lines.append("# per kernel, mean per dispatch (KiB, unscaled):")
     for k in sorted(set(fe) | set(wr)):
@@ -552,6 +589,62 @@ def run_mc(args):
     emit(line)
 
 
+# ------------------------------------------------------------------ gather
+def run_gather(args):
+    """sample/filter.cpp at N = 8192: one fused kernel of five gather2_floor
+    reads; algorithmic bytes 4 read + 4 written per element (the stencil's
+    neighbours come from L2 / the same rows)"""
+    import numpy as np
+    import torch
+    import chai_amd as chai
+    from chai_amd.trace import set_jit
+    from chai_amd.workloads import filter_expected, filter_trace
+    chai.init(0)
+    N = 8192
+    t, A, B = filter_trace(N)
+    a = uniform((N, N), 42)
+    b = torch.empty(N * N, device="cuda")
+    res = {}
+    for eng in ("jit", "vm"):
+        set_jit(eng == "jit")
+        prog = t.compile()
+        res[eng] = timed(lambda: prog.run({A: a.view(-1), B: b}), args.steps, args.warmup)
+    set_jit(True)
+    ms = res["jit"]
+    gbs = 8.0 * N * N / (ms * 1e-3) / 1e9
+    # correctness of the measured kernel on a corner block, against the periodic stencil
+    an = a.cpu().numpy()
+    ok = bool(np.array_equal(b.view(N, N).cpu().numpy()[:64, :64],
+                             filter_expected(an[None])[0][:64, :64]))
+    at = a.view(N, N)
+    ms_torch = timed(lambda: at + torch.roll(at, 1, 1) + torch.roll(at, -1, 1) + torch.roll(at, 1, 0)
+                     + torch.roll(at, -1, 0), args.steps, args.warmup)
+    line = base_line(args, round(gbs, 1), "GB/s", round(ms, 4), "f32",
+                     {"workload": "gather2_floor 5-point stencil (sample/filter.cpp) on 8192x8192 f32"})
+    line["metric"] = "fused gather stencil GB/s (algorithmic bytes)"
+    line["vm_ms"] = round(res["vm"], 4)
+    line["torch_roll_stencil_ms"] = round(ms_torch, 4)
+    line["matches_periodic_stencil"] = ok
+    line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": read_traffic(PMC_SUMMARY_GATHER),
+                        "kernel": "chai_trace_kernel"}
+    if not args.no_cpu:
+        from oracle import oracle as o
+        n = 2048
+        tc, Ac, Bc = filter_trace(n)
+        x = np.random.default_rng(0).standard_normal((1, n, n)).astype(np.float32)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < 10.0:
+            o.trace_eval(tc, {Ac.index: x})
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        line["cpu_baseline"] = {"value": round(8.0 * n * n / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
+                                "kind": "port", "sample": "oracle trace evaluation of the stencil on 2048x2048, "
+                                                          "%d runs, %.2f s each" % (reps, dt)}
+    emit(line)
+
+
 # ------------------------------------------------------------ dgemm / mixed
 def run_dense(args, mixed):
     import numpy as np
@@ -604,7 +697,7 @@ def run_dense(args, mixed):
 def main():
     args = parse()
     if args.pmc_summary:
-        pmc_summary(*args.pmc_summary)
+        pmc_summary(*args.pmc_summary, workload=args.workload)
         return
     if args.kernel_stats:
         kernel_stats(*args.kernel_stats)
@@ -619,6 +712,8 @@ def main():
         sys.exit(2)
     if args.profile_run and args.workload == "ragged":
         profile_run_ragged(args)
+    elif args.profile_run and args.workload == "gather":
+        profile_run_gather(args)
     elif args.profile_run:
         profile_run(args)
     elif args.workload == "sgemm":
@@ -629,6 +724,8 @@ def main():
         run_autotune(args)
     elif args.workload == "mc":
         run_mc(args)
+    elif args.workload == "gather":
+        run_gather(args)
     else:
         run_dense(args, args.workload == "mixed")
 

@@ -172,24 +172,7 @@ def bcast_trace(seed, W=37, H=5, slots=2, depth=4, n_stmt=3):
 
 
 # ------------------------------------------------------------------- gathers
-def filter_trace(N):
-    """The reference's sample/filter.cpp: a 5-point stencil of gather2_floor
-    with index_f32 subscripts shifted by -1 / +1, periodic through the
-    subscript wrap (x = -1 -> W - 1, x = W -> 0)."""
-    t = Trace()
-    A = t.input(F32, N, N)
-    X = Trace.index(F32, 0, N, N)
-    Y = Trace.index(F32, 1, N, N)
-    B = t.assign(t.output(F32, N, N),
-                 gather2_floor(A, X, Y) + gather2_floor(A, X - 1, Y) + gather2_floor(A, X + 1, Y)
-                 + gather2_floor(A, X, Y - 1) + gather2_floor(A, X, Y + 1))
-    return t, A, B
-
-
-def filter_expected(a):
-    """[S][H][W] -> the same stencil with numpy rolls, summed in the trace's order"""
-    r = lambda k, ax: np.roll(a, k, axis=ax)  # noqa: E731
-    return (((a + r(1, -1)) + r(-1, -1)) + r(1, -2)) + r(-1, -2)
+from chai_amd.workloads import filter_expected, filter_trace  # noqa: E402,F401  (shared with bench.py)
 
 
 LATTICE = dict(up=1.0207, pr_up=0.5102, pr_down=0.4898, disc=1.0005, strike=100.0)
