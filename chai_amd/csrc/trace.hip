@@ -190,8 +190,21 @@ bool is_index(uint32_t c) { return c >= 42 && c <= 49; }
 bool is_red(uint32_t c) { return c == 130 || c == 131 || c == 132; }
 bool is_mm(uint32_t c) { return c == 120 || c == 121; }
 bool is_gather(uint32_t c) { return c == 40 || c == 41; }
+bool is_litdata(uint32_t c) { return c >= 68 && c <= 82; }
 
 int prec_of_suffix(int k) { return k == 0 ? CHAI_U32 : k == 1 ? CHAI_I32 : k == 2 ? CHAI_F32 : CHAI_F64; }
+
+// ones1/ones2/zeros1/zeros2 leaves (bytecode/ByteCodes.hpp:90-105): the value
+// and type the reference's translator gives them (runtime/Translator.cpp:
+// 116-131, jitast/AstLitdata.cpp).  The dispatch table keeps the last
+// registration of a code, and zeros2_f64 is declared as 73, the code of
+// ones2_i32, so 73 is an f64 zero.
+void litdata(uint32_t c, int& prec, double& v)
+{
+    if (c == 73) { prec = CHAI_F64; v = 0.0; return; }
+    prec = prec_of_suffix((int)((c - 68) % 4));
+    v = c < 76 ? 1.0 : 0.0;
+}
 
 bool is_unary(uint32_t c, bool& fp_only)
 {
@@ -268,6 +281,7 @@ struct Group {
 
 struct TraceProgram {
     std::vector<chai_trace_var> vars;
+    uint32_t nuser = 0;               // variables with caller buffers (the rest are versions)
     std::vector<Stmt> stmts;
     std::vector<Group> groups;
     std::vector<bool> needs_mem;      // TEMP variables that must be stored
@@ -332,6 +346,11 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
         } else if (is_index(n.code)) {
             e.prec = prec_of_suffix((int)((n.code - 42) % 4));
             e.d.W = n.W; e.d.H = n.H ? n.H : 1; e.d.S = n.slots ? n.slots : 1;
+        } else if (is_litdata(n.code)) {
+            double v;
+            litdata(n.code, e.prec, v);
+            e.d.W = n.W; e.d.H = n.H ? n.H : 1;
+            if (n.W == 0) return fail("ones/zeros leaf with zero width");
         } else if (is_convert(n.code)) {
             arity = 1;
             e.prec = prec_of_suffix((int)n.code - 1);
@@ -542,13 +561,16 @@ struct Emitter {
             G.prog.push_back(I);
             return tgt;
         }
-        if (is_scalar(e.code)) {
+        if (is_scalar(e.code) || is_litdata(e.code)) {
+            // a constant: its dims only shape the statement, every element is equal
+            double val = src.s0;
+            if (is_litdata(e.code)) { int p; litdata(e.code, p, val); }
             VmIns I = ins(VM_IMM, e.prec, tgt);
             switch (e.prec) {
-                case CHAI_U32: I.u = (uint32_t)(int64_t)src.s0; break;
-                case CHAI_I32: I.u = (uint32_t)(int32_t)(int64_t)src.s0; break;
-                case CHAI_F32: { float f = (float)src.s0; uint32_t b; std::memcpy(&b, &f, 4); I.u = b; break; }
-                default: std::memcpy(&I.u, &src.s0, 8); break;
+                case CHAI_U32: I.u = (uint32_t)(int64_t)val; break;
+                case CHAI_I32: I.u = (uint32_t)(int32_t)(int64_t)val; break;
+                case CHAI_F32: { float f = (float)val; uint32_t b; std::memcpy(&b, &f, 4); I.u = b; break; }
+                default: std::memcpy(&I.u, &val, 8); break;
             }
             G.prog.push_back(I);
             return tgt;
@@ -665,6 +687,64 @@ struct Emitter {
 };
 
 }  // namespace
+
+// Reassignment.  Chai arrays are mutable ("modl += gather2_floor(...)" in a
+// loop, sample/kirch.cpp), so a trace may assign an OUTPUT or TEMP variable
+// several times and every read sees the latest assignment before it.  Each
+// assignment but the last gets a fresh TEMP variable (a version, appended
+// after the caller's variables); the last writes the variable itself, whose
+// buffer is therefore never read before it is written.  Reading such a
+// variable before its first assignment is refused.
+struct Versioned {
+    std::vector<chai_trace_var> vars;
+    std::vector<chai_trace_stmt> stmts;
+    std::vector<chai_trace_node> nodes;
+    chai_trace t;
+};
+
+static int version_trace(const chai_trace& in, Versioned& V, const chai_trace*& out)
+{
+    out = &in;
+    std::vector<uint32_t> nassign(in.num_vars, 0);
+    bool multi = false;
+    for (uint32_t i = 0; i < in.num_stmts; i++) {
+        const uint32_t v = in.stmts[i].lhs;
+        if (v >= in.num_vars) return fail("statement assigns variable %u of %u", v, in.num_vars);
+        if (++nassign[v] > 1) multi = true;
+    }
+    if (!multi) return CHAI_OK;
+    V.vars.assign(in.vars, in.vars + in.num_vars);
+    V.stmts.assign(in.stmts, in.stmts + in.num_stmts);
+    V.nodes.assign(in.nodes, in.nodes + in.num_nodes);
+    std::vector<uint32_t> cur(in.num_vars), done(in.num_vars, 0);
+    for (uint32_t v = 0; v < in.num_vars; v++) cur[v] = v;
+    for (auto& st : V.stmts) {
+        if ((uint64_t)st.first + st.count > in.num_nodes) return fail("bad node range");
+        for (uint32_t k = 0; k < st.count; k++) {
+            chai_trace_node& n = V.nodes[st.first + k];
+            if (n.code != CHAI_TRACE_VAR) continue;
+            if (n.arg >= in.num_vars) return fail("read of variable %u of %u", n.arg, in.num_vars);
+            if (nassign[n.arg] > 1 && !done[n.arg])
+                return fail("variable %u is read before its first assignment", n.arg);
+            n.arg = cur[n.arg];
+        }
+        const uint32_t v = st.lhs;
+        if (nassign[v] > 1 && ++done[v] < nassign[v]) {
+            chai_trace_var tv = in.vars[v];
+            tv.role = CHAI_VAR_TEMP;
+            cur[v] = (uint32_t)V.vars.size();
+            V.vars.push_back(tv);
+            st.lhs = cur[v];
+        } else {
+            cur[v] = v;
+        }
+    }
+    V.t.vars = V.vars.data(); V.t.num_vars = (uint32_t)V.vars.size();
+    V.t.stmts = V.stmts.data(); V.t.num_stmts = (uint32_t)V.stmts.size();
+    V.t.nodes = V.nodes.data(); V.t.num_nodes = (uint32_t)V.nodes.size();
+    out = &V.t;
+    return CHAI_OK;
+}
 
 // Statement-to-kernel grouping and VM code generation.
 static int compile_trace(const chai_trace& t, TraceProgram& P)
@@ -1105,7 +1185,7 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
     std::vector<void*> bufs(nv, nullptr);
     std::vector<void*> owned;
     for (uint32_t v = 0; v < nv; v++) {
-        bufs[v] = user ? user[v] : nullptr;
+        bufs[v] = (user && v < P.nuser) ? user[v] : nullptr;
         if (P.vars[v].role != CHAI_VAR_TEMP && !bufs[v]) return fail("variable %u has no buffer", v);
         if (P.vars[v].role == CHAI_VAR_TEMP && P.needs_mem[v] && !bufs[v]) {
             const size_t bytes = prec_size((int)P.vars[v].prec) * P.vars[v].W * P.vars[v].H * P.vars[v].slots;
@@ -1223,7 +1303,11 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
                 }
     }
     auto prog = std::make_shared<TraceProgram>();
-    int rc = compile_trace(*trace, *prog);
+    prog->nuser = trace->num_vars;
+    Versioned ver;
+    const chai_trace* tv = nullptr;
+    int rc = version_trace(*trace, ver, tv);
+    if (rc == CHAI_OK) rc = compile_trace(*tv, *prog);
     if (rc == CHAI_OK) rc = materialize(*prog);
     if (rc) return rc;
     {
@@ -1285,7 +1369,10 @@ extern "C" int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, c
 {
     if (!trace) { set_error("trace_emit_source: null trace"); return CHAI_ERR_ARG; }
     TraceProgram P;
-    const int rc = compile_trace(*trace, P);
+    Versioned ver;
+    const chai_trace* tv = nullptr;
+    int rc = version_trace(*trace, ver, tv);
+    if (rc == CHAI_OK) rc = compile_trace(*tv, P);
     if (rc) return rc;
     if (kernel < 0 || kernel >= (int)P.groups.size() || P.groups[kernel].kind == 2) {
         set_error("trace_emit_source: kernel %d is not a fused VM kernel", kernel);

@@ -145,6 +145,8 @@ def smoothstep(e0, e1, x): return Expr(307, [lit(e0), lit(e1), lit(x)])
 def mad24(a, b, c): return Expr(406, [lit(a), lit(b), lit(c)])
 def mad_hi(a, b, c): return Expr(407, [lit(a), lit(b), lit(c)])
 def mad_sat(a, b, c): return Expr(408, [lit(a), lit(b), lit(c)])
+
+
 def _gather_data(a):
     if not isinstance(a, Expr) or a.code != VAR:
         raise TypeError("gather data must be a trace variable (assign the expression to a temp first)")
@@ -232,6 +234,20 @@ class Trace:
     def index(prec, dim, W, H=1, slots=1):
         code = (42 if H == 1 else 46) + {U32: 0, I32: 1, F32: 2, F64: 3}[prec]
         return Expr(code, arg=dim, W=W, H=H, slots=slots)
+
+    @staticmethod
+    def ones(prec, W, H=1):
+        """ones1_* / ones2_* (bytecode/ByteCodes.hpp:90-97).  Note that the
+        reference declares zeros2_f64 with the code of ones2_i32 (73), so
+        ones(I32, W, H > 1) is an f64 zero array, as in the reference."""
+        return Expr((68 if H == 1 else 72) + {U32: 0, I32: 1, F32: 2, F64: 3}[prec], W=W, H=H)
+
+    @staticmethod
+    def zeros(prec, W, H=1):
+        """zeros1_* / zeros2_* (bytecode/ByteCodes.hpp:98-105; zeros2_f64 is 73)"""
+        if H > 1 and prec == F64:
+            return Expr(73, W=W, H=H)
+        return Expr((76 if H == 1 else 80) + {U32: 0, I32: 1, F32: 2, F64: 3}[prec], W=W, H=H)
 
     def to_c(self):
         """ctypes chai_trace (arrays kept alive on the returned object)."""

@@ -297,7 +297,10 @@ int chai_hip_gemm_predict_variant(const chai_gemm_desc* desc, void* mlp, int* va
  * gather2_floor (bytecodes 40 / 41, node order data, x subscript[, y
  * subscript]) read a variable at computed subscripts with the interpreter's
  * wrap (interp/InterpGather.hpp:20-52), and a statement gathering from a
- * variable the current kernel computes starts a new kernel.  A trace the
+ * variable the current kernel computes starts a new kernel.  An OUTPUT or
+ * TEMP variable may be assigned by several statements (Chai arrays are
+ * mutable); each read sees the latest assignment before it, and reading such
+ * a variable before its first assignment is refused.  A trace the
  * compiler cannot place on the GPU returns CHAI_ERR_ARG, so the caller falls
  * back to the interpreter (runtime/Scheduler.cpp:221-254). */
 enum { CHAI_TRACE_VAR = 0x10000 };

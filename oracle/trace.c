@@ -484,6 +484,17 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                 if (chai_or_rng_fill((int)n->arg, r.prec, (int)n->vec_len, n->seed, n->W, H, S, normal,
                                      lim, n->s0, n->s1, r.d))
                     rc = -1;
+            } else if (c >= 68 && c <= 82) {
+                /* ones / zeros (runtime/Translator.cpp:116-131, jitast/AstLitdata.cpp):
+                 * the translator's typed values; 73 is declared for both ones2_i32
+                 * and zeros2_f64 (bytecode/ByteCodes.hpp:95,105) and the dispatch
+                 * table keeps the later registration, zeros2_f64 */
+                static const int lp[] = {0, 1, 4, 8};
+                const int p = c == 73 ? 8 : lp[(c - 68) % 4];
+                const double v = (c == 73 || c >= 76) ? 0.0 : 1.0;
+                const size_t H = n->H ? n->H : 1;
+                r = arr_new(p, n->W, H, 1);
+                for (size_t i = 0; i < n->W * H; i++) put(&r, i, (uint32_t)v, v);
             } else if (c >= 42 && c <= 49) {
                 static const int ip[] = {0, 1, 4, 8};
                 const size_t H = n->H ? n->H : 1, S = n->slots ? n->slots : 1;

@@ -13,7 +13,7 @@ import torch
 
 from chai_amd import F32, F64, I32, U32
 from chai_amd.trace import (Trace, exp, f32, log, maxe, matmul, mean, sin, sqrt, sum_, tanh,
-                            u32, cos, Expr)
+                            u32, cos, Expr, gather1_floor, gather2_floor)
 from tests.trace_cases import (MSGS, bcast_trace, md5_expected, md5_hex, md5_inputs, md5_trace,
                                mc_trace, random_trace)
 
@@ -339,3 +339,60 @@ def test_gather_broadcast_reduction_and_second_kernel(chai, oracle, engine):
         assert np.array_equal(r[o.index], ref[o.index])
         assert np.array_equal(r[o.index].ravel(), e.ravel())
     assert prog.info()[0] >= 3
+
+
+# --------------------------------------------------- reassignment, ones / zeros
+@pytest.mark.parametrize("NT,N", [(12, 12), (1024, 64)])
+def test_kirchhoff_reassignment_bit_exact(chai, oracle, engine, NT, N):
+    """sample/kirch.cpp: modl assigned N + 1 times (versions in registers),
+    N gather2_floor reads, one kernel; bit-exact against the oracle and the
+    numpy restatement"""
+    from tests.trace_cases import kirchhoff_expected, kirchhoff_trace
+    t, data, modl, consts = kirchhoff_trace(NT, N)
+    d = np.array([[1.0 / (1 + i * j) for j in range(NT)] for i in range(N)], dtype=np.float32)
+    r, prog = run_gpu(t, {data.index: d})
+    assert np.array_equal(r[modl.index], oracle.trace_eval(t, {data.index: d})[modl.index])
+    assert np.array_equal(r[modl.index][0], kirchhoff_expected(d, NT, N, consts))
+    assert prog.info()[0] == 1
+
+
+def test_reassigned_temp_gathered_across_kernels(chai, oracle, engine):
+    """a version of a reassigned temp that a later statement gathers from is
+    stored and the gathering statement starts a new kernel; the output is
+    assigned twice and read between"""
+    t = Trace()
+    a = t.input(F32, 64, 2)
+    s = t.temp(F32, 64, 2)
+    o = t.output(F32, 64, 2)
+    t.assign(s, a * f32(2.0))
+    t.assign(o, s + f32(1.0))
+    t.assign(s, gather1_floor(s, Trace.index(I32, 0, 64, 2) + 1) + s)
+    t.assign(s, s * f32(3.0))
+    t.assign(o, o + gather2_floor(s, Trace.index(F32, 0, 64, 2) * f32(0.5), Trace.index(I32, 1, 64, 2) + 1))
+    A = np.random.default_rng(5).standard_normal((1, 2, 64)).astype(np.float32)
+    r, prog = run_gpu(t, {a.index: A})
+    assert np.array_equal(r[o.index], oracle.trace_eval(t, {a.index: A})[o.index])
+    s1 = A[0] * np.float32(2)
+    s2 = (np.roll(s1, -1, axis=1) + s1) * np.float32(3)
+    expect = (s1 + np.float32(1)) + s2[::-1][:, np.arange(64) // 2]
+    assert np.array_equal(r[o.index][0], expect)
+    assert prog.info()[0] == 3
+
+
+def test_ones_zeros_leaves_bit_exact(chai, oracle, engine):
+    t = Trace()
+    outs = [t.assign(t.output(p, 5, 3), Trace.ones(p, 5, 3) + Trace.ones(p, 5) + Trace.zeros(p, 5))
+            for p in (U32, I32, F32, F64)]
+    r, _ = run_gpu(t, {})
+    ref = oracle.trace_eval(t, {})
+    for o in outs:
+        assert np.array_equal(r[o.index], ref[o.index])
+
+
+def test_read_before_first_assignment_is_rejected(chai):
+    t = Trace()
+    o = t.output(F32, 8)
+    t.assign(o, o + f32(1.0))
+    t.assign(o, o * f32(2.0))
+    with pytest.raises(Exception):
+        t.compile()

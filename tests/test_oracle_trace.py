@@ -200,3 +200,45 @@ def test_gather_data_must_be_a_variable():
     a = t.input(F32, 8)
     with pytest.raises(TypeError):
         gather1_floor(a + 1, a)
+
+
+# --------------------------------------------------- reassignment, ones / zeros
+@pytest.mark.parametrize("NT,N", [(12, 12), (20, 7)])
+def test_kirchhoff_sample_reassigns_and_gathers(oracle, NT, N):
+    """sample/kirch.cpp: modl assigned N + 1 times, each read seeing the
+    previous assignment; equals the numpy restatement bit for bit"""
+    from tests.trace_cases import kirchhoff_expected, kirchhoff_trace
+    t, data, modl, consts = kirchhoff_trace(NT, N)
+    d = np.array([[1.0 / (1 + i * j) for j in range(NT)] for i in range(N)], dtype=np.float32)
+    r = oracle.trace_eval(t, {data.index: d})
+    assert np.array_equal(r[modl.index][0], kirchhoff_expected(d, NT, N, consts))
+
+
+def test_ones_zeros_leaves(oracle):
+    """ones / zeros leaves in each type; code 73 (declared for both ones2_i32
+    and zeros2_f64) is an f64 zero, as the reference's dispatch table keeps
+    the later registration"""
+    from chai_amd.trace import Expr
+    t = Trace()
+    outs = [t.assign(t.output(p, 5, 3), Trace.ones(p, 5, 3) + Trace.ones(p, 5)) for p in (U32, I32, F32, F64)]
+    z = t.assign(t.output(F64, 5, 3), Trace.zeros(F32, 5, 3) + Expr(73, W=5, H=3) + f32(2.5))
+    r = oracle.trace_eval(t, {})
+    for o, p in zip(outs, (U32, I32, F32, F64)):
+        # ones(I32, 5, 3) is code 73, an f64 zero: 0 + 1 for that row
+        assert (r[o.index] == (1 if p == I32 else 2)).all(), p
+    assert (r[z.index] == 2.5).all()
+    assert Trace.ones(I32, 5, 3).code == 73 and Trace.zeros(F64, 5, 3).code == 73
+
+
+def test_reassignment_reads_the_latest_value(oracle):
+    t = Trace()
+    a = t.input(F32, 16)
+    s = t.temp(F32, 16)
+    o = t.output(F32, 16)
+    t.assign(s, a * f32(2.0))
+    t.assign(o, s + f32(1.0))
+    t.assign(s, s * s)
+    t.assign(o, o + s)
+    A = np.arange(16, dtype=np.float32)
+    r = oracle.trace_eval(t, {a.index: A})
+    assert np.array_equal(r[o.index].ravel(), (A * 2 + 1) + (A * 2) * (A * 2))

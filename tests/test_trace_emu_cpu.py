@@ -111,6 +111,16 @@ def test_emitted_gather_under_broadcast_matches_oracle(tmp_path):
     np.testing.assert_array_equal(got[outs[0].index], ref[outs[0].index])
 
 
+def test_emitted_kirchhoff_reassignment_matches_oracle(tmp_path):
+    """sample/kirch.cpp: thirteen assignments of modl fused into one kernel"""
+    from tests.trace_cases import kirchhoff_expected, kirchhoff_trace
+    t, data, modl, consts = kirchhoff_trace(12, 12)
+    d = np.array([[1.0 / (1 + i * j) for j in range(12)] for i in range(12)], dtype=np.float32)
+    got, ref = emulate(t, {data.index: d}, tmp_path)
+    np.testing.assert_array_equal(got[modl.index], ref[modl.index])
+    np.testing.assert_array_equal(got[modl.index][0], kirchhoff_expected(d, 12, 12, consts))
+
+
 def test_emitted_md5_matches_hashlib(tmp_path):
     t, M, outs = md5_trace(len(MSGS))
     data = {m.index: w for m, w in zip(M, md5_inputs(MSGS))}

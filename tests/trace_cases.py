@@ -299,3 +299,41 @@ def gather_bcast_trace(seed=0):
             xf = np.float32(x) * np.float32(-1.5)
             e2[:, y, x] = T[:, wrap(y - 1, 3), subscript_closed_form(xf, F32, 5)]
     return t, {a.index: A, i.index: I, b.index: B}, (o1, s1, o2), (e1, e_s1, e2)
+
+
+def kirchhoff_trace(NT, N, NNN=12):
+    """The reference's sample/kirch.cpp (Kirchhoff migration): modl starts as
+    zeros_f32 and accumulates N gather2_floor reads of data at (it, index1),
+    so modl is assigned N + 1 times in one trace.  Literal types follow the
+    C++: float variables are f32 literals, 0.5 is a double."""
+    f = np.float32
+    dx, dt = f(f(NNN) / f(N)), f(f(NNN) / f(NT))
+    factor, idt = f(1.0 / (f(1) * f(1))), f(1.0 / float(dt))
+    t = Trace()
+    data = t.input(F32, NT, N)
+    modl = t.output(F32, NT, N)
+    t.assign(modl, Trace.zeros(F32, NT, N))
+    x = t.assign(t.temp(F32, NT, N), f32(dx) * Trace.index(F32, 1, NT, N))
+    z = t.assign(t.temp(F32, NT, N), f32(dt) * Trace.index(F32, 0, NT, N))
+    for iy in range(N):
+        y = f(f(iy) * dx)
+        index1 = f32(float(iy)) * Trace.ones(F32, NT, N)
+        it = 0.5 + sqrt(z * z + (x - f32(y)) * (x - f32(y)) * f32(factor)) * f32(idt)
+        t.assign(modl, modl + gather2_floor(data, it, index1))
+    return t, data, modl, (dx, dt, factor, idt)
+
+
+def kirchhoff_expected(d, NT, N, consts):
+    """numpy restatement of sample/kirch.cpp on data [N][NT] (H = N rows of W = NT)"""
+    f = np.float32
+    dx, dt, factor, idt = consts
+    yy, xx = np.meshgrid(np.arange(N), np.arange(NT), indexing="ij")
+    X, Z = f(dx) * yy.astype(f), f(dt) * xx.astype(f)
+    modl = np.zeros((N, NT), dtype=f)
+    for iy in range(N):
+        y = f(f(iy) * dx)
+        s = Z * Z + (X - y) * (X - y) * factor
+        it = 0.5 + (np.sqrt(s) * idt).astype(np.float64)
+        xs = np.array([subscript_closed_form(v, F64, NT) for v in it.ravel()]).reshape(N, NT)
+        modl = modl + d[iy % N, xs]
+    return modl
