@@ -241,3 +241,44 @@ def test_bf16x3_rank1_matches_split_model(chai, oracle):
     dev = np.abs(C.astype(np.float64) - model.astype(np.float64)) / ulp
     print("max deviation from the split model: %.2f ulp" % dev.max())
     assert dev.max() <= 1.0
+
+
+def test_tuner_journal_crash_resume(chai, tmp_path):
+    """Crash-resume through the journal (autotune/AutoTuneMemo.cpp:101-188):
+    a tuning run journals every variant; a run that died while compiling the
+    winner leaves PRINT_SOURCE, COMPILE_KERNEL as that kernel's last lines.
+    Reloading such a journal blacklists the winner, reuses every other
+    variant's timing without running it, and the search picks the best of
+    the rest."""
+    import torch
+    n = 512
+    A, B = torch.rand(n, n, device="cuda"), torch.rand(n, n, device="cuda")
+    C = torch.empty(n, n, device="cuda")
+    first = tmp_path / "journal0.AMD"
+    chai.memo_close()
+    chai.memo_open(first, 0)
+    v1, _ = chai.gemm_tune(A, B, C, trials=1)
+    chai.memo_close()
+    lines = [l.split() for l in first.read_text().split("\n") if l.strip()]
+    nv = len(chai.gemm_variants())
+    by_v = {v: [l for l in lines if l[1].endswith("_v%d" % v)] for v in range(nv)}
+    assert all(by_v[v] for v in range(nv))
+    times = {v: int(by_v[v][-1][2]) for v in range(nv)}
+    assert all(t > 0 for t in times.values())
+    crashed = tmp_path / "journal1.AMD"
+    with open(crashed, "w") as f:
+        for v in range(nv):
+            rows = by_v[v] if v != v1 else [r for r in by_v[v] if r[2] in ("-1", "-2")]
+            for r in rows:
+                f.write(" ".join(r) + "\n")
+    before = crashed.read_text()
+    chai.memo_open(crashed, 0)
+    try:
+        name = by_v[v1][0][1]
+        assert chai.memo_status(name)[0] == -1
+        v2, ms2 = chai.gemm_tune(A, B, C, trials=1)
+        rest = {v: t for v, t in times.items() if v != v1}
+        assert v2 == min(rest, key=rest.get) and abs(ms2 * 1000.0 - rest[v2]) <= 1.0
+        assert crashed.read_text() == before          # memo hits only: nothing re-timed
+    finally:
+        chai.memo_close()
