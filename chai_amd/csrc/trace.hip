@@ -279,7 +279,26 @@ struct Group {
 
 }  // namespace
 
+// a trace held by the library: the caller's copy, or a rewritten one
+struct Versioned {
+    std::vector<chai_trace_var> vars;
+    std::vector<chai_trace_stmt> stmts;
+    std::vector<chai_trace_node> nodes;
+    chai_trace t;
+    void adopt(const chai_trace& in)
+    {
+        vars.assign(in.vars, in.vars + in.num_vars);
+        stmts.assign(in.stmts, in.stmts + in.num_stmts);
+        nodes.assign(in.nodes, in.nodes + in.num_nodes);
+        t.vars = vars.data(); t.num_vars = (uint32_t)vars.size();
+        t.stmts = stmts.data(); t.num_stmts = (uint32_t)stmts.size();
+        t.nodes = nodes.data(); t.num_nodes = (uint32_t)nodes.size();
+    }
+};
+
 struct TraceProgram {
+    // the program's own copies of the trace (statements point into them)
+    Versioned own, spl, ver;
     std::vector<chai_trace_var> vars;
     uint32_t nuser = 0;               // variables with caller buffers (the rest are versions)
     std::vector<Stmt> stmts;
@@ -317,14 +336,40 @@ Dims dmax(const Dims& a, const Dims& b)
     return r;
 }
 
-int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
+// result shape of matmul(A, B) with the operands' transpose flags, the
+// oracle's dispatch: matrix * vector, vector * matrix, matrix * matrix
+Dims mm_dims(const Dims& a, const Dims& b, bool ta, bool tb)
 {
-    s.lhs = cs.lhs;
-    if (cs.lhs >= t.num_vars) return fail("statement assigns variable %u of %u", cs.lhs, t.num_vars);
-    if ((uint64_t)cs.first + cs.count > t.num_nodes || cs.count == 0) return fail("bad node range");
+    Dims r;
+    if (b.H == 1 && !tb && a.H > 1) {
+        r.W = ta ? a.W : a.H;
+    } else if (a.H == 1 && !ta && b.H > 1) {
+        r.W = tb ? b.H : b.W;
+    } else {
+        r.W = tb ? b.H : b.W;
+        r.H = ta ? a.W : a.H;
+    }
+    return r;
+}
+
+// operand count of a node, -1 for a code the GPU back end does not know
+int node_arity(uint32_t c)
+{
+    bool fp_only;
+    int bkind;
+    if (c == CHAI_TRACE_VAR || is_scalar(c) || is_rng(c) || is_index(c) || is_litdata(c)) return 0;
+    if (is_convert(c) || c == 131 || c == 132 || is_unary(c, fp_only)) return 1;
+    if (c == 130 || c == 120 || c == 40 || is_binary(c, bkind)) return 2;
+    if (c == 121 || c == 41 || is_ternary(c)) return 3;
+    return -1;
+}
+
+// types and shapes of one postfix expression (no checks against a variable)
+int parse_nodes(const chai_trace& t, const chai_trace_node* nodes, uint32_t count, Stmt& s)
+{
     std::vector<int> stack;
-    for (uint32_t i = 0; i < cs.count; i++) {
-        const chai_trace_node& n = t.nodes[cs.first + i];
+    for (uint32_t i = 0; i < count; i++) {
+        const chai_trace_node& n = nodes[i];
         ENode e;
         e.code = n.code;
         e.src = &n;
@@ -402,7 +447,7 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
             const ENode& b = s.nodes[e.kids[1]];
             e.prec = a.prec > b.prec ? a.prec : b.prec;
             if (n.code == 120) {
-                e.d = Dims();   // fixed below
+                e.d = mm_dims(a.d, b.d, a.src->flags & 1, b.src->flags & 1);
             } else {
                 e.d = dmax(a.d, b.d);
                 if (bkind == 1 && !is_fp(e.prec)) return fail("bytecode %u needs floating point", n.code);
@@ -434,6 +479,16 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
     }
     if (stack.size() != 1) return fail("statement leaves %zu values", stack.size());
     s.root = stack[0];
+    return CHAI_OK;
+}
+
+int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
+{
+    s.lhs = cs.lhs;
+    if (cs.lhs >= t.num_vars) return fail("statement assigns variable %u of %u", cs.lhs, t.num_vars);
+    if ((uint64_t)cs.first + cs.count > t.num_nodes || cs.count == 0) return fail("bad node range");
+    const int rc = parse_nodes(t, t.nodes + cs.first, cs.count, s);
+    if (rc) return rc;
     const ENode& r = s.nodes[s.root];
     s.kind = is_red(r.code) ? 1 : is_mm(r.code) ? 2 : 0;
     // reductions / matmul only at the root
@@ -695,12 +750,101 @@ struct Emitter {
 // after the caller's variables); the last writes the variable itself, whose
 // buffer is therefore never read before it is written.  Reading such a
 // variable before its first assignment is refused.
-struct Versioned {
-    std::vector<chai_trace_var> vars;
-    std::vector<chai_trace_stmt> stmts;
-    std::vector<chai_trace_node> nodes;
-    chai_trace t;
-};
+
+// Statement splitting (jitstmt/MakeStmt.cpp:579-726).  A reduction or matmul
+// below the root of a statement, a matmul operand that is not a variable and
+// gather data that is not a variable each become a statement of their own,
+// placed before the statement and assigning a fresh TEMP that the statement
+// then reads.  Inner expressions split first, so nested reductions work.
+static int split_trace(const chai_trace& in, Versioned& V, const chai_trace*& out)
+{
+    out = &in;
+    bool need = false;
+    for (uint32_t i = 0; i < in.num_stmts && !need; i++) {
+        const chai_trace_stmt& st = in.stmts[i];
+        if ((uint64_t)st.first + st.count > in.num_nodes || st.count == 0) return fail("bad node range");
+        for (uint32_t k = 0; k < st.count; k++) {
+            const uint32_t c = in.nodes[st.first + k].code;
+            if (((is_red(c) || is_mm(c)) && k + 1 < st.count) || is_mm(c) || is_gather(c)) need = true;
+        }
+    }
+    if (!need) return CHAI_OK;
+    V.vars.assign(in.vars, in.vars + in.num_vars);
+    // move cur[b, e) into a statement of its own; returns the VAR node reading its result
+    auto hoist = [&](std::vector<chai_trace_node>& cur, size_t b, size_t e, chai_trace_node& var) -> int {
+        std::vector<chai_trace_node> sub(cur.begin() + b, cur.begin() + e);
+        chai_trace view;
+        std::memset(&view, 0, sizeof view);
+        view.vars = V.vars.data();
+        view.num_vars = (uint32_t)V.vars.size();
+        Stmt ts;
+        const int rc = parse_nodes(view, sub.data(), (uint32_t)sub.size(), ts);
+        if (rc) return rc;
+        const ENode& r = ts.nodes[ts.root];
+        chai_trace_var tv;
+        tv.prec = (uint32_t)r.prec;
+        tv.role = CHAI_VAR_TEMP;
+        tv.W = r.d.W; tv.H = r.d.H; tv.slots = r.d.S;
+        const uint32_t v = (uint32_t)V.vars.size();
+        V.vars.push_back(tv);
+        chai_trace_stmt ns;
+        ns.lhs = v;
+        ns.first = (uint32_t)V.nodes.size();
+        ns.count = (uint32_t)sub.size();
+        ns.pad = 0;
+        V.nodes.insert(V.nodes.end(), sub.begin(), sub.end());
+        V.stmts.push_back(ns);
+        std::memset(&var, 0, sizeof var);
+        var.code = CHAI_TRACE_VAR;
+        var.arg = v;
+        var.flags = sub.back().flags & 1u;        // a transposed operand stays transposed
+        cur.erase(cur.begin() + b, cur.begin() + e);
+        cur.insert(cur.begin() + b, var);
+        return CHAI_OK;
+    };
+    for (uint32_t i = 0; i < in.num_stmts; i++) {
+        const chai_trace_stmt& st = in.stmts[i];
+        std::vector<chai_trace_node> cur;
+        std::vector<size_t> starts;                // subtree start in cur, one per stack entry
+        for (uint32_t k = 0; k < st.count; k++) {
+            const chai_trace_node& n = in.nodes[st.first + k];
+            const int ar = node_arity(n.code);
+            if (ar < 0) return fail("bytecode %u is not supported by the GPU back end", n.code);
+            if ((int)starts.size() < ar) return fail("stack underflow at bytecode %u", n.code);
+            const size_t base = starts.size() - (size_t)ar;
+            // operands that must be variables: all of a matmul's, a gather's data
+            const int nv = is_mm(n.code) ? ar : is_gather(n.code) ? 1 : 0;
+            for (int j = nv - 1; j >= 0; j--) {
+                const size_t b = starts[base + j];
+                const size_t e = (size_t)j + 1 < (size_t)ar ? starts[base + j + 1] : cur.size();
+                if (e - b == 1 && cur[b].code == CHAI_TRACE_VAR) continue;
+                chai_trace_node var;
+                const int rc = hoist(cur, b, e, var);
+                if (rc) return rc;
+                for (size_t q = base + j + 1; q < starts.size(); q++) starts[q] -= e - b - 1;
+            }
+            const size_t start = ar ? starts[base] : cur.size();
+            starts.resize(base);
+            cur.push_back(n);
+            starts.push_back(start);
+            if ((is_red(n.code) || is_mm(n.code)) && k + 1 < st.count) {
+                chai_trace_node var;
+                const int rc = hoist(cur, start, cur.size(), var);
+                if (rc) return rc;
+            }
+        }
+        chai_trace_stmt ns = st;
+        ns.first = (uint32_t)V.nodes.size();
+        ns.count = (uint32_t)cur.size();
+        V.nodes.insert(V.nodes.end(), cur.begin(), cur.end());
+        V.stmts.push_back(ns);
+    }
+    V.t.vars = V.vars.data(); V.t.num_vars = (uint32_t)V.vars.size();
+    V.t.stmts = V.stmts.data(); V.t.num_stmts = (uint32_t)V.stmts.size();
+    V.t.nodes = V.nodes.data(); V.t.num_nodes = (uint32_t)V.nodes.size();
+    out = &V.t;
+    return CHAI_OK;
+}
 
 static int version_trace(const chai_trace& in, Versioned& V, const chai_trace*& out)
 {
@@ -1304,9 +1448,10 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
     }
     auto prog = std::make_shared<TraceProgram>();
     prog->nuser = trace->num_vars;
-    Versioned ver;
-    const chai_trace* tv = nullptr;
-    int rc = version_trace(*trace, ver, tv);
+    prog->own.adopt(*trace);
+    const chai_trace *ts = nullptr, *tv = nullptr;
+    int rc = split_trace(prog->own.t, prog->spl, ts);
+    if (rc == CHAI_OK) rc = version_trace(*ts, prog->ver, tv);
     if (rc == CHAI_OK) rc = compile_trace(*tv, *prog);
     if (rc == CHAI_OK) rc = materialize(*prog);
     if (rc) return rc;
@@ -1369,9 +1514,10 @@ extern "C" int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, c
 {
     if (!trace) { set_error("trace_emit_source: null trace"); return CHAI_ERR_ARG; }
     TraceProgram P;
-    Versioned ver;
-    const chai_trace* tv = nullptr;
-    int rc = version_trace(*trace, ver, tv);
+    Versioned spl, ver;
+    const chai_trace *ts = nullptr, *tv = nullptr;
+    int rc = split_trace(*trace, spl, ts);
+    if (rc == CHAI_OK) rc = version_trace(*ts, ver, tv);
     if (rc == CHAI_OK) rc = compile_trace(*tv, P);
     if (rc) return rc;
     if (kernel < 0 || kernel >= (int)P.groups.size() || P.groups[kernel].kind == 2) {

@@ -148,9 +148,9 @@ def mad_sat(a, b, c): return Expr(408, [lit(a), lit(b), lit(c)])
 
 
 def _gather_data(a):
-    if not isinstance(a, Expr) or a.code != VAR:
-        raise TypeError("gather data must be a trace variable (assign the expression to a temp first)")
-    return a
+    if not isinstance(a, Expr):
+        raise TypeError("gather data must be an array expression")
+    return a        # an expression is computed into a temp by the compiler first
 
 
 def gather1_floor(a, i):

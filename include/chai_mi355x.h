@@ -300,7 +300,11 @@ int chai_hip_gemm_predict_variant(const chai_gemm_desc* desc, void* mlp, int* va
  * variable the current kernel computes starts a new kernel.  An OUTPUT or
  * TEMP variable may be assigned by several statements (Chai arrays are
  * mutable); each read sees the latest assignment before it, and reading such
- * a variable before its first assignment is refused.  A trace the
+ * a variable before its first assignment is refused.  Reductions and matmuls
+ * below the root of a statement, matmul operands that are expressions and
+ * gather data that is an expression are split into statements of their own
+ * (jitstmt/MakeStmt.cpp:579-726).  The compiled handle keeps its own copy of
+ * the trace: the caller may free it after compiling.  A trace the
  * compiler cannot place on the GPU returns CHAI_ERR_ARG, so the caller falls
  * back to the interpreter (runtime/Scheduler.cpp:221-254). */
 enum { CHAI_TRACE_VAR = 0x10000 };

@@ -455,6 +455,7 @@ static size_t gather_wrap(int xf, size_t W)
 int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
 {
     Arr stk[MAXSTK];
+    uint32_t root[MAXSTK];     /* node index of each stack entry's root (matmul transpose flags) */
     int sp = 0;
     int rc = 0;
     for (uint32_t si = 0; si < t->num_stmts && !rc; si++) {
@@ -544,8 +545,8 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                 if (general) C0 = stk[--sp];
                 B = stk[--sp];
                 A = stk[--sp];
-                const chai_trace_node* na = &t->nodes[st->first + ni - (general ? 3 : 2)];
-                const chai_trace_node* nb = &t->nodes[st->first + ni - (general ? 2 : 1)];
+                const chai_trace_node* na = &t->nodes[root[sp]];
+                const chai_trace_node* nb = &t->nodes[root[sp + 1]];
                 const int ta = na->flags & 1, tb = nb->flags & 1;
                 const int p = A.prec > B.prec ? A.prec : B.prec;
                 if (B.H == 1 && !tb && A.H > 1) {          /* matrix * vector */
@@ -691,6 +692,7 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                         }
                 for (int j = 0; j < arity; j++) free(k[j].d);
             }
+            root[sp] = st->first + ni;
             stk[sp++] = r;
         }
         if (rc) break;

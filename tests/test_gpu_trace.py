@@ -181,9 +181,11 @@ def test_slots_and_broadcast(chai, oracle, engine):
 
 
 def test_unsupported_trace_is_rejected(chai):
+    """a bytecode the back end does not know fails compilation, so the caller
+    falls back to the interpreter (runtime/Scheduler.cpp:221-254)"""
     t = Trace()
     x = t.input(F32, 16)
-    t.assign(t.output(F32, 16), x - mean(x))      # reduction below the root: caller must split
+    t.assign(t.output(F32, 16), x - Expr(999, [x]))
     with pytest.raises(chai.ChaiError):
         t.compile()
 
@@ -396,3 +398,30 @@ def test_read_before_first_assignment_is_rejected(chai):
     t.assign(o, o * f32(2.0))
     with pytest.raises(Exception):
         t.compile()
+
+
+# ------------------------------------------------------------ statement splitting
+def test_loopsum_sample_bit_exact(chai, oracle, engine):
+    """sample/loopsum.cpp: mean(B) inside A = A + mean(B) is split into its own
+    statement; A and B reassigned; closed form A + 33, sum(A) + 3300"""
+    from tests.trace_cases import loopsum_trace
+    t, A, C, An = loopsum_trace()
+    a = np.arange(100, dtype=np.float32).reshape(1, 1, 100)
+    r, prog = run_gpu(t, {A.index: a})
+    ref = oracle.trace_eval(t, {A.index: a})
+    assert np.array_equal(r[An.index], ref[An.index]) and np.array_equal(r[C.index], ref[C.index])
+    assert np.array_equal(r[An.index], a + np.float32(33))
+    assert r[C.index].ravel()[0] == np.float32(a.sum() + 3300)
+
+
+def test_reduction_matmul_and_gather_data_below_the_root(chai, oracle, engine):
+    """mean / sum inside an expression, matmul of expressions (one transposed)
+    inside an expression, gather from an expression: split into statements
+    of their own; integer data keeps every result exact"""
+    from tests.trace_cases import split_trace_case
+    t, data, outs, expect = split_trace_case()
+    r, prog = run_gpu(t, data)
+    ref = oracle.trace_eval(t, data)
+    for o, e in zip(outs, expect):
+        assert np.array_equal(r[o.index], ref[o.index]), o.index
+        assert np.array_equal(r[o.index].reshape(e.shape), e), o.index

@@ -194,12 +194,12 @@ def test_gather_broadcast_reduction_and_second_kernel(oracle):
     assert np.array_equal(r[outs[2].index], expect[2])
 
 
-def test_gather_data_must_be_a_variable():
+def test_gather_data_must_be_an_array():
     from chai_amd.trace import gather1_floor
     t = Trace()
     a = t.input(F32, 8)
     with pytest.raises(TypeError):
-        gather1_floor(a + 1, a)
+        gather1_floor(2.0, a)
 
 
 # --------------------------------------------------- reassignment, ones / zeros
@@ -242,3 +242,26 @@ def test_reassignment_reads_the_latest_value(oracle):
     A = np.arange(16, dtype=np.float32)
     r = oracle.trace_eval(t, {a.index: A})
     assert np.array_equal(r[o.index].ravel(), (A * 2 + 1) + (A * 2) * (A * 2))
+
+
+
+# ------------------------------------------------------------ statement splitting
+def test_loopsum_sample_closed_form(oracle):
+    """sample/loopsum.cpp: A = A + mean(B) three times, B += 1, twice; C = sum(A).
+    A ends as A + 3*5 + 3*6 and C = sum(A) + 33 * 100"""
+    from tests.trace_cases import loopsum_trace
+    t, A, C, An = loopsum_trace()
+    a = np.arange(100, dtype=np.float32).reshape(1, 1, 100)
+    r = oracle.trace_eval(t, {A.index: a})
+    assert np.array_equal(r[An.index], a + np.float32(33))
+    assert r[C.index].ravel()[0] == np.float32(a.sum() + 33 * 100)
+
+
+def test_nested_reduction_and_matmul_in_expressions(oracle):
+    """a reduction and a matmul below the root of a statement, a matmul of
+    expressions with a transposed operand: closed forms on integer data"""
+    from tests.trace_cases import split_trace_case
+    t, data, outs, expect = split_trace_case()
+    r = oracle.trace_eval(t, data)
+    for o, e in zip(outs, expect):
+        assert np.array_equal(r[o.index].reshape(e.shape), e), o.index

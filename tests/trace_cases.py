@@ -8,7 +8,7 @@ import numpy as np
 from chai_amd import F32, F64, I32, U32
 from chai_amd.trace import (Trace, cond, exp, f32, fabs, fma, hadd, log, lor, maxe, mean, mine,
                             mul_hi, rotate, sqrt, sum_, u32, i32, add_sat, sub_sat, rhadd,
-                            abs_diff, mul24, clamp, dot, gather1_floor, gather2_floor)
+                            abs_diff, mul24, clamp, dot, gather1_floor, gather2_floor, matmul)
 
 # ---------------------------------------------------------------- MD5 (RFC 1321)
 MD5_S = [7, 12, 17, 22] * 4 + [5, 9, 14, 20] * 4 + [4, 11, 16, 23] * 4 + [6, 10, 15, 21] * 4
@@ -337,3 +337,45 @@ def kirchhoff_expected(d, NT, N, consts):
         xs = np.array([subscript_closed_form(v, F64, NT) for v in it.ravel()]).reshape(N, NT)
         modl = modl + d[iy % N, xs]
     return modl
+
+
+
+def loopsum_trace():
+    """sample/loopsum.cpp (one instance): B = 5 * ones_f32(1); twice { three
+    times A = A + mean(B); B += 1 }; C = sum(A).  mean sits inside the
+    statement; the compiler splits it out."""
+    t = Trace()
+    A = t.input(F32, 100)
+    An = t.output(F32, 100)
+    B = t.temp(F32, 1)
+    C = t.output(F32, 1)
+    t.assign(B, 5 * Trace.ones(F32, 1))
+    cur = A
+    for j in range(2):
+        for k in range(3):
+            cur = t.assign(An, cur + mean(B))
+        t.assign(B, B + 1)
+    t.assign(C, sum_(An))
+    return t, A, C, An
+
+
+def split_trace_case():
+    """statements with a reduction, a matmul and gather data below the root"""
+    rng = np.random.default_rng(11)
+    t = Trace()
+    a = t.input(F32, 32, 24)
+    b = t.input(F32, 24, 32)
+    c = t.input(F32, 24, 24)
+    o1 = t.assign(t.output(F32, 32, 24), a - mean(a) + sum_(a * f32(0.5)))
+    o2 = t.assign(t.output(F32, 24, 24), matmul(a * f32(2.0), b) + c)
+    o3 = t.assign(t.output(F32, 24, 24), matmul((b + f32(1.0)).T, a.T) - f32(3.0))
+    o4 = t.assign(t.output(F32, 32, 24), gather1_floor(a + f32(1.0), Trace.index(I32, 0, 32, 24) + 1))
+    A = rng.integers(-4, 5, size=(1, 24, 32)).astype(np.float32)
+    B = rng.integers(-4, 5, size=(1, 32, 24)).astype(np.float32)
+    Cc = rng.integers(-4, 5, size=(1, 24, 24)).astype(np.float32)
+    a2, b2 = A[0], B[0]
+    e1 = (a2 - np.float32(a2.astype(np.float64).mean())) + np.float32((a2 * np.float32(0.5)).astype(np.float64).sum())
+    e2 = (a2 * 2) @ b2 + Cc[0]
+    e3 = (b2 + 1).T @ a2.T - 3
+    e4 = np.roll(a2 + 1, -1, axis=1)
+    return t, {a.index: A, b.index: B, c.index: Cc}, (o1, o2, o3, o4), (e1, e2, e3, e4)
