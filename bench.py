@@ -44,7 +44,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="sgemm",
-                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed", "gather"])
+                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed", "gather", "errors"])
     p.add_argument("--size", type=int, default=8192)
     p.add_argument("--fp32-steps", type=int, default=3)
     p.add_argument("--cpu-size", type=int, default=1024,
@@ -694,6 +694,42 @@ def run_dense(args, mixed):
     emit(line)
 
 
+# ------------------------------------------------------------------ errors per shape
+def run_errors(args):
+    """Split-BF16x3 and plain FP32-FMA GEMM errors against the FP64 product, per shape.
+    M = N = 1024 (every element checked) and K from 256 to 16384, for operands uniform in
+    [-1, 1) (config inputs, seed 42) and in [0, 1) (no cancellation).  Per shape: max and rms
+    absolute error, the max of |C - C64| / sum_k |a b| (the quantity the written bounds cap),
+    and the bf16x3 / fp32 ratios."""
+    import numpy as np
+    import torch
+    import chai_amd as chai
+    chai.init(0)
+    m = n = 1024
+    rows = []
+    for lo in (-1.0, 0.0):
+        for k in (256, 1024, 2048, 4096, 8192, 16384):
+            A = torch.empty(m, k, device="cuda")
+            B = torch.empty(k, n, device="cuda")
+            chai.rng_fill(A, seed=42, limits=(lo, 1.0), W=k, H=m)
+            chai.rng_fill(B, seed=42 + m * k, limits=(lo, 1.0), W=n, H=k)
+            A64, B64 = A.cpu().numpy().astype(np.float64), B.cpu().numpy().astype(np.float64)
+            C64 = A64 @ B64
+            scale = np.abs(A64) @ np.abs(B64)
+            r = {"M": m, "N": n, "K": k, "inputs": "uniform[%g,1)" % lo}
+            for name, mode in (("bf16x3", chai.GEMM_BF16X3), ("fp32_fma", chai.GEMM_FMA)):
+                e = np.abs(chai.gemm(A, B, mode=mode).cpu().numpy().astype(np.float64) - C64)
+                r[name] = {"max_abs": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
+                           "max_rel_to_sum_abs": float((e / scale).max())}
+            r["ratio_max"] = r["bf16x3"]["max_abs"] / r["fp32_fma"]["max_abs"]
+            r["ratio_rms"] = r["bf16x3"]["rms"] / r["fp32_fma"]["rms"]
+            rows.append(r)
+            print("K=%5d %-16s bf16x3 max %.3e rms %.3e | fp32 max %.3e rms %.3e | ratio max %.2f rms %.2f"
+                  % (k, r["inputs"], r["bf16x3"]["max_abs"], r["bf16x3"]["rms"], r["fp32_fma"]["max_abs"],
+                     r["fp32_fma"]["rms"], r["ratio_max"], r["ratio_rms"]), flush=True)
+    print(json.dumps({"errors_vs_fp64_per_shape": rows}))
+
+
 def main():
     args = parse()
     if args.pmc_summary:
@@ -724,6 +760,8 @@ def main():
         run_autotune(args)
     elif args.workload == "mc":
         run_mc(args)
+    elif args.workload == "errors":
+        run_errors(args)
     elif args.workload == "gather":
         run_gather(args)
     else:

@@ -31,6 +31,9 @@
 #include <vector>
 
 namespace chai_hip {
+// program_cl.hip: launch of an inline OpenCL kernel by handle
+int cl_launch_kernel(void* kernel, const chai_cl_arg* args, uint32_t nargs, uint32_t dims, const uint64_t* global,
+                     const uint64_t* local, hipStream_t st);
 
 // ===================================================================== device
 
@@ -191,6 +194,12 @@ bool is_red(uint32_t c) { return c == 130 || c == 131 || c == 132; }
 bool is_mm(uint32_t c) { return c == 120 || c == 121; }
 bool is_gather(uint32_t c) { return c == 40 || c == 41; }
 bool is_litdata(uint32_t c) { return c >= 68 && c <= 82; }
+bool is_cl(uint32_t c) { return c == 50; }          // kernel_from_opencl
+// an inline OpenCL kernel statement: its last node is kernel_from_opencl
+bool is_cl_stmt(const chai_trace& t, const chai_trace_stmt& st)
+{
+    return st.count > 0 && (uint64_t)st.first + st.count <= t.num_nodes && is_cl(t.nodes[st.first + st.count - 1].code);
+}
 
 int prec_of_suffix(int k) { return k == 0 ? CHAI_U32 : k == 1 ? CHAI_I32 : k == 2 ? CHAI_F32 : CHAI_F64; }
 
@@ -482,11 +491,69 @@ int parse_nodes(const chai_trace& t, const chai_trace_node* nodes, uint32_t coun
     return CHAI_OK;
 }
 
+// An inline OpenCL kernel statement (api/ProgramCL.cpp:202-219): argument
+// nodes then the kernel_from_opencl root.  Arguments are checked against the
+// kernel's metadata here, so a bad call fails at compile time.
+int parse_cl_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
+{
+    const chai_trace_node& r = t.nodes[cs.first + cs.count - 1];
+    const uint32_t na = cs.count - 1;
+    if (r.arg != na) return fail("inline kernel statement: %u arguments declared, %u nodes", r.arg, na);
+    if (!r.seed) return fail("inline kernel statement without a kernel handle");
+    if (r.vec_len < 1 || r.vec_len > 2) return fail("inline kernel work space must be 1- or 2-dimensional");
+    std::vector<chai_cl_arg> args(na);
+    bool first_array = true;
+    ENode root;
+    root.code = r.code;
+    root.src = &r;
+    for (uint32_t k = 0; k < na; k++) {
+        const chai_trace_node& n = t.nodes[cs.first + k];
+        ENode e;
+        e.code = n.code;
+        e.src = &n;
+        chai_cl_arg& a = args[k];
+        std::memset(&a, 0, sizeof a);
+        if (n.code == CHAI_TRACE_VAR) {
+            if (n.arg >= t.num_vars) return fail("read of variable %u of %u", n.arg, t.num_vars);
+            const chai_trace_var& v = t.vars[n.arg];
+            e.prec = (int)v.prec;
+            e.d.W = v.W; e.d.H = v.H; e.d.S = v.slots;
+            a.kind = CHAI_CL_ARG_ARRAY;
+            a.prec = v.prec;
+            if (first_array && n.arg != cs.lhs)
+                return fail("an inline kernel statement assigns its first array argument");
+            first_array = false;
+        } else if (is_scalar(n.code)) {
+            e.prec = prec_of_suffix((int)n.code - 5);
+            a.kind = CHAI_CL_ARG_SCALAR;
+            a.prec = (uint32_t)e.prec;
+            if (e.prec == CHAI_U32) a.v.u = (uint32_t)(int64_t)n.s0;
+            else if (e.prec == CHAI_I32) a.v.i = (int32_t)n.s0;
+            else if (e.prec == CHAI_F32) a.v.f = (float)n.s0;
+            else a.v.d = n.s0;
+        } else if (n.code == CHAI_TRACE_CL_LOCAL) {
+            a.kind = CHAI_CL_ARG_LOCAL;
+            a.v.count = n.W;
+        } else {
+            return fail("bytecode %u cannot be an inline kernel argument", n.code);
+        }
+        root.kids.push_back((int)s.nodes.size());
+        s.nodes.push_back(e);
+    }
+    if (first_array) return fail("an inline kernel statement needs an array argument");
+    s.nodes.push_back(root);
+    s.root = (int)s.nodes.size() - 1;
+    s.kind = 3;
+    const uint64_t g[2] = {r.W, r.H}, l[2] = {(uint64_t)r.s0, (uint64_t)r.s1};
+    return chai_hip_cl_check((void*)r.seed, args.data(), na, r.vec_len, g, l);
+}
+
 int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
 {
     s.lhs = cs.lhs;
     if (cs.lhs >= t.num_vars) return fail("statement assigns variable %u of %u", cs.lhs, t.num_vars);
     if ((uint64_t)cs.first + cs.count > t.num_nodes || cs.count == 0) return fail("bad node range");
+    if (is_cl_stmt(t, cs)) return parse_cl_stmt(t, cs, s);
     const int rc = parse_nodes(t, t.nodes + cs.first, cs.count, s);
     if (rc) return rc;
     const ENode& r = s.nodes[s.root];
@@ -763,6 +830,7 @@ static int split_trace(const chai_trace& in, Versioned& V, const chai_trace*& ou
     for (uint32_t i = 0; i < in.num_stmts && !need; i++) {
         const chai_trace_stmt& st = in.stmts[i];
         if ((uint64_t)st.first + st.count > in.num_nodes || st.count == 0) return fail("bad node range");
+        if (is_cl_stmt(in, st)) continue;
         for (uint32_t k = 0; k < st.count; k++) {
             const uint32_t c = in.nodes[st.first + k].code;
             if (((is_red(c) || is_mm(c)) && k + 1 < st.count) || is_mm(c) || is_gather(c)) need = true;
@@ -804,6 +872,13 @@ static int split_trace(const chai_trace& in, Versioned& V, const chai_trace*& ou
     };
     for (uint32_t i = 0; i < in.num_stmts; i++) {
         const chai_trace_stmt& st = in.stmts[i];
+        if (is_cl_stmt(in, st)) {            // inline kernel: arguments are variables and scalars already
+            chai_trace_stmt ns = st;
+            ns.first = (uint32_t)V.nodes.size();
+            V.nodes.insert(V.nodes.end(), in.nodes + st.first, in.nodes + st.first + st.count);
+            V.stmts.push_back(ns);
+            continue;
+        }
         std::vector<chai_trace_node> cur;
         std::vector<size_t> starts;                // subtree start in cur, one per stack entry
         for (uint32_t k = 0; k < st.count; k++) {
@@ -854,6 +929,7 @@ static int version_trace(const chai_trace& in, Versioned& V, const chai_trace*& 
     for (uint32_t i = 0; i < in.num_stmts; i++) {
         const uint32_t v = in.stmts[i].lhs;
         if (v >= in.num_vars) return fail("statement assigns variable %u of %u", v, in.num_vars);
+        if (is_cl_stmt(in, in.stmts[i])) continue;      // an inline kernel mutates in place
         if (++nassign[v] > 1) multi = true;
     }
     if (!multi) return CHAI_OK;
@@ -873,6 +949,10 @@ static int version_trace(const chai_trace& in, Versioned& V, const chai_trace*& 
             n.arg = cur[n.arg];
         }
         const uint32_t v = st.lhs;
+        if (is_cl_stmt(in, st)) {       // mutates the current version of its arrays in place
+            st.lhs = cur[v];
+            continue;
+        }
         if (nassign[v] > 1 && ++done[v] < nassign[v]) {
             chai_trace_var tv = in.vars[v];
             tv.role = CHAI_VAR_TEMP;
@@ -904,6 +984,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
     for (uint32_t i = 0; i < t.num_stmts; i++) {
         int rc = parse_stmt(t, t.stmts[i], P.stmts[i]);
         if (rc) return rc;
+        if (P.stmts[i].kind == 3) continue;             // in-place kernel, not an assignment
         if (P.vars[P.stmts[i].lhs].role == CHAI_VAR_INPUT) return fail("statement assigns an input");
         if (writes[P.stmts[i].lhs]++) return fail("variable %u assigned twice", P.stmts[i].lhs);
     }
@@ -924,13 +1005,13 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
             if (is_gather(e.code) && !P.groups.empty() &&
                 group_of_var[s.nodes[e.kids[0]].src->arg] == (int)P.groups.size() - 1)
                 reads_red = true;
-        const bool join = s.kind != 2 && !P.groups.empty() && P.groups.back().kind == 0 &&
+        const bool join = s.kind != 2 && s.kind != 3 && !P.groups.empty() && P.groups.back().kind == 0 &&
                           P.groups.back().space == s.space && !reads_red &&
                           (s.kind == 0 || std::count_if(P.groups.back().stmts.begin(), P.groups.back().stmts.end(),
                                                         [&](int q) { return P.stmts[q].kind == 1; }) < VM_MAX_ACC);
         if (!join) {
             Group g;
-            g.kind = s.kind == 2 ? 2 : 0;
+            g.kind = s.kind == 2 || s.kind == 3 ? s.kind : 0;
             g.space = s.space;
             P.groups.push_back(g);
         }
@@ -942,7 +1023,8 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
     // which TEMP variables cross groups (or feed matmul) and need memory
     P.needs_mem.assign(t.num_vars, false);
     std::vector<int> def_group(t.num_vars, -1);
-    for (uint32_t i = 0; i < t.num_stmts; i++) def_group[P.stmts[i].lhs] = group_of_stmt[i];
+    for (uint32_t i = 0; i < t.num_stmts; i++)
+        if (P.stmts[i].kind != 3) def_group[P.stmts[i].lhs] = group_of_stmt[i];
     for (uint32_t i = 0; i < t.num_stmts; i++) {
         const Stmt& s = P.stmts[i];
         for (const auto& e : s.nodes) {
@@ -950,7 +1032,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
             const uint32_t v = e.src->arg;
             if (P.vars[v].role != CHAI_VAR_TEMP) continue;
             if (def_group[v] < 0) return fail("temp variable %u read before it is assigned", v);
-            if (def_group[v] != group_of_stmt[i] || s.kind == 2) P.needs_mem[v] = true;
+            if (def_group[v] != group_of_stmt[i] || s.kind == 2 || s.kind == 3) P.needs_mem[v] = true;
         }
         for (const auto& e : s.nodes)
             if (is_gather(e.code)) P.needs_mem[s.nodes[e.kids[0]].src->arg] = true;
@@ -959,7 +1041,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
     }
     // VM code per group
     for (auto& G : P.groups) {
-        if (G.kind == 2) continue;
+        if (G.kind != 0) continue;
         Emitter E{t, P, G};
         // last statement of the group reading each variable: its register is
         // released after that statement
@@ -1064,7 +1146,7 @@ static void jit_build(const TraceProgram& P, Group& G);
 static int materialize(TraceProgram& P)
 {
     for (auto& G : P.groups) {
-        if (G.kind == 2) continue;
+        if (G.kind != 0) continue;
         CHAI_HIP_TRY(hipMalloc((void**)&G.dprog, sizeof(VmIns) * G.prog.size()));
         CHAI_HIP_TRY(hipMemcpy(G.dprog, G.prog.data(), sizeof(VmIns) * G.prog.size(), hipMemcpyHostToDevice));
         if (jit_enabled()) jit_build(P, G);
@@ -1113,6 +1195,36 @@ static int run_matmul(const TraceProgram& P, const Stmt& s, void* const* bufs, h
     d.mode = CHAI_GEMM_AUTO;
     d.variant = -1;
     return chai_hip_gemm(&d, bufs[va], bufs[vb], bufs[s.lhs], st);
+}
+
+// an inline OpenCL kernel statement on the trace's buffers
+static int run_cl(const TraceProgram& P, const Stmt& s, void* const* bufs, hipStream_t st)
+{
+    const ENode& r = s.nodes[s.root];
+    std::vector<chai_cl_arg> args(r.kids.size());
+    for (size_t k = 0; k < r.kids.size(); k++) {
+        const ENode& e = s.nodes[r.kids[k]];
+        chai_cl_arg& a = args[k];
+        std::memset(&a, 0, sizeof a);
+        if (e.code == CHAI_TRACE_VAR) {
+            a.kind = CHAI_CL_ARG_ARRAY;
+            a.prec = (uint32_t)e.prec;
+            a.v.ptr = bufs[e.src->arg];
+        } else if (e.code == CHAI_TRACE_CL_LOCAL) {
+            a.kind = CHAI_CL_ARG_LOCAL;
+            a.v.count = e.src->W;
+        } else {
+            a.kind = CHAI_CL_ARG_SCALAR;
+            a.prec = (uint32_t)e.prec;
+            const double v = e.src->s0;
+            if (e.prec == CHAI_U32) a.v.u = (uint32_t)(int64_t)v;
+            else if (e.prec == CHAI_I32) a.v.i = (int32_t)v;
+            else if (e.prec == CHAI_F32) a.v.f = (float)v;
+            else a.v.d = v;
+        }
+    }
+    const uint64_t g[2] = {r.src->W, r.src->H}, l[2] = {(uint64_t)r.src->s0, (uint64_t)r.src->s1};
+    return cl_launch_kernel((void*)r.src->seed, args.data(), (uint32_t)args.size(), r.src->vec_len, g, l, st);
 }
 
 // ------------------------------------------------------------ trace JIT
@@ -1344,6 +1456,11 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
             if (rc) break;
             continue;
         }
+        if (G.kind == 3) {
+            rc = run_cl(P, P.stmts[G.stmts[0]], bufs.data(), st);
+            if (rc) break;
+            continue;
+        }
         VmLaunchArgs L;
         std::memset(&L, 0, sizeof L);
         L.prog = G.dprog;
@@ -1520,7 +1637,7 @@ extern "C" int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, c
     if (rc == CHAI_OK) rc = version_trace(*ts, ver, tv);
     if (rc == CHAI_OK) rc = compile_trace(*tv, P);
     if (rc) return rc;
-    if (kernel < 0 || kernel >= (int)P.groups.size() || P.groups[kernel].kind == 2) {
+    if (kernel < 0 || kernel >= (int)P.groups.size() || P.groups[kernel].kind != 0) {
         set_error("trace_emit_source: kernel %d is not a fused VM kernel", kernel);
         return CHAI_ERR_ARG;
     }

@@ -196,7 +196,9 @@ class Trace:
 
     def var(self, prec, W, H=1, slots=1, role=TEMP):
         self.vars.append((prec, role, W, H, slots))
-        return Var(len(self.vars) - 1)
+        v = Var(len(self.vars) - 1)
+        v.trace = self
+        return v
 
     def input(self, prec, W, H=1, slots=1):
         return self.var(prec, W, H, slots, INPUT)
@@ -212,6 +214,24 @@ class Trace:
         self._flatten(lit(expr), nodes)
         self.stmts.append((lhs.index, nodes))
         return lhs
+
+    def opencl(self, kernel, vals, ndim, glob, loc):
+        """Record an inline OpenCL kernel statement (kernel_from_opencl, bytecode 50):
+        argument nodes in order, then the root with the kernel handle and the work
+        space.  `vals` are (kind, prec, value) from program_cl.KernelCall; the
+        statement's lhs is the first array argument (api/ProgramCL.cpp:202-219)."""
+        nodes = []
+        for kind, prec, v in vals:
+            if kind == 0:
+                nodes.append((VAR, {"arg": v.index}))
+            elif kind == 1:
+                nodes.append((0x10001, {"W": int(v)}))
+            else:
+                nodes.append(({U32: 5, I32: 6, F32: 7, F64: 8}[prec], {"s0": float(v)}))
+        nodes.append((50, {"arg": len(vals), "seed": int(kernel), "vec_len": ndim, "W": glob[0], "H": glob[1],
+                           "s0": float(loc[0]), "s1": float(loc[1])}))
+        first = next(v for kind, _, v in vals if kind == 0)
+        self.stmts.append((first.index, nodes))
 
     def _flatten(self, e, out):
         for k in e.kids:

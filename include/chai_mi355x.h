@@ -366,6 +366,76 @@ int chai_hip_trace_kernel_source(void* handle, int kernel, char* buf, size_t len
  * (the same text as chai_hip_trace_kernel_source, produced without a GPU). */
 int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, char* buf, size_t len);
 
+/* ------------------------------------------------------- inline OpenCL C */
+/* The ProgramCL hook (api/ProgramCL.cpp:133-376, api/chai.h:703-771): user
+ * OpenCL C kernels launched on the back end's arrays.  Replaces the OpenCL
+ * build and enqueue of source kernels (vendor/OCLdevice.cpp:940-968,
+ * enqueue/EnqueueAST.cpp:432-489, memmgr/MemTransJIT.cpp:161-260).  The text
+ * is compiled as OpenCL C 1.2 for gfx950 by ROCm's code-object manager;
+ * argument kinds and sizes come from the code object's kernel metadata.
+ * Argument kinds follow ProgramCL's ARRAY_BASE / LOCAL_MEM / SCALAR
+ * (api/chai.h:708-713). */
+enum { CHAI_CL_ARG_ARRAY = 0, CHAI_CL_ARG_LOCAL = 1, CHAI_CL_ARG_SCALAR = 2, CHAI_CL_ARG_OTHER = 3 };
+
+typedef struct chai_cl_arg {
+    uint32_t kind;                 /* CHAI_CL_ARG_ARRAY / LOCAL / SCALAR      */
+    uint32_t prec;                 /* scalar precision (CHAI_U32 ... F64)     */
+    union {
+        void* ptr;                 /* ARRAY: device buffer                    */
+        uint64_t count;            /* LOCAL: bytes of __local memory          */
+        uint32_t u; int32_t i; float f; double d;   /* SCALAR                 */
+    } v;
+} chai_cl_arg;
+
+typedef struct chai_cl_arg_info {
+    uint32_t kind;                 /* CHAI_CL_ARG_*                           */
+    uint32_t size;                 /* bytes in the kernel argument segment    */
+    char type_name[64];            /* OpenCL type as written ("float*", "uint") */
+    char name[64];                 /* parameter name                          */
+} chai_cl_arg_info;
+
+/* Compile (or find in the cache: same text and options) a program.  On a
+ * compile error returns CHAI_ERR_ARG and chai_hip_last_error() holds the
+ * compiler log (ProgramCL's build failure, vendor/OCLdevice.cpp:940-968).
+ * Needs no device: compiling works without a GPU.  Handles stay valid until
+ * the process ends. */
+int chai_hip_cl_build(const char* text, size_t len, const char* options, void** program);
+int chai_hip_cl_build_log(void* program, char* buf, size_t len);
+/* explicit arguments of kernel `name` (ProgramCL::parseKernelArgs,
+ * api/ProgramCL.cpp:37-131, from metadata instead of the text) */
+int chai_hip_cl_kernel_args(void* program, const char* name, int* num_args, chai_cl_arg_info* info, int max_info);
+/* Launch kernel `name` over a 1- or 2-D work space (ProgramCL::Kernel's
+ * operator(), api/ProgramCL.cpp:327-342).  global sizes count work items, as
+ * in OpenCL, and must be multiples of the local sizes (local NULL or 0: 64
+ * when it divides, else 1).  Arguments are checked against the kernel's
+ * metadata (kind, and size for scalars); a mismatch returns CHAI_ERR_ARG
+ * before anything is launched.  __local arguments get their byte count, as
+ * clSetKernelArg(k, i, bytes, NULL) does. */
+int chai_hip_cl_launch(void* program, const char* name, const chai_cl_arg* args, uint32_t nargs, uint32_t dims,
+                       const uint64_t* global, const uint64_t* local, void* stream);
+/* A kernel handle (ProgramCL::Kernel, api/ProgramCL.cpp:185-200), valid as
+ * long as its program, for inline-kernel statements of traces. */
+int chai_hip_cl_kernel(void* program, const char* name, void** kernel);
+/* The checks of chai_hip_cl_launch without a launch (no device needed;
+ * array pointers may be NULL).  Compiles the __local wrapper if needed. */
+int chai_hip_cl_check(void* kernel, const chai_cl_arg* args, uint32_t nargs, uint32_t dims,
+                      const uint64_t* global, const uint64_t* local);
+
+/* Inline kernels inside traces: the statement the reference records for
+ * (progCL, "name", args...)(global, local) (api/ProgramCL.cpp:202-219,
+ * bytecode kernel_from_opencl = 50).  Nodes, in argument order: a
+ * CHAI_TRACE_VAR node per array argument, a scalar_u32/i32/f32/f64 node
+ * (bytecodes 5-8, value in s0) per scalar argument, a CHAI_TRACE_CL_LOCAL
+ * node (W = bytes) per __local argument; then the root node: code 50,
+ * seed = kernel handle from chai_hip_cl_kernel, arg = argument count,
+ * vec_len = work dimensions (1 or 2), W/H = global sizes, s0/s1 = local
+ * sizes.  The statement's lhs is its first array argument.  The kernel
+ * reads and writes its array arguments in place (inputs included, as the
+ * reference's kernels write the arrays they are given) and runs as a kernel
+ * of its own between the fused kernels before and after it; it does not
+ * count as an assignment of lhs. */
+enum { CHAI_TRACE_CL_LOCAL = 0x10001 };
+
 /* ------------------------------------------------------------------- GEMV */
 /* y = op(A) x or y = alpha op(A) x + beta y (Evergreen MV,
  * autotune/EvergreenMatmulMV.cpp:884-1403): A is M x N row-major (op = A^T
