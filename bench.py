@@ -35,6 +35,7 @@ PEAK_HBM_GBS = 8000.0
 METRIC = "GEMM TFLOP/s (bf16x3 MFMA vs FP32) at FP32-level error; autotune cold-start s"
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "gemm_bf16x3_8192_pmc.txt")
 PMC_SUMMARY_GATHER = os.path.join(ROOT, "profiles", "gather_stencil_8192_pmc.txt")
+PMC_SUMMARY_STREAM = os.path.join(ROOT, "profiles", "stream_trace_2e28_pmc.txt")
 MODEL_FILE = os.path.join(ROOT, "chai_amd", "data", "tile_mlp.npz")
 
 
@@ -44,7 +45,7 @@ def parse():
     p.add_argument("--steps", type=int, default=10)
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--workload", default="sgemm",
-                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed", "gather", "errors"])
+                   choices=["sgemm", "ragged", "autotune", "mc", "dgemm", "mixed", "gather", "errors", "stream"])
     p.add_argument("--size", type=int, default=8192)
     p.add_argument("--fp32-steps", type=int, default=3)
     p.add_argument("--cpu-size", type=int, default=1024,
@@ -198,9 +199,21 @@ def profile_run(args):
     print("profile run done", flush=True)
 
 
+def stream_trace(n):
+    """C = max(A * 0.5f + B, 0) over n f32: (trace, A, B, C)"""
+    from chai_amd import F32
+    from chai_amd import trace as tr
+    t = tr.Trace()
+    A, B = t.input(F32, n), t.input(F32, n)
+    C = t.output(F32, n)
+    t.assign(C, tr.maxe(A * tr.f32(0.5) + B, tr.f32(0.0)))
+    return t, A, B, C
+
+
 def profile_run_gather(args):
     """Short program for rocprofv3: the calibration kernels, then three JIT
-    launches of the 8192^2 gather stencil."""
+    launches of the 8192^2 gather stencil (or, for --workload stream, of the
+    2^28-element elementwise trace)."""
     import torch
     import chai_amd as chai
     from chai_amd.workloads import filter_trace
@@ -210,13 +223,17 @@ def profile_run_gather(args):
     chai.reduce(big, "sum")
     torch.empty_like(big).copy_(big)      # plain device copy: reads and writes 2^30 bytes
     del big
-    N = 8192
-    t, A, B = filter_trace(N)
-    a = uniform((N, N), 42).view(-1)
-    b = torch.empty(N * N, device="cuda")
+    if args.workload == "stream":
+        n = 1 << 28
+        t, A, B, C = stream_trace(n)
+        bufs = {A: uniform((n,), 42), B: uniform((n,), 43), C: torch.empty(n, device="cuda")}
+    else:
+        N = 8192
+        t, A, B = filter_trace(N)
+        bufs = {A: uniform((N, N), 42).view(-1), B: torch.empty(N * N, device="cuda")}
     prog = t.compile()
     for _ in range(3):
-        prog.run({A: a, B: b})
+        prog.run(bufs)
     torch.cuda.synchronize()
     print("profile run done", flush=True)
 
@@ -260,15 +277,16 @@ def pmc_summary(fetch_dir, write_dir, workload="sgemm"):
         return (best, sum(rows[best]) / len(rows[best])) if best else (None, None)
     _, f_cal = pick(fe, r"reduce_partial")
     _, w_cal = pick(wr, r"rng_fill_32")
-    pat = r"chai_trace_kernel" if workload == "gather" else r"gemm_x3_kernel"
+    trace_wl = workload in ("gather", "stream")
+    pat = r"chai_trace_kernel" if trace_wl else r"gemm_x3_kernel"
     kf, f_gemm = pick(fe, pat)
     kw, w_gemm = pick(wr, pat)
     fetch_scale = (2 ** 30) / (f_copy * 1024) if f_copy else None
     write_scale = (2 ** 30) / (w_copy * 1024) if w_copy else None
-    wl = " --workload gather" if workload == "gather" else ""
-    lines = ["# %s: HBM traffic per launch from rocprofv3 counters" % (
-                 "gather2_floor 5-point stencil 8192^2 f32 (JIT kernel)" if workload == "gather"
-                 else "bf16x3 GEMM 8192^3"),
+    wl = " --workload " + workload if trace_wl else ""
+    title = {"gather": "gather2_floor 5-point stencil 8192^2 f32 (JIT kernel)",
+             "stream": "fused elementwise trace C = max(A*0.5f + B, 0), 2^28 f32 (JIT kernel)"}
+    lines = ["# %s: HBM traffic per launch from rocprofv3 counters" % title.get(workload, "bf16x3 GEMM 8192^3"),
              "# commands (from the repository root):",
              "#   rocprofv3 --pmc FETCH_SIZE -d <fetch_dir> -o run -- python3 bench.py --profile-run" + wl,
              "#   rocprofv3 --pmc WRITE_SIZE -d <write_dir> -o run -- python3 bench.py --profile-run" + wl,
@@ -283,11 +301,12 @@ def pmc_summary(fetch_dir, write_dir, workload="sgemm"):
     if f_gemm and w_gemm and fetch_scale and write_scale:
         t = f_gemm * 1024 * fetch_scale + w_gemm * 1024 * write_scale
         n = 8192
-        alg = 8 * n * n if workload == "gather" else 2 * 2 * 2 * n * n + 4 * n * n
+        alg = {"gather": 8 * n * n, "stream": 12 * 2 ** 28}.get(workload, 2 * 2 * 2 * n * n + 4 * n * n)
         lines.append("traffic_bytes_per_launch = %.0f" % t)
-        lines.append("algorithmic_bytes_per_launch = %d  # %s" % (alg, (
-            "A read once, B written once (f32)" if workload == "gather"
-            else "bf16 hi+lo panels of A and B^T read once, C written")))
+        lines.append("algorithmic_bytes_per_launch = %d  # %s" % (alg, {
+            "gather": "A read once, B written once (f32)",
+            "stream": "A and B read once, C written once (f32)"}.get(
+                workload, "bf16 hi+lo panels of A and B^T read once, C written")))
         lines.append("traffic_over_algorithmic = %.2f" % (t / alg))
     lines.append("# per kernel, mean per dispatch (KiB, unscaled):")
     for k in sorted(set(fe) | set(wr)):
@@ -730,6 +749,47 @@ def run_errors(args):
     print(json.dumps({"errors_vs_fp64_per_shape": rows}))
 
 
+# ------------------------------------------------------------------ stream
+def run_stream(args):
+    """An HBM-bound fused elementwise trace, C = max(A * 0.5f + B, 0) over 2^28 f32
+    (1 GiB per array): 8 B read + 4 B written per element, one fused kernel.  Beside it:
+    torch's two-input elementwise kernel and a plain device copy (8 B per element),
+    the bandwidths this part reaches for the same access pattern."""
+    import numpy as np
+    import torch
+    import chai_amd as chai
+    from chai_amd import trace as tr
+    chai.init(0)
+    n = 1 << 28
+    t, A, B, C = stream_trace(n)
+    a, b = uniform((n,), 42), uniform((n,), 43)
+    c = torch.empty(n, device="cuda")
+    res = {}
+    for eng in ("jit", "vm"):
+        tr.set_jit(eng == "jit")
+        prog = t.compile()
+        res[eng] = timed(lambda: prog.run({A: a, B: b, C: c}), args.steps, args.warmup)
+    tr.set_jit(True)
+    ms = res["jit"]
+    k = 1 << 20
+    an, bn = a[:k].cpu().numpy(), b[:k].cpu().numpy()
+    ok = bool(np.array_equal(c[:k].cpu().numpy(), np.maximum(an * np.float32(0.5) + bn, np.float32(0))))
+    gbs = 12.0 * n / (ms * 1e-3) / 1e9
+    ms_torch = timed(lambda: torch.add(b, a, alpha=0.5, out=c), args.steps, args.warmup)
+    ms_copy = timed(lambda: c.copy_(a), args.steps, args.warmup)
+    line = base_line(args, round(gbs, 1), "GB/s", round(ms, 4), "f32",
+                     {"workload": "fused elementwise trace C = max(A*0.5f + B, 0) over 2^28 f32"})
+    line["metric"] = "fused elementwise trace GB/s (algorithmic bytes)"
+    line["vm_ms"] = round(res["vm"], 4)
+    line["matches_numpy_first_2^20"] = ok
+    line["torch_add_GBs"] = round(12.0 * n / (ms_torch * 1e-3) / 1e9, 1)
+    line["device_copy_GBs"] = round(8.0 * n / (ms_copy * 1e-3) / 1e9, 1)
+    line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                        "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": read_traffic(PMC_SUMMARY_STREAM),
+                        "kernel": "chai_trace_kernel"}
+    emit(line)
+
+
 def main():
     args = parse()
     if args.pmc_summary:
@@ -748,7 +808,7 @@ def main():
         sys.exit(2)
     if args.profile_run and args.workload == "ragged":
         profile_run_ragged(args)
-    elif args.profile_run and args.workload == "gather":
+    elif args.profile_run and args.workload in ("gather", "stream"):
         profile_run_gather(args)
     elif args.profile_run:
         profile_run(args)
@@ -760,6 +820,8 @@ def main():
         run_autotune(args)
     elif args.workload == "mc":
         run_mc(args)
+    elif args.workload == "stream":
+        run_stream(args)
     elif args.workload == "errors":
         run_errors(args)
     elif args.workload == "gather":

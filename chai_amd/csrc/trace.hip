@@ -1480,8 +1480,10 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
         }
         for (size_t k = 0; k < G.rng.size(); k++) L.rng[k] = G.rng[k];
         for (size_t k = 0; k < G.chains.size(); k++) L.chains[k] = G.chains[k];
+        // one vector per thread for elementwise kernels (a grid-stride loop only past
+        // 2^22 blocks); reductions keep at most 1024 partials per slot
         uint64_t bps = (L.vps + VM_THREADS - 1) / VM_THREADS;
-        const uint64_t cap = L.nacc ? 1024 : 65535;
+        const uint64_t cap = L.nacc ? 1024 : (1ull << 22);
         if (bps > cap) bps = cap;
         if (bps < 1) bps = 1;
         double* partial = nullptr;

@@ -604,7 +604,7 @@ __device__ __forceinline__ V4 vm_load(const VmVar& V, uint64_t slot, uint64_t y,
         const uint64_t base = (slot * H + y) * W + x0;
         if (nval == 4 && V.prec != CHAI_F64 && ((W & 3) == 0) && (((uintptr_t)V.in & 15) == 0)) {
             typedef uint32_t u4 __attribute__((ext_vector_type(4)));
-            const u4 q = *reinterpret_cast<const u4*>((const uint32_t*)V.in + base);
+            const u4 q = __builtin_nontemporal_load(reinterpret_cast<const u4*>((const uint32_t*)V.in + base));
             r.l[0] = q[0]; r.l[1] = q[1]; r.l[2] = q[2]; r.l[3] = q[3];
             return r;
         }
@@ -680,7 +680,7 @@ __device__ __forceinline__ void vm_store(const VmVar& V, const V4& v, uint64_t s
     } else if (nval == 4 && (W & 3) == 0 && (((uintptr_t)V.out & 15) == 0)) {
         typedef uint32_t u4 __attribute__((ext_vector_type(4)));
         u4 q = {(uint32_t)v.l[0], (uint32_t)v.l[1], (uint32_t)v.l[2], (uint32_t)v.l[3]};
-        *reinterpret_cast<u4*>((uint32_t*)V.out + base) = q;
+        __builtin_nontemporal_store(q, reinterpret_cast<u4*>((uint32_t*)V.out + base));
     } else {
         for (int i = 0; i < 4; i++) if (i < nval) ((uint32_t*)V.out)[base + i] = (uint32_t)v.l[i];
     }
