@@ -1140,6 +1140,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
 }
 
 static bool jit_enabled();
+static size_t jit_max_instructions();
 static void jit_build(const TraceProgram& P, Group& G);
 
 // device side of compilation: upload VM programs, JIT-compile when enabled
@@ -1149,7 +1150,18 @@ static int materialize(TraceProgram& P)
         if (G.kind != 0) continue;
         CHAI_HIP_TRY(hipMalloc((void**)&G.dprog, sizeof(VmIns) * G.prog.size()));
         CHAI_HIP_TRY(hipMemcpy(G.dprog, G.prog.data(), sizeof(VmIns) * G.prog.size(), hipMemcpyHostToDevice));
-        if (jit_enabled()) jit_build(P, G);
+        if (!jit_enabled()) continue;
+        if (G.prog.size() <= jit_max_instructions()) {
+            jit_build(P, G);
+        } else {
+            // hipRTC compile time grows faster than linearly with the straight-line
+            // code (measured: 0.5 s at 16 statements, 7.4 s at 512, 98 s at 2048), so
+            // a kernel this long runs on the VM, which needs no compile
+            char b[160];
+            snprintf(b, sizeof b, "not compiled: %zu VM instructions exceed the JIT limit of %zu "
+                     "(CHAI_TRACE_JIT_MAX); runs on the trace VM", G.prog.size(), jit_max_instructions());
+            G.jit_log = b;
+        }
     }
     return CHAI_OK;
 }
@@ -1235,6 +1247,17 @@ static int run_cl(const TraceProgram& P, const Stmt& s, void* const* bufs, hipSt
 // plays, enqueue/EnqueueIdSpace.cpp:19-320).  Semantics are the VM's own
 // device functions (trace_vm.hpp), so JIT and VM agree bit for bit.
 static int g_jit_mode = -1;   // -1: read CHAI_TRACE_JIT once; 0 off; 1 on
+
+// longest fused kernel (VM instructions) compiled by the JIT; longer ones run on the VM
+static size_t jit_max_instructions()
+{
+    static const size_t lim = [] {
+        const char* e = getenv("CHAI_TRACE_JIT_MAX");
+        const long v = e ? strtol(e, nullptr, 10) : 0;
+        return v > 0 ? (size_t)v : (size_t)1024;
+    }();
+    return lim;
+}
 
 static bool jit_enabled()
 {
