@@ -23,6 +23,7 @@
 #include <hip/hiprtc.h>
 
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -1141,6 +1142,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
 
 static bool jit_enabled();
 static size_t jit_max_instructions();
+static size_t jit_rolled_size(const std::vector<VmIns>& prog);
 static void jit_build(const TraceProgram& P, Group& G);
 
 // device side of compilation: upload VM programs, JIT-compile when enabled
@@ -1151,15 +1153,16 @@ static int materialize(TraceProgram& P)
         CHAI_HIP_TRY(hipMalloc((void**)&G.dprog, sizeof(VmIns) * G.prog.size()));
         CHAI_HIP_TRY(hipMemcpy(G.dprog, G.prog.data(), sizeof(VmIns) * G.prog.size(), hipMemcpyHostToDevice));
         if (!jit_enabled()) continue;
-        if (G.prog.size() <= jit_max_instructions()) {
+        if (jit_rolled_size(G.prog) <= jit_max_instructions()) {
             jit_build(P, G);
         } else {
             // hipRTC compile time grows faster than linearly with the straight-line
             // code (measured: 0.5 s at 16 statements, 7.4 s at 512, 98 s at 2048), so
             // a kernel this long runs on the VM, which needs no compile
             char b[160];
-            snprintf(b, sizeof b, "not compiled: %zu VM instructions exceed the JIT limit of %zu "
-                     "(CHAI_TRACE_JIT_MAX); runs on the trace VM", G.prog.size(), jit_max_instructions());
+            snprintf(b, sizeof b, "not compiled: %zu VM instructions (after loop rolling) exceed the JIT limit "
+                     "of %zu (CHAI_TRACE_JIT_MAX); runs on the trace VM", jit_rolled_size(G.prog),
+                     jit_max_instructions());
             G.jit_log = b;
         }
     }
@@ -1278,6 +1281,121 @@ static std::string hexd(double d)
     return b;
 }
 
+// ------------------------------------------------------------ loop rolling
+// A traced C++ loop reaches the compiler unrolled (the reference folds such
+// runs into loops, jitbc/ByteTrace.cpp:211-387).  In a fused kernel's VM
+// program the repetitions are equal up to a fixed permutation pi of register
+// slots: repetition j + 1 is repetition j with every register r renamed pi(r).
+// The JIT then emits the first repetition once in a loop and, after each
+// iteration, moves R[r] <- R[pi(r)]; the values stay those of the unrolled
+// program (R_loop[r] = R_unrolled[pi^j(r)] before iteration j), and a final
+// move restores the unrolled register assignment.  The VM runs the unrolled
+// program as it is.
+struct JitSeg {
+    size_t start;
+    int L, k;                                   // pattern length, repetitions (k <= 1: one instruction)
+    std::array<int, VM_NSLOTS> pi;
+};
+
+// register operands an instruction reads or writes, in a fixed order
+static int ins_regs(const VmIns& I, int* r)
+{
+    switch (I.op) {
+        case VM_LOAD: case VM_IMM: case VM_RNG: case VM_INDEX: r[0] = I.dst; return 1;
+        case VM_CVT: case VM_UN: case VM_MOV: r[0] = I.dst; r[1] = I.s1; return 2;
+        case VM_BIN: case VM_GATHER: r[0] = I.dst; r[1] = I.s1; r[2] = I.s2; return 3;
+        case VM_TER: case VM_SEL: r[0] = I.dst; r[1] = I.s1; r[2] = I.s2; r[3] = I.s3; return 4;
+        case VM_STORE: case VM_ACC: r[0] = I.s1; return 1;
+        default: r[0] = I.dst; r[1] = I.s1; return 2;
+    }
+}
+
+static bool same_but_regs(const VmIns& x, const VmIns& y)
+{
+    return x.op == y.op && x.p == y.p && x.p2 == y.p2 && x.a == y.a && x.b == y.b && x.c == y.c && x.e == y.e &&
+           x.u == y.u && std::memcmp(&x.d0, &y.d0, 8) == 0 && std::memcmp(&x.d1, &y.d1, 8) == 0;
+}
+
+// extend the partial map m (-1 = unset) from prog[a, a+L) to prog[b, b+L); false on a mismatch
+static bool match_rename(const std::vector<VmIns>& prog, size_t a, size_t b, int L, std::array<int, VM_NSLOTS>& m,
+                         std::array<int, VM_NSLOTS>& mi)
+{
+    for (int q = 0; q < L; q++) {
+        const VmIns &x = prog[a + q], &y = prog[b + q];
+        if (!same_but_regs(x, y)) return false;
+        int rx[4], ry[4];
+        const int n = ins_regs(x, rx);
+        ins_regs(y, ry);
+        for (int t = 0; t < n; t++) {
+            if (rx[t] >= VM_NSLOTS || ry[t] >= VM_NSLOTS) return false;
+            if (m[rx[t]] < 0 && mi[ry[t]] < 0) { m[rx[t]] = ry[t]; mi[ry[t]] = rx[t]; }
+            else if (m[rx[t]] != ry[t] || mi[ry[t]] != rx[t]) return false;
+        }
+    }
+    return true;
+}
+
+static std::vector<JitSeg> jit_plan(const std::vector<VmIns>& prog)
+{
+    std::vector<JitSeg> plan;
+    const size_t n = prog.size();
+    size_t i = 0;
+    while (i < n) {
+        JitSeg best{i, 1, 1, {}};
+        size_t best_cov = 1;
+        for (int L = 1; L <= 64 && i + 2 * (size_t)L <= n; L++) {
+            std::array<int, VM_NSLOTS> m, mi;
+            m.fill(-1);
+            mi.fill(-1);
+            if (!match_rename(prog, i, i + L, L, m, mi)) continue;
+            // complete the partial renaming to a permutation of all slots
+            std::array<int, VM_NSLOTS> pi = m;
+            std::vector<int> free_dst;
+            for (int r = 0; r < VM_NSLOTS; r++) if (mi[r] < 0) free_dst.push_back(r);
+            size_t f = 0;
+            for (int r = 0; r < VM_NSLOTS; r++) if (pi[r] < 0) pi[r] = free_dst[f++];
+            // further repetitions must follow the same permutation
+            int k = 2;
+            while (i + (size_t)(k + 1) * L <= n) {
+                std::array<int, VM_NSLOTS> m2, mi2;
+                m2.fill(-1);
+                mi2.fill(-1);
+                if (!match_rename(prog, i + (size_t)(k - 1) * L, i + (size_t)k * L, L, m2, mi2)) break;
+                bool ok = true;
+                for (int r = 0; r < VM_NSLOTS && ok; r++) if (m2[r] >= 0 && m2[r] != pi[r]) ok = false;
+                if (!ok) break;
+                k++;
+            }
+            const size_t cov = (size_t)k * L;
+            if (cov > best_cov && cov >= 8) { best = JitSeg{i, L, k, pi}; best_cov = cov; }
+        }
+        plan.push_back(best);
+        i += best_cov;
+    }
+    return plan;
+}
+
+// R[r] <- R[src[r]] for every slot at once
+static std::string perm_code(const std::array<int, VM_NSLOTS>& src)
+{
+    std::string s, t;
+    char b[96];
+    for (int r = 0; r < VM_NSLOTS; r++)
+        if (src[r] != r) { snprintf(b, sizeof b, " const V4 t%d = R%d;", r, src[r]); s += b; }
+    if (s.empty()) return "";
+    for (int r = 0; r < VM_NSLOTS; r++)
+        if (src[r] != r) { snprintf(b, sizeof b, " R%d = t%d;", r, r); t += b; }
+    return "        {" + s + t + " }\n";
+}
+
+// instructions the JIT source holds after rolling
+static size_t jit_rolled_size(const std::vector<VmIns>& prog)
+{
+    size_t n = 0;
+    for (const JitSeg& g : jit_plan(prog)) n += g.k <= 1 ? 1 : (size_t)g.L;
+    return n;
+}
+
 static std::string jit_source(const TraceProgram& P, const Group& G)
 {
     const uint64_t W = G.space.W, H = G.space.H, S = G.space.S;
@@ -1300,7 +1418,8 @@ static std::string jit_source(const TraceProgram& P, const Group& G)
              (unsigned long long)vps, (unsigned long long)vpr, (unsigned long long)vpr,
              (unsigned long long)W, (unsigned long long)W);
     s += b;
-    for (const VmIns& I : G.prog) {
+    // one VM instruction as a line of HIP
+    const auto emit_ins = [&](const VmIns& I) -> std::string {
         const auto var_lit = [&](uint32_t k, bool in) {
             const uint32_t v = G.var_of_slot[k];
             const chai_trace_var& tv = P.vars[v];
@@ -1399,7 +1518,28 @@ static std::string jit_source(const TraceProgram& P, const Group& G)
                 snprintf(b, sizeof b, "        R%d = R%d;\n", I.dst, I.s1);
                 break;
         }
+        return std::string(b);
+    };
+    // repeated runs (a traced loop) become one copy of the pattern in a for loop
+    const std::vector<JitSeg> plan = jit_plan(G.prog);
+    for (const JitSeg& g : plan) {
+        if (g.k <= 1) {
+            s += emit_ins(G.prog[g.start]);
+            continue;
+        }
+        snprintf(b, sizeof b, "        for (int it = 0; it < %d; ++it) {\n", g.k);
         s += b;
+        for (int q = 0; q < g.L; q++) s += "    " + emit_ins(G.prog[g.start + q]);
+        s += "    " + perm_code(g.pi);         // R[r] <- R[pi(r)]
+        s += "        }\n";
+        // after k iterations R_loop[r] = R_unrolled[pi^k(r)]: put values back
+        std::array<int, VM_NSLOTS> pk;
+        for (int r = 0; r < VM_NSLOTS; r++) pk[r] = r;
+        for (int it = 0; it < g.k; it++)
+            for (int r = 0; r < VM_NSLOTS; r++) pk[r] = g.pi[pk[r]];
+        std::array<int, VM_NSLOTS> inv;
+        for (int r = 0; r < VM_NSLOTS; r++) inv[pk[r]] = r;
+        s += perm_code(inv);                       // R[q] <- R_loop[inv(q)]
     }
     s += "    }\n";
     if (!G.red_vars.empty()) {

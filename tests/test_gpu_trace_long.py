@@ -1,7 +1,8 @@
-"""Long fused kernels: a kernel longer than the JIT limit (CHAI_TRACE_JIT_MAX, 1024 VM
+"""Long fused kernels: repeated statements (a traced loop) are emitted as a loop and
+compiled; a kernel still longer than the JIT limit after that (CHAI_TRACE_JIT_MAX, 1024 VM
 instructions by default) runs on the trace VM instead of being compiled by hipRTC, whose
-compile time grows faster than linearly with the kernel's length; a short kernel is
-compiled.  Both give the step-by-step f32 result bit for bit."""
+compile time grows faster than linearly with the kernel's length.  All give the
+step-by-step f32 result bit for bit."""
 import numpy as np
 import pytest
 import torch
@@ -12,28 +13,34 @@ from chai_amd import trace as tr
 pytestmark = pytest.mark.gpu
 
 
-def loop_trace(reps, n):
-    """C = A; then `reps` times C = C * 0.999f + B (a traced C++ loop, unrolled)"""
+def loop_trace(reps, n, varying=False):
+    """C = A; then `reps` times C = C * c_i + B (a traced C++ loop, unrolled); c_i =
+    0.999f each time, or a different constant per statement when `varying`"""
     t = tr.Trace()
     A, B = t.input(F32, n), t.input(F32, n)
     C = t.output(F32, n)
     t.assign(C, A + tr.f32(0.0))
-    for _ in range(reps):
-        t.assign(C, C * tr.f32(0.999) + B)
-    return t, A, B, C
+    consts = [np.float32(0.999 - (i * 1e-4 if varying else 0.0)) for i in range(reps)]
+    for c in consts:
+        t.assign(C, C * tr.f32(float(c)) + B)
+    return t, A, B, C, consts
 
 
-@pytest.mark.parametrize("reps", [8, 400])
-def test_long_kernel_runs_on_the_vm(chai, reps):
+@pytest.mark.parametrize("reps,varying", [(8, False), (400, False), (400, True)])
+def test_long_kernels_roll_or_run_on_the_vm(chai, reps, varying):
+    """400 repetitions of one statement roll into a loop and are compiled; 400
+    different statements exceed the JIT limit and run on the VM"""
     n = 4096
     tr.set_jit(True)
-    t, A, B, C = loop_trace(reps, n)
+    t, A, B, C, consts = loop_trace(reps, n, varying)
     prog = t.compile()
     assert prog.info()[0] == 1
     src, jitted = prog.source(0)
-    assert jitted == (reps == 8)
-    if reps == 400:
+    assert jitted == (not varying)
+    if varying:
         assert "exceed the JIT limit" in src
+    elif reps == 400:
+        assert "for (int it = 0; it < 400;" in src
     rng = np.random.default_rng(reps)
     a = rng.standard_normal(n).astype(np.float32)
     b = rng.standard_normal(n).astype(np.float32)
@@ -41,8 +48,8 @@ def test_long_kernel_runs_on_the_vm(chai, reps):
     prog.run({A: torch.from_numpy(a).cuda(), B: torch.from_numpy(b).cuda(), C: c})
     torch.cuda.synchronize()
     ref = a + np.float32(0.0)
-    for _ in range(reps):
-        ref = (ref * np.float32(0.999)).astype(np.float32) + b
+    for k in consts:
+        ref = (ref * k).astype(np.float32) + b
     assert np.array_equal(c.cpu().numpy(), ref)
 
 

@@ -189,3 +189,67 @@ def test_emitted_scalar_int_ops_match_oracle(prec, tmp_path):
             411, 412, 413, 414, 415]
     int1 = [10, 15, 16, 400, 404, 225]
     _check_ops(prec, ((1, int1), (2, int2), (3, INT3)), tmp_path)
+
+
+def _loop_trace(reps, n=37):
+    from chai_amd.trace import Trace, f32
+    t = Trace()
+    A, B = t.input(F32, n, 3), t.input(F32, n, 3)
+    C = t.output(F32, n, 3)
+    t.assign(C, A + f32(0.0))
+    for _ in range(reps):
+        t.assign(C, C * f32(0.999) + B)
+    return t, A, B, C
+
+
+@pytest.mark.parametrize("reps", [3, 20, 21])
+def test_rolled_loop_matches_oracle(reps, tmp_path):
+    """a traced loop (one carried variable) is emitted as a for loop with a
+    register shuffle, and computes what the unrolled statements compute"""
+    from chai_amd import trace as tr
+    t, A, B, C = _loop_trace(reps)
+    src = tr.emit_source(t, 0)
+    assert ("for (int it = 0; it < %d;" % reps) in src or reps < 4    # 5-instruction body, phase-shifted
+    rng = np.random.default_rng(reps)
+    data = {A.index: rng.standard_normal((1, 3, 37)).astype(np.float32),
+            B.index: rng.standard_normal((1, 3, 37)).astype(np.float32)}
+    got, ref = emulate(t, data, tmp_path)
+    np.testing.assert_array_equal(got[C.index], ref[C.index])
+
+
+@pytest.mark.parametrize("reps", [6, 7])
+def test_rolled_loop_two_carried_variables(reps, tmp_path):
+    """two variables carried through the loop (a rotation), an odd and an even
+    trip count (the registers' final positions differ), an integer counter too"""
+    from chai_amd.trace import Trace, f32
+    from chai_amd import trace as tr
+    t = Trace()
+    X0, Y0 = t.input(F32, 50), t.input(F32, 50)
+    X, Y, K = t.output(F32, 50), t.output(F32, 50), t.output(I32, 50)
+    t.assign(X, X0 * f32(1.0))
+    t.assign(Y, Y0 * f32(1.0))
+    t.assign(K, Trace.index(I32, 0, 50))
+    for _ in range(reps):
+        t.assign(X, X * f32(0.8) - Y * f32(0.6))
+        t.assign(Y, X * f32(0.6) + Y * f32(0.8))
+        t.assign(K, K * 3 + 1)
+    src = tr.emit_source(t, 0)
+    assert "for (int it = 0; it < %d;" % (reps - 1) in src     # the last repetition stores instead
+    rng = np.random.default_rng(reps)
+    data = {X0.index: rng.standard_normal((1, 1, 50)).astype(np.float32),
+            Y0.index: rng.standard_normal((1, 1, 50)).astype(np.float32)}
+    got, ref = emulate(t, data, tmp_path)
+    for v in (X, Y, K):
+        np.testing.assert_array_equal(got[v.index], ref[v.index])
+
+
+def test_varying_statements_are_not_rolled():
+    from chai_amd.trace import Trace, f32
+    from chai_amd import trace as tr
+    t = Trace()
+    A = t.input(F32, 64)
+    C = t.output(F32, 64)
+    t.assign(C, A + f32(0.0))
+    for i in range(12):
+        t.assign(C, C * f32(1.0 + i / 64.0) + A)
+    assert "for (int it" not in tr.emit_source(t, 0)
