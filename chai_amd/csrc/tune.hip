@@ -18,6 +18,7 @@
 #include "common.hpp"
 #include "gemm_internal.hpp"
 
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <map>
@@ -101,6 +102,18 @@ struct Memo {
 
     bool good(const std::string& k) const { return !bad.count(k) && bench.count(k); }
 };
+
+// number of elements of a and b that differ in value (NaN equals NaN, -0 equals +0)
+template <typename T>
+__global__ void count_mismatch(const T* a, const T* b, uint64_t n, unsigned int* out)
+{
+    unsigned int c = 0;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const T x = a[i], y = b[i];
+        if (!(x == y) && !(x != x && y != y)) c++;
+    }
+    if (c) atomicAdd(out, c);
+}
 
 std::mutex g_mu;
 Memo g_memo;
@@ -244,18 +257,100 @@ int chai_hip_gemm_time_variant(const chai_gemm_desc* desc, const void* A, const 
     return CHAI_OK;
 }
 
+// The output check of a tuning run (AutoTuneBench::runTime -> checkOutput,
+// autotune/AutoTuneBench.cpp:17-251).  Every tile variant of a mode
+// accumulates each output element in the same k order, so all of them must
+// give the same C: the first variant run in this search is the reference and
+// every later one is compared with it by value, journalled as BEFORE_CHECK ->
+// AFTER_CHECK, or BEFORE_CHECK -> BAD_OUTPUT (blacklisted, also after a
+// reload).  alpha A B + beta C reads C, which the timed runs overwrite, so
+// general GEMMs are not checked.
+struct OutputCheck {
+    void* ref = nullptr;
+    uint64_t n = 0;
+    size_t es = 0;
+    unsigned int* cnt = nullptr;
+    hipStream_t st = nullptr;
+    ~OutputCheck()
+    {
+        if (ref) (void)hipFreeAsync(ref, st);
+        if (cnt) (void)hipFreeAsync(cnt, st);
+    }
+};
+
+static uint64_t c_extent(const chai_gemm_desc& d)
+{
+    const uint64_t nb = d.batch ? d.batch : 1;
+    return (nb - 1) * d.stride_c + (d.M - 1) * d.ldc + d.N;
+}
+
+// 1: same as the reference (or the reference was just taken), 0: differs, <0: error
+static int check_output(OutputCheck& k, const chai_gemm_desc& d, const void* C, hipStream_t st)
+{
+    if (!k.ref) {
+        k.es = d.prec_c == CHAI_F64 ? 8 : 4;
+        k.n = c_extent(d);
+        k.st = st;
+        if (hipMallocAsync(&k.ref, k.n * k.es, st) != hipSuccess ||
+            hipMallocAsync((void**)&k.cnt, sizeof(unsigned int), st) != hipSuccess) {
+            set_error("gemm_tune: no memory for the output check");
+            return CHAI_ERR_NOMEM;
+        }
+        if (hipMemcpyAsync(k.ref, C, k.n * k.es, hipMemcpyDeviceToDevice, st) != hipSuccess) return CHAI_ERR_HIP;
+        return 1;
+    }
+    unsigned int h = 0;
+    if (hipMemsetAsync(k.cnt, 0, sizeof(unsigned int), st) != hipSuccess) return CHAI_ERR_HIP;
+    const unsigned blocks = (unsigned)std::min<uint64_t>((k.n + 255) / 256, 2048);
+    if (k.es == 8)
+        count_mismatch<double><<<blocks, 256, 0, st>>>((const double*)k.ref, (const double*)C, k.n, k.cnt);
+    else
+        count_mismatch<float><<<blocks, 256, 0, st>>>((const float*)k.ref, (const float*)C, k.n, k.cnt);
+    if (hipMemcpyAsync(&h, k.cnt, sizeof h, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipStreamSynchronize(st) != hipSuccess) {
+        set_error("gemm_tune: output check failed to run");
+        return CHAI_ERR_HIP;
+    }
+    return h == 0 ? 1 : 0;
+}
+
 int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B, void* C, int trials,
                        int* best_variant, float* best_ms, void* stream)
 {
     if (!desc) { set_error("gemm_tune: null descriptor"); return CHAI_ERR_ARG; }
     const int mode = mode_of(*desc);
     const int nv = gemm_num_variants(mode);
+    const bool checked = !desc->general && desc->M && desc->N && C && mode != CHAI_GEMM_FMA;
+    hipStream_t st = as_stream(stream);
+    OutputCheck ck;
     int bv = -1;
     float bt = 1e30f;
     for (int v = 0; v < nv; v++) {
         float ms = -1.f;
+        bool ran = false;
+        {
+            chai_gemm_desc d = *desc;
+            d.mode = mode;
+            const std::string name = kernel_name(d, mode, v);
+            std::lock_guard<std::mutex> lk(g_mu);
+            ran = !g_memo.bad.count(name) && !g_memo.good(name);
+        }
         const int rc = chai_hip_gemm_time_variant(desc, A, B, C, v, trials, &ms, stream);
         if (rc != CHAI_OK && rc != CHAI_ERR_ARG) return rc;
+        if (ms > 0.f && ran && checked && !ck.ref) {
+            const int rc2 = check_output(ck, *desc, C, st);    // the first variant run: the reference
+            if (rc2 < 0) return rc2;
+        } else if (ms > 0.f && ran && checked) {
+            chai_gemm_desc d = *desc;
+            d.mode = mode;
+            const std::string name = kernel_name(d, mode, v);
+            { std::lock_guard<std::mutex> lk(g_mu); g_memo.add(name, BEFORE_CHECK); }
+            const int ok = check_output(ck, *desc, C, st);
+            if (ok < 0) return ok;
+            std::lock_guard<std::mutex> lk(g_mu);
+            g_memo.add(name, ok ? AFTER_CHECK : BAD_OUTPUT);
+            if (!ok) ms = -1.f;
+        }
         if (ms > 0.f && ms < bt) { bt = ms; bv = v; }
     }
     if (bv < 0) { set_error("gemm_tune: no variant ran"); return CHAI_ERR_STATE; }

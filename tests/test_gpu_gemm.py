@@ -263,8 +263,12 @@ def test_tuner_journal_crash_resume(chai, tmp_path):
     nv = len(chai.gemm_variants())
     by_v = {v: [l for l in lines if l[1].endswith("_v%d" % v)] for v in range(nv)}
     assert all(by_v[v] for v in range(nv))
-    times = {v: int(by_v[v][-1][2]) for v in range(nv)}
+    times = {v: next(int(r[2]) for r in by_v[v] if int(r[2]) > 0) for v in range(nv)}
     assert all(t > 0 for t in times.values())
+    # output checks: the first variant run is the reference, every other one is
+    # compared with it (BEFORE_CHECK -100, AFTER_CHECK -101) and none differs
+    assert all([r[2] for r in by_v[v]][-2:] == ["-100", "-101"] for v in range(1, nv))
+    assert not any(r[2] == "-7" for r in lines)
     crashed = tmp_path / "journal1.AMD"
     with open(crashed, "w") as f:
         for v in range(nv):
@@ -282,3 +286,30 @@ def test_tuner_journal_crash_resume(chai, tmp_path):
         assert crashed.read_text() == before          # memo hits only: nothing re-timed
     finally:
         chai.memo_close()
+
+
+@pytest.mark.parametrize("mode", ["bf16x3", "f64"])
+def test_tuner_checks_every_variant_against_the_first(chai, tmp_path, mode):
+    """The tuning search compares each variant's output with the first one's
+    (every variant accumulates in the same k order; AutoTuneBench's
+    checkOutput).  Integer-valued A (all lo panels zero) makes some tiles skip
+    products, which may only change the sign of zeros: compared by value."""
+    import torch
+    M, N, K = 300, 520, 200
+    dt = torch.float64 if mode == "f64" else torch.float32
+    A = torch.randint(-8, 8, (M, K), device="cuda").to(dt)
+    A[:100] = torch.rand(100, K, device="cuda", dtype=dt)        # lo panels nonzero in the first rows only
+    B = torch.rand(K, N, device="cuda", dtype=dt)
+    C = torch.empty(M, N, device="cuda", dtype=dt)
+    j = tmp_path / "journal0.AMD"
+    chai.memo_close()
+    chai.memo_open(j, 0)
+    try:
+        m = chai.GEMM_F64 if mode == "f64" else chai.GEMM_BF16X3
+        chai.gemm_tune(A, B, C, mode=m, trials=1)
+    finally:
+        chai.memo_close()
+    lines = [l.split() for l in j.read_text().split("\n") if l.strip()]
+    states = [r[2] for r in lines]
+    assert "-7" not in states
+    assert states.count("-101") == len(chai.gemm_variants(m)) - 1
