@@ -29,6 +29,7 @@
 
 #include <dlfcn.h>
 
+#include <algorithm>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstdio>
@@ -246,8 +247,11 @@ int compile_cl(ClProgram& P)
     amd_comgr_action_info_t ai = {0};
     bool have_src = false, have_in = false, have_bc = false, have_reloc = false, have_out = false, have_ai = false,
          have_exe = false;
-    // options: kernel-argument metadata (type and parameter names) plus the caller's
-    std::vector<std::string> opts = {"-cl-kernel-arg-info", "-O3"};
+    // options: kernel-argument metadata (type and parameter names), no packed
+    // FP32 instructions (the target feature is recorded in every function's
+    // attributes, so code generation honours it too), plus the caller's
+    std::vector<std::string> opts = {"-cl-kernel-arg-info", "-O3", "-Xclang", "-target-feature", "-Xclang",
+                                     "-packed-fp32-ops"};
     {
         size_t p = 0;
         const std::string& o = P.options;
@@ -286,7 +290,14 @@ int compile_cl(ClProgram& P)
     append_logs(bc, P.log);
     COMGR_TRY(cg().create_data_set(&reloc), "create_data_set");
     have_reloc = true;
-    COMGR_TRY(cg().action_info_set_option_list(ai, nullptr, 0), "set_option_list");
+    {
+        // code generation runs the optimisation pipeline only when asked to
+        // (an empty option list compiles the bitcode at -O0: nothing inlined,
+        // every value spilled to lanes); the target feature is repeated so
+        // no packed FP32 instruction is selected
+        static const char* cgopts[] = {"-O3", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+        COMGR_TRY(cg().action_info_set_option_list(ai, cgopts, sizeof cgopts / sizeof cgopts[0]), "set_option_list");
+    }
     if (cg().do_action(AMD_COMGR_ACTION_CODEGEN_BC_TO_RELOCATABLE, ai, bc, reloc) != AMD_COMGR_STATUS_SUCCESS) {
         append_logs(reloc, P.log);
         rc = cfail("code generation failed:\n%s", P.log.c_str());
@@ -523,6 +534,15 @@ extern "C" int chai_hip_cl_build_log(void* program, char* buf, size_t len)
     if (!program || !buf || !len) return cfail("null argument");
     const std::string& s = static_cast<ClProgram*>(program)->log;
     snprintf(buf, len, "%s", s.c_str());
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_cl_code_object(void* program, void* buf, size_t len, size_t* size)
+{
+    if (!program || !size) return cfail("null argument");
+    const std::vector<char>& code = static_cast<ClProgram*>(program)->code;
+    *size = code.size();
+    if (buf && len) std::memcpy(buf, code.data(), std::min(len, code.size()));
     return CHAI_OK;
 }
 

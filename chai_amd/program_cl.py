@@ -58,6 +58,7 @@ def _bind(L):
     L.chai_hip_cl_kernel.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(vp)]
     L.chai_hip_cl_launch.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(ClArg), u32, u32, u64p, u64p, vp]
     L.chai_hip_cl_check.argtypes = [vp, ctypes.POINTER(ClArg), u32, u32, u64p, u64p]
+    L.chai_hip_cl_code_object.argtypes = [vp, vp, sz, ctypes.POINTER(sz)]
     return L
 
 
@@ -97,6 +98,15 @@ class ProgramCL:
             check(L.chai_hip_cl_kernel_args(self.h, name.encode(), ctypes.byref(n), info, 64), "ProgramCL args")
             self._args[name] = [(a.kind, a.size, a.type_name.decode(), a.name.decode()) for a in info[:n.value]]
         return self._args[name]
+
+    def code_object(self):
+        """the gfx950 executable code object the text compiled into (bytes)"""
+        L = _bind(lib())
+        n = ctypes.c_size_t()
+        check(L.chai_hip_cl_code_object(self.h, None, 0, ctypes.byref(n)), "ProgramCL code object")
+        buf = ctypes.create_string_buffer(n.value)
+        check(L.chai_hip_cl_code_object(self.h, buf, n.value, ctypes.byref(n)), "ProgramCL code object")
+        return buf.raw[:n.value]
 
     def kernel_handle(self, name):
         if name not in self._kernels:

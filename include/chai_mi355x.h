@@ -401,6 +401,11 @@ typedef struct chai_cl_arg_info {
  * the process ends. */
 int chai_hip_cl_build(const char* text, size_t len, const char* options, void** program);
 int chai_hip_cl_build_log(void* program, char* buf, size_t len);
+/* The program's gfx950 executable code object: what the OpenCL build of
+ * OCLkernel::buildCL (vendor/OCLdevice.cpp:940-968) produces and keeps inside
+ * the OpenCL runtime.  *size receives its length; up to len bytes are copied
+ * to buf when buf is not NULL.  Needs no device. */
+int chai_hip_cl_code_object(void* program, void* buf, size_t len, size_t* size);
 /* explicit arguments of kernel `name` (ProgramCL::parseKernelArgs,
  * api/ProgramCL.cpp:37-131, from metadata instead of the text) */
 int chai_hip_cl_kernel_args(void* program, const char* name, int* num_args, chai_cl_arg_info* info, int max_info);

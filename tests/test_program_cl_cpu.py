@@ -4,6 +4,10 @@ code object's metadata with ProgramCL's argument kinds, compile errors come
 back with the compiler log, launches are checked before anything runs, and
 traces with inline kernel statements compile into separate kernels."""
 import ctypes
+import os
+import re
+import subprocess
+import tempfile
 
 import pytest
 
@@ -141,3 +145,30 @@ def test_trace_inline_kernel_needs_its_first_array_as_lhs():
     t.vars.append((F32, tr.TEMP, 20, 1, 1))
     with pytest.raises(ChaiError, match="first array argument"):
         tr.emit_source(t, 0)
+
+
+# two independent scalar multiply-adds on neighbouring elements: at -O3 the
+# gfx950 back end pairs such float code into packed FP32 instructions unless
+# the packed-fp32-ops target feature is off
+PAIRED_FMA = """
+__kernel void axpb2(__global float* a, float s, float t)
+{
+    size_t i = get_global_id(0);
+    a[2 * i] = a[2 * i] * s + t;
+    a[2 * i + 1] = a[2 * i + 1] * s + t;
+}
+"""
+OBJDUMP = "/opt/rocm/llvm/bin/llvm-objdump"
+
+
+@pytest.mark.skipif(not os.path.exists(OBJDUMP), reason="llvm-objdump not installed")
+def test_inline_kernels_carry_no_packed_fp32_instructions():
+    code = pcl.ProgramCL(PAIRED_FMA).code_object()
+    assert code[:4] == b"\x7fELF"
+    with tempfile.NamedTemporaryFile(suffix=".co") as f:
+        f.write(code)
+        f.flush()
+        asm = subprocess.run([OBJDUMP, "-d", "--mcpu=gfx950", f.name], capture_output=True, text=True,
+                             check=True).stdout
+    assert re.search(r"\bv_(fma|mul|add)_f32", asm), "kernel body not found in the disassembly"
+    assert not re.search(r"\bv_pk_(fma|mul|add)_f32", asm), asm
