@@ -35,6 +35,8 @@ void chai_or_threefry4x32_20(const uint32_t ctr[4], const uint32_t key[4], uint3
 void chai_or_threefry4x32_20_r123(const uint32_t ctr[4], const uint32_t key[4], uint32_t out[4]);
 void chai_or_threefry2x32_20(const uint32_t ctr[2], const uint32_t key[2], uint32_t out[2]);
 void chai_or_threefry2x64_20(const uint64_t ctr[2], const uint64_t key[2], uint64_t out[2]);
+/* raw words of one stream vector (key / counter layout of the stream) */
+int chai_or_rng_words(int variant, int pr, int vl, uint64_t vidx, uint64_t seed, uint64_t* out);
 
 /* ---- canonical Box-Muller math (double, no contraction, explicit fma) */
 double chai_or_log(double x);

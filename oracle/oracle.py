@@ -84,6 +84,34 @@ def ref_lib():
     return L
 
 
+_R123 = os.path.join(_HERE, "_ref", "libchai_r123.so")
+
+
+def r123_lib():
+    """oracle/_ref/libchai_r123.so (the reference's emitted Random123 OpenCL
+    text compiled for the host), or None if not built."""
+    if not os.path.exists(_R123):
+        return None
+    L = ctypes.CDLL(_R123)
+    vp = ctypes.c_void_p
+    for name in ("chai_ref_philox4x32", "chai_ref_philox2x32", "chai_ref_threefry4x32",
+                 "chai_ref_threefry2x32", "chai_ref_threefry2x64"):
+        getattr(L, name).argtypes = [vp, vp, vp]
+    L.chai_ref_stream_words.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, vp]
+    return L
+
+
+def rng_words(variant, pr, vl, vidx, seed):
+    """raw words of one stream vector (pr 32: vl words; pr 64: 2 words)"""
+    o = (ctypes.c_uint64 * 4)()
+    L = lib()
+    L.chai_or_rng_words.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_uint64,
+                                    ctypes.c_uint64, ctypes.c_void_p]
+    if L.chai_or_rng_words(variant, pr, vl, vidx, seed, o) != 0:
+        raise ValueError("bad stream parameters")
+    return list(o)[:(2 if pr == 64 else vl)]
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 

@@ -383,6 +383,27 @@ static void block64(int variant, uint64_t vidx, uint64_t seed, uint64_t w[2])
     }
 }
 
+/* The raw words of one generated vector: vl words of the 32-bit stream
+ * (pr = 32, vl = 1, 2, 4) or 2 words of the 64-bit stream (pr = 64) for
+ * vector index vidx and generator seed (the tid / idx arguments of the
+ * reference's *_uniform_* wrappers, Random123/R123Philox.cpp:366-381,
+ * Random123/R123Threefry.cpp:377-387). */
+int chai_or_rng_words(int variant, int pr, int vl, uint64_t vidx, uint64_t seed, uint64_t* out)
+{
+    if (variant != 1230 && variant != 1231) return -1;
+    if (pr == 64) {
+        uint64_t w[2];
+        block64(variant, vidx, seed, w);
+        out[0] = w[0]; out[1] = w[1];
+        return 0;
+    }
+    if (pr != 32 || (vl != 1 && vl != 2 && vl != 4)) return -1;
+    uint32_t w[4];
+    block32(variant, (size_t)vl, (uint32_t)vidx, (uint32_t)seed, w);
+    for (int i = 0; i < vl; i++) out[i] = w[i];
+    return 0;
+}
+
 int chai_or_rng_fill(int variant, int prec, int vec_len, uint64_t seed,
                      size_t W, size_t H, size_t slots, int normal,
                      int has_limits, double lo, double hi, void* out)
