@@ -11,6 +11,8 @@
 #include "common.hpp"
 #include "reduce_device.hpp"
 
+#include <mutex>
+
 namespace chai_hip {
 
 constexpr int kRedThreads = 256;
@@ -116,6 +118,19 @@ static uint32_t pick_blocks(uint64_t n, size_t slots)
     return (uint32_t)want;
 }
 
+// The two launches of one reduction over a caller-provided partials buffer
+// of at least bps * slots doubles.
+template <typename T>
+static hipError_t launch_reduce(int op, const void* in, size_t n, size_t slots, uint32_t bps,
+                                double* partial, void* out, double* out_f64, hipStream_t st)
+{
+    dim3 grid(bps, (unsigned)slots);
+    reduce_partial<T, std::is_integral<T>::value><<<grid, kRedThreads, 0, st>>>((const T*)in, n, bps, partial);
+    const double div = op == CHAI_OP_MEAN ? (double)n : 1.0;
+    reduce_final<T><<<(unsigned)slots, kRedThreads, 0, st>>>(partial, bps, div, (T*)out, out_f64);
+    return hipGetLastError();
+}
+
 template <typename T>
 static int run_reduce(int op, const void* in, size_t n, size_t slots, void* out, double* out_f64,
                       hipStream_t st)
@@ -123,13 +138,83 @@ static int run_reduce(int op, const void* in, size_t n, size_t slots, void* out,
     const uint32_t bps = pick_blocks(n, slots);
     double* partial = nullptr;
     CHAI_HIP_TRY(hipMallocAsync((void**)&partial, sizeof(double) * bps * slots, st));
-    dim3 grid(bps, (unsigned)slots);
-    reduce_partial<T, std::is_integral<T>::value><<<grid, kRedThreads, 0, st>>>((const T*)in, n, bps, partial);
-    const double div = op == CHAI_OP_MEAN ? (double)n : 1.0;
-    reduce_final<T><<<(unsigned)slots, kRedThreads, 0, st>>>(partial, bps, div, (T*)out, out_f64);
-    hipError_t e = hipGetLastError();
+    hipError_t e = launch_reduce<T>(op, in, n, slots, bps, partial, out, out_f64, st);
     CHAI_HIP_TRY(hipFreeAsync(partial, st));
     if (e != hipSuccess) return hip_fail(e, "reduce launch");
+    return CHAI_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reduction without allocation.  The block partials live in a file-scope
+// device array (one copy per GPU, held until the process ends).  Host code
+// never passes the array's name to a kernel: on the host that name is only a
+// stand-in, so each device's address is looked up once with
+// hipGetSymbolAddress (at chai_hip_init, or on the first call on a device
+// that init did not set up) and kept per device.  Every partial is written by
+// reduce_partial before reduce_final reads it, so nothing needs clearing
+// between calls; calls that overlap on one GPU share the array and race.
+constexpr size_t kNoAllocPartials = 65536;   // 512 KiB of FP64 partials
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) double scratch[kNoAllocPartials];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];
+
+int reduce_scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) {
+        set_error("reduce scratch: device %d beyond the %d supported", device, kMaxDevices);
+        return CHAI_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return CHAI_OK;
+    int cur = -1;
+    CHAI_HIP_TRY(hipGetDevice(&cur));
+    if (cur != device) CHAI_HIP_TRY(hipSetDevice(device));
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(reduce scratch)");
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+    return CHAI_OK;
+}
+
+static int scratch_for_current_device(double** out)
+{
+    int dev = -1;
+    CHAI_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("reduce scratch: device %d beyond the %d supported", dev, kMaxDevices);
+        return CHAI_ERR_ARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (g_scratch_ptr[dev]) { *out = g_scratch_ptr[dev]; return CHAI_OK; }
+    }
+    int rc = reduce_scratch_setup(dev);
+    if (rc != CHAI_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    *out = g_scratch_ptr[dev];
+    return CHAI_OK;
+}
+
+template <typename T>
+static int run_reduce_noalloc(int op, const void* in, size_t n, size_t slots, void* out,
+                              double* out_f64, hipStream_t st)
+{
+    const uint32_t bps = pick_blocks(n, slots);
+    if ((uint64_t)bps * slots > kNoAllocPartials) {
+        set_error("reduce_noalloc: %zu slots of %zu elements need %llu FP64 partials; the scratch "
+                  "array holds %zu (up to 256 slots always fit)", slots, n,
+                  (unsigned long long)bps * slots, kNoAllocPartials);
+        return CHAI_ERR_ARG;
+    }
+    double* partial = nullptr;
+    int rc = scratch_for_current_device(&partial);
+    if (rc != CHAI_OK) return rc;
+    hipError_t e = launch_reduce<T>(op, in, n, slots, bps, partial, out, out_f64, st);
+    if (e != hipSuccess) return hip_fail(e, "reduce_noalloc launch");
     return CHAI_OK;
 }
 
@@ -137,20 +222,42 @@ static int run_reduce(int op, const void* in, size_t n, size_t slots, void* out,
 
 using namespace chai_hip;
 
+static int check_reduce_args(const char* who, int op, int prec, const void* in, size_t n, size_t slots,
+                             const void* out)
+{
+    if (op != CHAI_OP_SUM && op != CHAI_OP_MEAN) { set_error("%s: op %d is not sum/mean", who, op); return CHAI_ERR_ARG; }
+    if (!valid_prec(prec)) { set_error("%s: bad precision %d", who, prec); return CHAI_ERR_ARG; }
+    if (slots == 0) return CHAI_OK;
+    if (n == 0) { set_error("%s: empty array", who); return CHAI_ERR_ARG; }
+    if (slots > 65535) { set_error("%s: at most 65535 slots per call", who); return CHAI_ERR_ARG; }
+    if (!in || !out) { set_error("%s: null pointer", who); return CHAI_ERR_ARG; }
+    return CHAI_OK;
+}
+
 extern "C" int chai_hip_reduce(int op, int prec, const void* in, size_t n, size_t slots, void* out,
                                double* out_f64, void* stream)
 {
-    if (op != CHAI_OP_SUM && op != CHAI_OP_MEAN) { set_error("reduce: op %d is not sum/mean", op); return CHAI_ERR_ARG; }
-    if (!valid_prec(prec)) { set_error("reduce: bad precision %d", prec); return CHAI_ERR_ARG; }
-    if (slots == 0) return CHAI_OK;
-    if (n == 0) { set_error("reduce: empty array"); return CHAI_ERR_ARG; }
-    if (slots > 65535) { set_error("reduce: at most 65535 slots per call"); return CHAI_ERR_ARG; }
-    if (!in || !out) { set_error("reduce: null pointer"); return CHAI_ERR_ARG; }
+    int rc = check_reduce_args("reduce", op, prec, in, n, slots, out);
+    if (rc != CHAI_OK || slots == 0) return rc;
     hipStream_t st = as_stream(stream);
     switch (prec) {
         case CHAI_U32: return run_reduce<uint32_t>(op, in, n, slots, out, out_f64, st);
         case CHAI_I32: return run_reduce<int32_t>(op, in, n, slots, out, out_f64, st);
         case CHAI_F32: return run_reduce<float>(op, in, n, slots, out, out_f64, st);
         default: return run_reduce<double>(op, in, n, slots, out, out_f64, st);
+    }
+}
+
+extern "C" int chai_hip_reduce_noalloc(int op, int prec, const void* in, size_t n, size_t slots, void* out,
+                                       double* out_f64, void* stream)
+{
+    int rc = check_reduce_args("reduce_noalloc", op, prec, in, n, slots, out);
+    if (rc != CHAI_OK || slots == 0) return rc;
+    hipStream_t st = as_stream(stream);
+    switch (prec) {
+        case CHAI_U32: return run_reduce_noalloc<uint32_t>(op, in, n, slots, out, out_f64, st);
+        case CHAI_I32: return run_reduce_noalloc<int32_t>(op, in, n, slots, out, out_f64, st);
+        case CHAI_F32: return run_reduce_noalloc<float>(op, in, n, slots, out, out_f64, st);
+        default: return run_reduce_noalloc<double>(op, in, n, slots, out, out_f64, st);
     }
 }

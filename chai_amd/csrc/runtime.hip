@@ -8,6 +8,8 @@
 
 namespace chai_hip {
 
+int reduce_scratch_setup(int device);   // reduce.hip
+
 static thread_local char g_err[512];
 
 void set_error(const char* fmt, ...)
@@ -52,6 +54,9 @@ int chai_hip_init(int device_ordinal)
                   device_ordinal, prop.gcnArchName);
         return CHAI_ERR_STATE;
     }
+    // device address of the no-allocation reduction's scratch array
+    int rc = reduce_scratch_setup(device_ordinal);
+    if (rc != CHAI_OK) return rc;
     g_device = device_ordinal;
     return CHAI_OK;
 }

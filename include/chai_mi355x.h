@@ -108,6 +108,20 @@ int chai_hip_rng_fill(int variant, int prec, int vec_len, uint64_t seed,
 int chai_hip_reduce(int op, int prec, const void* in, size_t n, size_t slots,
                     void* out, double* out_f64, void* stream);
 
+/* chai_hip_reduce without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The FP64
+ * block partials live in a fixed device array of 65536 doubles (512 KiB per
+ * GPU, held until the process ends).  A call needs blocks x slots of them,
+ * with ceil(n / 4096) blocks per slot capped at 2048 / min(slots, 8): any
+ * call with up to 256 slots fits, and a call that does not fit returns
+ * CHAI_ERR_ARG before anything is launched.  Same kernels and
+ * block counts as chai_hip_reduce, so results are identical to it.  Calls
+ * on one GPU share the array: calls that overlap in time on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller. */
+int chai_hip_reduce_noalloc(int op, int prec, const void* in, size_t n, size_t slots,
+                            void* out, double* out_f64, void* stream);
+
 /* ------------------------------------------------------------------- GEMM */
 /* Kernel family selector.  AUTO picks BF16X3 for f32/int operands with an f32
  * result and F64 when any operand is f64. */
