@@ -530,6 +530,7 @@ struct Batcher {
     int variant = -1, use_graph = 1;
     hipStream_t stream = nullptr;
     std::vector<chai_ragged_item> pending;
+    std::vector<hipEvent_t> pending_ev;   // per pending item: its producer's event
     uint64_t gen = 0;              // generation of the pending batch
     std::vector<int> result;       // status of each finished generation
     // cached plan of the last batch, keyed by its items
@@ -550,10 +551,15 @@ bool same_items(const std::vector<chai_ragged_item>& a, const std::vector<chai_r
            (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(chai_ragged_item)) == 0);
 }
 
-// Runs one batch; called with run_mu held and mu released.
-int run_batch(Batcher* B, std::vector<chai_ragged_item>& items)
+// Runs one batch; called with run_mu held and mu released.  The batch's
+// stream first waits for the event each submitter recorded on the stream
+// that produced its operands.
+int run_batch(Batcher* B, std::vector<chai_ragged_item>& items, const std::vector<hipEvent_t>& events)
 {
     int rc = CHAI_OK;
+    hipEvent_t last = nullptr;
+    for (hipEvent_t ev : events)
+        if (ev != last) { CHAI_HIP_TRY(hipStreamWaitEvent(B->stream, ev, 0)); last = ev; }
     if (!B->plan || !same_items(items, B->plan_items)) {
         if (B->plan) { chai_hip_ragged_release(B->plan); B->plan = nullptr; }
         if (B->variant < 0) {
@@ -582,13 +588,15 @@ int run_batch(Batcher* B, std::vector<chai_ragged_item>& items)
 int flush_locked(Batcher* B, std::unique_lock<std::mutex>& lk)
 {
     std::vector<chai_ragged_item> items;
+    std::vector<hipEvent_t> events;
     items.swap(B->pending);
+    events.swap(B->pending_ev);
     const uint64_t g = B->gen++;
     lk.unlock();
     int rc;
     {
         std::lock_guard<std::mutex> rl(B->run_mu);
-        rc = run_batch(B, items);
+        rc = run_batch(B, items, events);
     }
     lk.lock();
     if (B->result.size() <= g) B->result.resize(g + 1, 1);
@@ -618,16 +626,22 @@ int chai_hip_batcher_create(size_t expected, const chai_epilogue* epi, int varia
     return CHAI_OK;
 }
 
-int chai_hip_batcher_submit(void* batcher, const chai_ragged_item* items, size_t count)
+int chai_hip_batcher_submit(void* batcher, const chai_ragged_item* items, size_t count, void* stream)
 {
     if (!batcher || (count && !items)) { set_error("batcher_submit: null argument"); return CHAI_ERR_ARG; }
     auto* B = (Batcher*)batcher;
-    std::unique_lock<std::mutex> lk(B->mu);
     if (count == 0) return CHAI_OK;
+    // order the batch after the work already queued on the producer's stream
+    hipEvent_t ev = nullptr;
+    CHAI_HIP_TRY(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+    hipError_t e = hipEventRecord(ev, as_stream(stream));
+    if (e != hipSuccess) { (void)hipEventDestroy(ev); return hip_fail(e, "batcher_submit: event record"); }
+    std::unique_lock<std::mutex> lk(B->mu);
     const uint64_t first = B->gen;
     uint64_t last = first;
     for (size_t i = 0; i < count; i++) {
         B->pending.push_back(items[i]);
+        B->pending_ev.push_back(ev);
         last = B->gen;
         if (B->pending.size() >= B->expected) flush_locked(B, lk);
     }
@@ -647,6 +661,9 @@ int chai_hip_batcher_submit(void* batcher, const chai_ragged_item* items, size_t
     int rc = CHAI_OK;
     for (uint64_t g = first; g <= last; g++)
         if (B->result[g] != CHAI_OK) rc = B->result[g];
+    lk.unlock();
+    // every batch holding these items has finished (run_batch synchronises)
+    (void)hipEventDestroy(ev);
     return rc;
 }
 

@@ -115,6 +115,106 @@ __global__ void count_mismatch(const T* a, const T* b, uint64_t n, unsigned int*
     if (c) atomicAdd(out, c);
 }
 
+// Independent check of the first variant a search runs (the reference the
+// later ones are compared with), in the spirit of MatmulMM::checkOutput's
+// paranoid mode (autotune/EvergreenMatmulMM.cpp:909-1075): up to four sampled
+// rows of up to two instances are recomputed in FP64 from A and B, and each
+// element of C must lie within the family's error bound of that value:
+//   FMA     |C - C64| <= (K + 2) 2^-24 S
+//   BF16X3  |C - C64| <= (3 2^-16 + (3K + 2) 2^-24) S
+//   F64     |C - C64| <= (3K + 4) 2^-53 S
+// with S = sum_k |a_ik b_kj| (plus K 2^-126 for underflow).  Non-finite
+// values must match (NaN where the FP64 value is NaN, the same infinity).
+template <typename T>
+__device__ __forceinline__ double load_as_f64(const void* p, uint64_t i)
+{
+    return (double)static_cast<const T*>(p)[i];
+}
+
+__device__ __forceinline__ double load_prec(const void* p, int prec, uint64_t i)
+{
+    switch (prec) {
+        case CHAI_U32: return load_as_f64<uint32_t>(p, i);
+        case CHAI_I32: return load_as_f64<int32_t>(p, i);
+        case CHAI_F32: return load_as_f64<float>(p, i);
+        default: return load_as_f64<double>(p, i);
+    }
+}
+
+struct CheckShape {
+    int prec_a, prec_b, prec_c, trans_a, trans_b;
+    uint64_t M, N, K, lda, ldb, ldc, stride_a, stride_b, stride_c;
+    int same_a, same_b;
+    uint32_t rows[4];
+    uint32_t insts[2];
+    double rel;            // relative bound coefficient (times S)
+};
+
+__global__ void validate_rows(CheckShape s, const void* A, const void* B, const void* C, unsigned int* bad)
+{
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= s.N) return;
+    const uint64_t i = s.rows[blockIdx.y];
+    const uint64_t b = s.insts[blockIdx.z];
+    const uint64_t oa = s.same_a ? 0 : b * s.stride_a, ob = s.same_b ? 0 : b * s.stride_b;
+    double acc = 0.0, mag = 0.0;
+    for (uint64_t k = 0; k < s.K; k++) {
+        const double a = load_prec(A, s.prec_a, oa + (s.trans_a ? k * s.lda + i : i * s.lda + k));
+        const double bv = load_prec(B, s.prec_b, ob + (s.trans_b ? j * s.ldb + k : k * s.ldb + j));
+        acc = fma(a, bv, acc);
+        mag = fma(fabs(a), fabs(bv), mag);
+    }
+    const double c = load_prec(C, s.prec_c, b * s.stride_c + i * s.ldc + j);
+    bool ok;
+    if (!isfinite(acc) || !isfinite(c)) ok = (c == acc) || (c != c && acc != acc) || (!isfinite(mag) && c != c);
+    else ok = fabs(c - acc) <= s.rel * mag + (double)s.K * 0x1p-126;
+    if (!ok) atomicAdd(bad, 1u);
+}
+
+// 1: C is within the bound on every sampled element, 0: not, <0: error
+static int validate_output(const chai_gemm_desc& d, int mode, const void* A, const void* B, const void* C,
+                           hipStream_t st)
+{
+    if (d.prec_c != CHAI_F32 && d.prec_c != CHAI_F64) return 1;
+    CheckShape s{};
+    s.prec_a = d.prec_a; s.prec_b = d.prec_b; s.prec_c = d.prec_c;
+    s.trans_a = d.trans_a; s.trans_b = d.trans_b;
+    s.M = d.M; s.N = d.N; s.K = d.K; s.lda = d.lda; s.ldb = d.ldb; s.ldc = d.ldc;
+    s.stride_a = d.stride_a; s.stride_b = d.stride_b; s.stride_c = d.stride_c;
+    s.same_a = d.same_a; s.same_b = d.same_b;
+    const double K = (double)d.K;
+    s.rel = mode == CHAI_GEMM_F64 ? (3 * K + 4) * 0x1p-53
+          : mode == CHAI_GEMM_FMA ? (K + 2) * 0x1p-24
+                                  : 3 * 0x1p-16 + (3 * K + 2) * 0x1p-24;
+    const uint64_t cand[4] = {0, d.M / 3, (2 * d.M) / 3, d.M - 1};
+    unsigned nr = 0;
+    for (uint64_t r : cand) {
+        bool dup = false;
+        for (unsigned q = 0; q < nr; q++) dup |= s.rows[q] == r;
+        if (!dup) s.rows[nr++] = (uint32_t)r;
+    }
+    const uint64_t nb = d.batch ? d.batch : 1;
+    s.insts[0] = 0;
+    s.insts[1] = (uint32_t)(nb - 1);
+    const unsigned ni = nb > 1 ? 2 : 1;
+    unsigned int* cnt = nullptr;
+    if (hipMallocAsync((void**)&cnt, sizeof(unsigned int), st) != hipSuccess) {
+        set_error("gemm_tune: no memory for the FP64 check");
+        return CHAI_ERR_NOMEM;
+    }
+    unsigned int h = 0;
+    bool okrun = hipMemsetAsync(cnt, 0, sizeof(unsigned int), st) == hipSuccess;
+    if (okrun) {
+        validate_rows<<<dim3(ceil_div_u(d.N, 256), nr, ni), 256, 0, st>>>(s, A, B, C, cnt);
+        okrun = hipGetLastError() == hipSuccess &&
+                hipMemcpyAsync(&h, cnt, sizeof h, hipMemcpyDeviceToHost, st) == hipSuccess &&
+                hipStreamSynchronize(st) == hipSuccess;
+    }
+    (void)hipFreeAsync(cnt, st);
+    if (!okrun) { set_error("gemm_tune: FP64 check failed to run"); return CHAI_ERR_HIP; }
+    return h == 0 ? 1 : 0;
+}
+
 std::mutex g_mu;
 Memo g_memo;
 std::map<std::string, int> g_best;            // exogenous key -> variant (JustInTimeMemo)
@@ -258,13 +358,14 @@ int chai_hip_gemm_time_variant(const chai_gemm_desc* desc, const void* A, const 
 }
 
 // The output check of a tuning run (AutoTuneBench::runTime -> checkOutput,
-// autotune/AutoTuneBench.cpp:17-251).  Every tile variant of a mode
-// accumulates each output element in the same k order, so all of them must
-// give the same C: the first variant run in this search is the reference and
-// every later one is compared with it by value, journalled as BEFORE_CHECK ->
-// AFTER_CHECK, or BEFORE_CHECK -> BAD_OUTPUT (blacklisted, also after a
-// reload).  alpha A B + beta C reads C, which the timed runs overwrite, so
-// general GEMMs are not checked.
+// autotune/AutoTuneBench.cpp:17-251).  Every variant that runs is journalled
+// BEFORE_CHECK -> AFTER_CHECK, or BEFORE_CHECK -> BAD_OUTPUT (blacklisted,
+// also after a reload).  The first variant run must pass the FP64 check of
+// sampled rows (validate_output); it then becomes the reference.  Every tile
+// variant of a mode accumulates each output element in the same k order, so
+// every later one must give the reference's C, compared by value.
+// alpha A B + beta C reads C, which the timed runs overwrite, so general
+// GEMMs are not checked.
 struct OutputCheck {
     void* ref = nullptr;
     uint64_t n = 0;
@@ -320,7 +421,7 @@ int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B,
     if (!desc) { set_error("gemm_tune: null descriptor"); return CHAI_ERR_ARG; }
     const int mode = mode_of(*desc);
     const int nv = gemm_num_variants(mode);
-    const bool checked = !desc->general && desc->M && desc->N && C && mode != CHAI_GEMM_FMA;
+    const bool checked = !desc->general && desc->M && desc->N && C;
     hipStream_t st = as_stream(stream);
     OutputCheck ck;
     int bv = -1;
@@ -337,16 +438,19 @@ int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B,
         }
         const int rc = chai_hip_gemm_time_variant(desc, A, B, C, v, trials, &ms, stream);
         if (rc != CHAI_OK && rc != CHAI_ERR_ARG) return rc;
-        if (ms > 0.f && ran && checked && !ck.ref) {
-            const int rc2 = check_output(ck, *desc, C, st);    // the first variant run: the reference
-            if (rc2 < 0) return rc2;
-        } else if (ms > 0.f && ran && checked) {
+        if (ms > 0.f && ran && checked) {
             chai_gemm_desc d = *desc;
             d.mode = mode;
             const std::string name = kernel_name(d, mode, v);
             { std::lock_guard<std::mutex> lk(g_mu); g_memo.add(name, BEFORE_CHECK); }
-            const int ok = check_output(ck, *desc, C, st);
+            // the first variant that passes the FP64 check becomes the
+            // reference; every later one must equal it
+            int ok = validate_output(d, mode, A, B, C, st);
             if (ok < 0) return ok;
+            if (ok) {
+                ok = check_output(ck, *desc, C, st);
+                if (ok < 0) return ok;
+            }
             std::lock_guard<std::mutex> lk(g_mu);
             g_memo.add(name, ok ? AFTER_CHECK : BAD_OUTPUT);
             if (!ok) ms = -1.f;

@@ -47,7 +47,7 @@ def _bind():
     L.chai_hip_ragged_tune.argtypes = [P(RaggedItem), _sz, P(Epilogue), ctypes.c_int, P(ctypes.c_int),
                                        P(ctypes.c_float), _vp]
     L.chai_hip_batcher_create.argtypes = [_sz, P(Epilogue), ctypes.c_int, ctypes.c_int, ctypes.c_int, P(_vp)]
-    L.chai_hip_batcher_submit.argtypes = [_vp, P(RaggedItem), _sz]
+    L.chai_hip_batcher_submit.argtypes = [_vp, P(RaggedItem), _sz, _vp]
     L.chai_hip_batcher_flush.argtypes = [_vp]
     L.chai_hip_batcher_stats.argtypes = [_vp, P(ctypes.c_uint64)]
     L.chai_hip_batcher_destroy.argtypes = [_vp]
@@ -123,12 +123,16 @@ class Batcher:
                                         int(graph), timeout_ms, ctypes.byref(h)), "chai_hip_batcher_create")
         self._h = h
 
-    def submit(self, items):
-        """Queue one item or a list of items (one thread's trace) and wait for them."""
+    def submit(self, items, stream=None):
+        """Queue one item or a list of items (one thread's trace) and wait for
+        them.  The batch runs after the work queued so far on `stream` (a
+        torch stream; default: the calling thread's current stream), which
+        is expected to produce the items' operands."""
         if isinstance(items, RaggedItem):
             items = [items]
         arr, n = _items_array(list(items))
-        check(_bind().chai_hip_batcher_submit(self._h, arr, n), "chai_hip_batcher_submit")
+        s = _vp(stream.cuda_stream) if stream is not None else _stream()
+        check(_bind().chai_hip_batcher_submit(self._h, arr, n, s), "chai_hip_batcher_submit")
 
     def flush(self):
         check(_bind().chai_hip_batcher_flush(self._h), "chai_hip_batcher_flush")

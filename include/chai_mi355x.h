@@ -229,8 +229,12 @@ int chai_hip_ragged_tune(const chai_ragged_item* items, size_t count, const chai
  * replays its hipGraph.  variant -1 tunes the tile variant on the first batch. */
 int chai_hip_batcher_create(size_t expected, const chai_epilogue* epi, int variant, int use_graph,
                             int timeout_ms, void** batcher);
-/* queue `count` items (one host thread's trace) and wait for them */
-int chai_hip_batcher_submit(void* batcher, const chai_ragged_item* items, size_t count);
+/* Queue `count` items (one host thread's trace) and wait for them.  `stream`
+ * is the stream whose queued work produces the items' A and B (NULL = the
+ * null stream): the submit records an event on it and the batch's own stream
+ * waits for that event before the GEMMs read the operands, so a submitter
+ * need not synchronise its producer first. */
+int chai_hip_batcher_submit(void* batcher, const chai_ragged_item* items, size_t count, void* stream);
 /* run whatever is queued now (for a last, partial batch) */
 int chai_hip_batcher_flush(void* batcher);
 /* stats[0..4]: batches run, items run, plans built, graph replays, variant */

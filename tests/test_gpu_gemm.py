@@ -265,9 +265,10 @@ def test_tuner_journal_crash_resume(chai, tmp_path):
     assert all(by_v[v] for v in range(nv))
     times = {v: next(int(r[2]) for r in by_v[v] if int(r[2]) > 0) for v in range(nv)}
     assert all(t > 0 for t in times.values())
-    # output checks: the first variant run is the reference, every other one is
-    # compared with it (BEFORE_CHECK -100, AFTER_CHECK -101) and none differs
-    assert all([r[2] for r in by_v[v]][-2:] == ["-100", "-101"] for v in range(1, nv))
+    # output checks (BEFORE_CHECK -100, AFTER_CHECK -101): the first variant run
+    # passes the FP64 check of sampled rows and becomes the reference, every
+    # other one is compared with it, and none differs
+    assert all([r[2] for r in by_v[v]][-2:] == ["-100", "-101"] for v in range(nv))
     assert not any(r[2] == "-7" for r in lines)
     crashed = tmp_path / "journal1.AMD"
     with open(crashed, "w") as f:
@@ -312,4 +313,47 @@ def test_tuner_checks_every_variant_against_the_first(chai, tmp_path, mode):
     lines = [l.split() for l in j.read_text().split("\n") if l.strip()]
     states = [r[2] for r in lines]
     assert "-7" not in states
-    assert states.count("-101") == len(chai.gemm_variants(m)) - 1
+    assert states.count("-101") == len(chai.gemm_variants(m))
+
+
+def test_tuner_skips_a_variant_journalled_bad_output(chai, tmp_path):
+    """A variant whose output check failed in an earlier run (BEFORE_CHECK ->
+    BAD_OUTPUT in the journal, autotune/AutoTuneMemo.cpp:101-188) stays
+    blacklisted after a reload: the search neither runs nor picks it."""
+    import torch
+    n = 384
+    A, B = torch.rand(n, n, device="cuda"), torch.rand(n, n, device="cuda")
+    C = torch.empty(n, n, device="cuda")
+    first = tmp_path / "journal0.AMD"
+    chai.memo_close()
+    chai.memo_open(first, 0)
+    try:
+        v1, _ = chai.gemm_tune(A, B, C, trials=1)
+    finally:
+        chai.memo_close()
+    lines = [l.split() for l in first.read_text().split("\n") if l.strip()]
+    name = next(r[1] for r in lines if r[1].endswith("_v%d" % v1))
+    # the winner's last check failed: its times are dropped, BAD_OUTPUT follows BEFORE_CHECK
+    j = tmp_path / "journal1.AMD"
+    with open(j, "w") as f:
+        for r in lines:
+            if r[1] == name and int(r[2]) > 0:
+                continue
+            if r[1] == name and r[2] == "-101":
+                f.write("%s %s -7\n" % (r[0], r[1]))
+                continue
+            f.write(" ".join(r) + "\n")
+    chai.memo_open(j, 0)
+    try:
+        assert chai.memo_status(name)[0] == -1
+        before = j.read_text()
+        v2, _ = chai.gemm_tune(A, B, C, trials=1)
+        assert v2 != v1
+        assert j.read_text() == before            # nothing re-run, the bad variant included
+    finally:
+        chai.memo_close()
+    chai.memo_open(j, 0)                          # and it is still bad after another reload
+    try:
+        assert chai.memo_status(name)[0] == -1
+    finally:
+        chai.memo_close()

@@ -228,3 +228,42 @@ def test_batcher_timeout_runs_partial_batch(chai, oracle):
         assert torch.equal(C, chai.gemm(A, B, mode=chai.GEMM_BF16X3, variant=2))
     assert b.stats()["batches"] == 1
     b.close()
+
+
+def test_batcher_waits_for_the_producer_stream(chai, oracle):
+    """Submitters write their operands on their own streams and submit at once,
+    without a host synchronize: the batch must run after that work.  The copy
+    into A queues behind a spin kernel, so a batch that did not wait would
+    read the NaN that A holds before the copy."""
+    from chai_amd import ragged
+    nthreads = 4
+    batch = make_batch(oracle, [(96, 80, 64), (128, 128, 128), (64, 200, 48), (256, 64, 96)], seed0=5000)
+    src = [A.clone() for A, _, _ in batch]
+    for A, _, C in batch:
+        A.fill_(float("nan"))
+        C.fill_(float("nan"))
+    torch.cuda.synchronize()
+    b = ragged.Batcher(nthreads, variant=2, graph=False)
+    errors = []
+
+    def worker(t):
+        try:
+            s = torch.cuda.Stream()
+            with torch.cuda.stream(s):
+                torch.cuda._sleep(20_000_000)          # tens of ms of device time before the copy
+                batch[t][0].copy_(src[t])
+            b.submit([ragged.item(*batch[t])], stream=s)
+        except Exception as e:   # noqa: BLE001 - reported below
+            errors.append(e)
+
+    th = [threading.Thread(target=worker, args=(t,)) for t in range(nthreads)]
+    for x in th:
+        x.start()
+    for x in th:
+        x.join()
+    assert not errors, errors
+    torch.cuda.synchronize()
+    for (A, B, C), s in zip(batch, src):
+        assert torch.equal(A, s)
+        assert torch.equal(C, chai.gemm(A, B, mode=chai.GEMM_BF16X3, variant=2))
+    b.close()
