@@ -2,7 +2,9 @@
 
 Default workload (what a plain `python bench.py` measures): FP32 GEMM
 M = N = K = 8192 on one MI355X, split-BF16x3 MFMA vs the plain FP32-FMA kernel,
-errors against an FP64 reference.  One "step" = one chai_hip_gemm call in
+errors against an FP64 reference; then, in the same run and line ("legs"), the
+other configurations each with a roofline and a CPU baseline (--no-legs skips
+them).  One "step" = one chai_hip_gemm call in
 split-BF16x3 mode on resident inputs (operand split pre-pass + MFMA kernel).
 A and B are uniform[-1,1] float32 from Chai's Philox4x32-10 stream (one
 generator, seed 42: A then B, the seed advancing by W*H between the two calls
@@ -51,6 +53,10 @@ def parse():
     p.add_argument("--cpu-size", type=int, default=1024,
                    help="edge of the interpreter-restatement sample timed on one host core")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--cpu-budget", type=float, default=25.0,
+                   help="seconds of host work for a sampled CPU baseline")
+    p.add_argument("--no-legs", action="store_true",
+                   help="default workload: only the headline GEMM, not the other configurations' legs")
     p.add_argument("--err-rows", type=int, default=16)
     p.add_argument("--train-shapes", type=int, default=2000)
     p.add_argument("--test-shapes", type=int, default=500)
@@ -113,9 +119,51 @@ def cpu_baseline_gemm(n):
     t0 = time.perf_counter()
     o.matmul_interp(A, B)
     dt = time.perf_counter() - t0
-    return {"value": 2.0 * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-            "sample": "interpreter-semantics f32 matmul %d^3 (uniform[-1,1], seed 42), 1 core, %.1f s"
-                      % (n, dt)}
+    out = {"value": 2.0 * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+           "sample": "the oracle's restatement of the interpreter: f32 matmul %d^3 (uniform[-1,1], seed 42), "
+                     "1 core, %.1f s" % (n, dt)}
+    out.update(host_info())
+    return out
+
+
+def host_info():
+    """nproc and CPU model of the machine the CPU baseline ran on"""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for l in f:
+                if l.startswith("model name"):
+                    model = l.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "cpu_model": model}
+
+
+def cpu_reference_gemm(n):
+    """The reference's own interpreter (compiled from its sources into
+    oracle/_ref by oracle/frontend.mk and tests/frontend/Makefile) timed on one
+    host core: C = matmul(A, B) on n x n f32 through chai.h, from the
+    statements to the readout that evaluates them.  None when not built."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "fe_bin", "fe_cases")
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OCL_ICD_VENDORS="/nonexistent")
+    env.pop("CHAI_MI355X", None)
+    try:
+        r = subprocess.run([exe, "mm_time", str(n)], env=env, capture_output=True, text=True, timeout=300)
+        m = re.search(r"seconds ([0-9.eE+-]+) flops ([0-9.eE+-]+)", r.stdout)
+    except (OSError, subprocess.SubprocessError):
+        return None
+    if r.returncode != 0 or not m:
+        return None
+    dt, fl = float(m.group(1)), float(m.group(2))
+    out = {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "reference",
+           "sample": "the reference's interpreter (built from its sources), f32 matmul %d^3 through chai.h, "
+                     "1 core, %.1f s" % (n, dt)}
+    out.update(host_info())
+    return out
 
 
 def read_traffic(path=PMC_SUMMARY):
@@ -389,6 +437,11 @@ def run_sgemm(args):
     ridx = torch.from_numpy(rows).to(dev)
     Cx3 = C[ridx].cpu().numpy().astype(np.float64)
 
+    # split-BF16x6 (three pieces, six products): FP32-FMA-level error at twice the MFMA work
+    ms_x6 = timed(lambda: run(chai.GEMM_BF16X6), args.fp32_steps, 1)
+    tflops_x6 = flops / (ms_x6 * 1e-3) / 1e12
+    Cx6 = C[ridx].cpu().numpy().astype(np.float64)
+
     ms_f32 = timed(lambda: run(chai.GEMM_FMA), args.fp32_steps, 1)
     tflops_f32 = flops / (ms_f32 * 1e-3) / 1e12
     Cf32 = C[ridx].cpu().numpy().astype(np.float64)
@@ -401,6 +454,11 @@ def run_sgemm(args):
            "fp32_fma_max_abs": float(ef.max()), "fp32_fma_rms": float(np.sqrt((ef ** 2).mean())),
            "ratio_max": float(e3.max() / ef.max()),
            "ratio_rms": float(np.sqrt((e3 ** 2).mean()) / np.sqrt((ef ** 2).mean()))}
+    e6 = np.abs(Cx6 - C64)
+    err["bf16x6_max_abs"] = float(e6.max())
+    err["bf16x6_rms"] = float(np.sqrt((e6 ** 2).mean()))
+    err["bf16x6_ratio_max"] = float(e6.max() / ef.max())
+    err["bf16x6_ratio_rms"] = float(np.sqrt((e6 ** 2).mean()) / np.sqrt((ef ** 2).mean()))
 
     mfma_flops = 3 * flops           # three bf16 products per FP32 multiply-add
     achieved = mfma_flops / (core_ms * 1e-3) / 1e12
@@ -415,6 +473,10 @@ def run_sgemm(args):
     line["fp32_fma"] = {"tflops": round(tflops_f32, 2), "ms": round(ms_f32, 3),
                         "frac_of_fp32_peak": round(tflops_f32 / PEAK_FP32_TFLOPS, 4)}
     line["speedup_vs_fp32"] = round(tflops_x3 / tflops_f32, 3)
+    line["bf16x6"] = {"tflops": round(tflops_x6, 2), "ms": round(ms_x6, 3),
+                      "speedup_vs_fp32": round(tflops_x6 / tflops_f32, 3),
+                      "note": "three bf16 pieces per operand, six MFMA products, two passes"}
+    line["auto_family"] = "bf16x3" if n >= 16384 else "bf16x6"
     line["split_prepass_ms"] = round(prep_ms, 4)
     line["errors_vs_fp64"] = err
     line["autotune_cold_start_s"] = cold
@@ -422,12 +484,49 @@ def run_sgemm(args):
                             "params": chai.gemm_variants()[cold["exhaustive_variant"]]}
     line["wall_s_timed"] = round(wall, 3)
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline_gemm(args.cpu_size)
+        ref = cpu_reference_gemm(768)
+        line["cpu_baseline"] = ref if ref is not None else cpu_baseline_gemm(args.cpu_size)
+        if ref is not None:
+            line["cpu_baseline_port"] = cpu_baseline_gemm(args.cpu_size)
+    if not args.no_legs:
+        del A, B, C
+        line["legs"] = run_legs(args)
     emit(line)
 
 
+LEG_DROP = ("metric", "n_gpus", "steps", "warmup", "higher_is_better", "scaling", "vs_baseline", "data")
+
+
+def run_legs(args):
+    """The other configurations measured in the same run, each with its own
+    roofline and CPU baseline: FP64 and mixed i32 x f32 GEMM 4096^3, the
+    4096-instance ragged batch, the fused MC trace with the Philox stream and
+    GEMV 16384^2, the fused elementwise stream and the gather stencil."""
+    import copy
+    import gc
+    import torch
+    la = copy.copy(args)
+    la.steps, la.warmup, la.cpu_budget = min(args.steps, 10), min(args.warmup, 2), 5.0
+    legs = {}
+    for name, fn in (("dgemm_f64_4096", lambda: run_dense(la, False, leg=True)),
+                     ("mixed_i32xf32_4096", lambda: run_dense(la, True, leg=True)),
+                     ("ragged_4096", lambda: run_ragged(la, leg=True)),
+                     ("mc_philox_gemv", lambda: run_mc(la, leg=True)),
+                     ("stream_trace_2e28", lambda: run_stream(la, leg=True)),
+                     ("gather_stencil_8192", lambda: run_gather(la, leg=True))):
+        t0 = time.perf_counter()
+        d = fn()
+        legs[name] = {k: v for k, v in d.items() if k not in LEG_DROP}
+        legs[name]["steps"] = la.steps
+        legs[name]["wall_s"] = round(time.perf_counter() - t0, 1)
+        print("# leg %s: %.1f s" % (name, time.perf_counter() - t0), file=sys.stderr, flush=True)
+        gc.collect()
+        torch.cuda.empty_cache()
+    return legs
+
+
 # ------------------------------------------------------------------ ragged
-def run_ragged(args):
+def run_ragged(args, leg=False):
     import numpy as np
     import torch
     import chai_amd as chai
@@ -488,12 +587,15 @@ def run_ragged(args):
             Ch = o.matmul_interp(Ah, Bh)
             np.maximum(Ch.astype(np.float32) * np.float32(0.5) + np.float32(1), 0)
             f_sub += 2.0 * k ** 3
-            if time.perf_counter() - t0 > 25:
+            if time.perf_counter() - t0 > args.cpu_budget:
                 break
         dt = time.perf_counter() - t0
         line["cpu_baseline"] = {"value": f_sub / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
-                                "sample": "interpreter-semantics matmul + epilogue on a random subsample "
-                                          "of the instances (%.2e flop, %.1f s, 1 core)" % (f_sub, dt)}
+                                "sample": "the oracle's interpreter-semantics matmul + epilogue on a random "
+                                          "subsample of the instances (%.2e flop, %.1f s, 1 core)" % (f_sub, dt)}
+        line["cpu_baseline"].update(host_info())
+    if leg:
+        return line
     emit(line)
 
 
@@ -557,7 +659,7 @@ def run_autotune(args):
 
 
 # ----------------------------------------------------------------------- mc
-def run_mc(args):
+def run_mc(args, leg=False):
     import torch
     import chai_amd as chai
     from chai_amd.workloads import mc_trace
@@ -605,11 +707,14 @@ def run_mc(args):
         dt = time.perf_counter() - t0
         line["cpu_baseline"] = {"value": k / dt / 1e9, "unit": "Gelem/s", "cores": 1, "kind": "port",
                                 "sample": "oracle trace evaluation of the MC trace on 2^20 elements (%.1f s)" % dt}
+        line["cpu_baseline"].update(host_info())
+    if leg:
+        return line
     emit(line)
 
 
 # ------------------------------------------------------------------ gather
-def run_gather(args):
+def run_gather(args, leg=False):
     """sample/filter.cpp at N = 8192: one fused kernel of five gather2_floor
     reads; algorithmic bytes 4 read + 4 written per element (the stencil's
     neighbours come from L2 / the same rows)"""
@@ -654,18 +759,21 @@ def run_gather(args):
         x = np.random.default_rng(0).standard_normal((1, n, n)).astype(np.float32)
         t0 = time.perf_counter()
         reps = 0
-        while time.perf_counter() - t0 < 10.0:
+        while time.perf_counter() - t0 < min(10.0, args.cpu_budget):
             o.trace_eval(tc, {Ac.index: x})
             reps += 1
         dt = (time.perf_counter() - t0) / reps
         line["cpu_baseline"] = {"value": round(8.0 * n * n / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
                                 "kind": "port", "sample": "oracle trace evaluation of the stencil on 2048x2048, "
                                                           "%d runs, %.2f s each" % (reps, dt)}
+        line["cpu_baseline"].update(host_info())
+    if leg:
+        return line
     emit(line)
 
 
 # ------------------------------------------------------------ dgemm / mixed
-def run_dense(args, mixed):
+def run_dense(args, mixed, leg=False):
     import numpy as np
     import torch
     import chai_amd as chai
@@ -710,7 +818,36 @@ def run_dense(args, mixed):
                         "mfma_products_per_mad": products}
     line["tile_variant"] = {"tuned": tv, "params": chai.gemm_variants(mode)[tv]}
     line["max_abs_err_vs_fp64"] = err
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline_dense(mixed, args.cpu_budget)
+    if leg:
+        return line
     emit(line)
+
+
+def cpu_baseline_dense(mixed, budget):
+    """The oracle's interpreter-semantics product on one host core for the
+    dense legs: f64 (DGEMM) or i32 x f32 (mixed) at 384^3"""
+    import numpy as np
+    from oracle import oracle as o
+    n = 384
+    B = o.rng_fill(1230, 4, 4, 43, n, n, limits=(-1.0, 1.0)).reshape(n, n)
+    if mixed:
+        A = (np.random.default_rng(0).integers(-128, 128, (n, n))).astype(np.int32)
+    else:
+        A = o.rng_fill(1230, 8, 2, 42, n, n, limits=(-1.0, 1.0)).reshape(n, n)
+        B = B.astype(np.float64)
+    t0 = time.perf_counter()
+    reps = 0
+    while reps == 0 or time.perf_counter() - t0 < min(3.0, budget):
+        o.matmul_interp(A, B)
+        reps += 1
+    dt = (time.perf_counter() - t0) / reps
+    out = {"value": 2.0 * n ** 3 / dt / 1e12, "unit": "TFLOP/s", "cores": 1, "kind": "port",
+           "sample": "the oracle's interpreter-semantics %s matmul %d^3, 1 core, %.2f s"
+                     % ("i32 x f32" if mixed else "f64", n, dt)}
+    out.update(host_info())
+    return out
 
 
 # ------------------------------------------------------------------ errors per shape
@@ -736,21 +873,24 @@ def run_errors(args):
             C64 = A64 @ B64
             scale = np.abs(A64) @ np.abs(B64)
             r = {"M": m, "N": n, "K": k, "inputs": "uniform[%g,1)" % lo}
-            for name, mode in (("bf16x3", chai.GEMM_BF16X3), ("fp32_fma", chai.GEMM_FMA)):
+            for name, mode in (("bf16x3", chai.GEMM_BF16X3), ("bf16x6", chai.GEMM_BF16X6),
+                               ("fp32_fma", chai.GEMM_FMA)):
                 e = np.abs(chai.gemm(A, B, mode=mode).cpu().numpy().astype(np.float64) - C64)
                 r[name] = {"max_abs": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
                            "max_rel_to_sum_abs": float((e / scale).max())}
             r["ratio_max"] = r["bf16x3"]["max_abs"] / r["fp32_fma"]["max_abs"]
             r["ratio_rms"] = r["bf16x3"]["rms"] / r["fp32_fma"]["rms"]
+            r["x6_ratio_max"] = r["bf16x6"]["max_abs"] / r["fp32_fma"]["max_abs"]
+            r["x6_ratio_rms"] = r["bf16x6"]["rms"] / r["fp32_fma"]["rms"]
             rows.append(r)
-            print("K=%5d %-16s bf16x3 max %.3e rms %.3e | fp32 max %.3e rms %.3e | ratio max %.2f rms %.2f"
-                  % (k, r["inputs"], r["bf16x3"]["max_abs"], r["bf16x3"]["rms"], r["fp32_fma"]["max_abs"],
-                     r["fp32_fma"]["rms"], r["ratio_max"], r["ratio_rms"]), flush=True)
+            print("K=%5d %-16s x3/fp32 max %.2f rms %.2f | x6/fp32 max %.2f rms %.2f"
+                  % (k, r["inputs"], r["ratio_max"], r["ratio_rms"], r["x6_ratio_max"], r["x6_ratio_rms"]),
+                  flush=True)
     print(json.dumps({"errors_vs_fp64_per_shape": rows}))
 
 
 # ------------------------------------------------------------------ stream
-def run_stream(args):
+def run_stream(args, leg=False):
     """An HBM-bound fused elementwise trace, C = max(A * 0.5f + B, 0) over 2^28 f32
     (1 GiB per array): 8 B read + 4 B written per element, one fused kernel.  Beside it:
     torch's two-input elementwise kernel and a plain device copy (8 B per element),
@@ -787,6 +927,23 @@ def run_stream(args):
     line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": read_traffic(PMC_SUMMARY_STREAM),
                         "kernel": "chai_trace_kernel"}
+    if not args.no_cpu:
+        from oracle import oracle as o
+        kc = 1 << 22
+        tc, Ac, Bc, Cc = stream_trace(kc)
+        xa = np.random.default_rng(0).standard_normal(kc).astype(np.float32)
+        t0 = time.perf_counter()
+        reps = 0
+        while time.perf_counter() - t0 < min(5.0, args.cpu_budget):
+            o.trace_eval(tc, {Ac.index: xa, Bc.index: xa})
+            reps += 1
+        dt = (time.perf_counter() - t0) / reps
+        line["cpu_baseline"] = {"value": round(12.0 * kc / dt / 1e9, 3), "unit": "GB/s", "cores": 1,
+                                "kind": "port", "sample": "oracle trace evaluation on 2^22 elements, "
+                                                          "%d runs, %.2f s each" % (reps, dt)}
+        line["cpu_baseline"].update(host_info())
+    if leg:
+        return line
     emit(line)
 
 

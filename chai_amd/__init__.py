@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libchai_mi355x.so")
 U32, I32, F32, F64 = 0, 1, 4, 8
 PHILOX, THREEFRY = 1230, 1231
 OP_DOT, OP_MEAN, OP_SUM = 130, 131, 132
-GEMM_AUTO, GEMM_FMA, GEMM_BF16X3, GEMM_F64 = 0, 1, 2, 3
+GEMM_AUTO, GEMM_FMA, GEMM_BF16X3, GEMM_F64, GEMM_INT, GEMM_BF16X6 = 0, 1, 2, 3, 4, 5
 
 _lib = None
 _inited = False
@@ -168,7 +168,7 @@ def epilogue(scale, shift, lo):
 
 
 def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, beta=0.0,
-         general=False, out_dtype=None, prec_a=None, prec_b=None, variant=-1, epi=None):
+         general=False, out_dtype=None, prec_a=None, prec_b=None, variant=-1, epi=None, prec_c=None):
     """C = op(A) op(B) (or alpha op(A) op(B) + beta C) on 2-D / 3-D (batched) tensors;
     `epi` (an Epilogue) is fused into the store."""
     import torch
@@ -188,7 +188,7 @@ def gemm(A, B, C=None, trans_a=False, trans_b=False, mode=GEMM_AUTO, alpha=1.0, 
             out_dtype = torch.float64 if F64 in (pa, pb) else torch.float32
         C = torch.empty((nb, M, N) if batched else (M, N), dtype=out_dtype, device=A.device)
     d = GemmDesc()
-    d.prec_a, d.prec_b, d.prec_c = pa, pb, prec_of(C.dtype)
+    d.prec_a, d.prec_b, d.prec_c = pa, pb, prec_c if prec_c is not None else prec_of(C.dtype)
     d.trans_a, d.trans_b = int(trans_a), int(trans_b)
     d.M, d.N, d.K = M, N, K
     d.lda, d.ldb, d.ldc = A.stride(-2), B.stride(-2), C.stride(-2)
@@ -220,7 +220,10 @@ def gemv(A, x, y=None, trans_a=False, general=False, alpha=1.0, beta=0.0, prec_a
     M, N = A.shape
     pa = prec_a if prec_a is not None else prec_of(A.dtype)
     px = prec_x if prec_x is not None else prec_of(x.dtype)
-    py = F64 if F64 in (pa, px) else F32
+    if pa <= I32 and px <= I32:
+        py = max(pa, px)                 # integer products: the promoted integer type
+    else:
+        py = F64 if F64 in (pa, px) else F32
     if y is None:
         y = torch.zeros(N if trans_a else M, dtype=dtype_of(py), device=A.device)
     L = lib()

@@ -54,6 +54,27 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, u
         nz[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (uint8_t)(any != 0);
 }
 
+// The three-piece split of the BF16x6 family: p0 = hi, p1 = lo, p2 = the
+// third piece, each in the k-blocked panel layout, instances dst_bstride
+// apart; nz as split_kernel's (of the lo panel).
+template <typename T, bool TRANS>
+__global__ __launch_bounds__(256) void split3_kernel(const T* __restrict__ src, uint64_t ld, uint64_t R,
+                                                     uint64_t K, uint64_t Rp, uint64_t Kp, uint64_t src_bstride,
+                                                     uint64_t dst_bstride, uint16_t* __restrict__ p0,
+                                                     uint16_t* __restrict__ p1, uint16_t* __restrict__ p2,
+                                                     uint8_t* __restrict__ nz)
+{
+    __shared__ float tile[64][65];
+    const uint64_t z = blockIdx.z;
+    const uint32_t lo_or =
+        split_tile<T, TRANS, true>(src + z * src_bstride, ld, R, K, Kp, Rp, p0 + z * dst_bstride,
+                                   p1 + z * dst_bstride, (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64,
+                                   tile, p2 + z * dst_bstride);
+    const int any = __syncthreads_or(lo_or != 0u);
+    if (threadIdx.x == 0)
+        nz[(z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (uint8_t)(any != 0);
+}
+
 // Pack op(X) into a dense zero-padded [Rp][Cp] array of TO (element (r, c) of
 // op(X) at src[r*ld + c] or, transposed, src[c*ld + r]).  A block covers 64 x 64
 // elements; each thread writes 16 consecutive c of one row as 4-element
@@ -416,6 +437,31 @@ static void split_any(int prec, const void* src, uint64_t ld, bool trans, uint64
     }
 }
 
+template <typename T>
+static void launch_split3(const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K, uint64_t Rp,
+                          uint64_t Kp, uint64_t src_bstride, uint64_t dst_bstride, uint64_t nb, uint16_t* p0,
+                          uint16_t* p1, uint16_t* p2, uint8_t* nz, hipStream_t st)
+{
+    dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64), (unsigned)nb);
+    if (trans)
+        split3_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, dst_bstride,
+                                                     p0, p1, p2, nz);
+    else
+        split3_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, dst_bstride,
+                                                      p0, p1, p2, nz);
+}
+
+static void split3_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K, uint64_t Rp,
+                       uint64_t Kp, uint64_t src_bstride, uint64_t dst_bstride, uint64_t nb, uint16_t* p0,
+                       uint16_t* p1, uint16_t* p2, uint8_t* nz, hipStream_t st)
+{
+    switch (prec) {
+        case CHAI_F32: launch_split3<float>(src, ld, trans, R, K, Rp, Kp, src_bstride, dst_bstride, nb, p0, p1, p2, nz, st); break;
+        case CHAI_I32: launch_split3<int32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, dst_bstride, nb, p0, p1, p2, nz, st); break;
+        default: launch_split3<uint32_t>(src, ld, trans, R, K, Rp, Kp, src_bstride, dst_bstride, nb, p0, p1, p2, nz, st); break;
+    }
+}
+
 template <typename T, typename TO>
 static void launch_pack(const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t Cn,
                         uint64_t Rp, uint64_t Cp, uint64_t src_bstride, uint64_t nb, TO* dst,
@@ -545,6 +591,82 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         if (e != hipSuccess) return hip_fail(e, "gemm_x3 launch");
         return CHAI_OK;
     }
+    if (mode == CHAI_GEMM_BF16X6) {
+        // Split-BF16x6: A = a0 + a1 + a2, B = b0 + b1 + b2 (three bf16 pieces,
+        // all 24 bits of an f32).  The products of order 0 and 1 (a0 b0, a0 b1,
+        // a1 b0) are the BF16x3 tile's; the order-2 ones (a2 b0 + a0 b2 + a1 b1)
+        // are one plain bf16 GEMM over panels concatenated along k:
+        // [a2 | a0 | a1] x [b0 | b2 | b1].  The first pass writes T; the second
+        // accumulates the small terms and adds T before alpha / beta and the
+        // epilogue.  The terms dropped (order >= 3) are below 2^-24 |a b|.
+        if (d.prec_c != CHAI_F32) { set_error("gemm: bf16x6 writes f32 C"); return CHAI_ERR_ARG; }
+        const int vid = variant < 0 ? g_x3_default : variant;
+        if (vid >= kNumX3) { set_error("gemm: bf16x6 variant %d out of range", vid); return CHAI_ERR_ARG; }
+        const X3Variant& v = kX3Variants[vid];
+        const uint64_t Mp = round_up(M, v.BM), Np = round_up(N, v.BN), Kp = round_up(K, 64);
+        const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
+        const uint64_t sa3 = 3 * Mp * Kp, sb3 = 3 * Np * Kp;        // elements per instance
+        const size_t panel_bytes = sizeof(uint16_t) * (sa3 * na + sb3 * nbb);
+        const uint64_t fa = (Mp / 64) * (Kp / 64), fb = (Np / 64) * (Kp / 64);
+        const size_t t_bytes = sizeof(float) * M * N * nb;
+        const size_t bytes = panel_bytes + t_bytes + fa * na + fb * nbb;
+        uint16_t* ws = nullptr;
+        CHAI_HIP_TRY(hipMallocAsync((void**)&ws, bytes, st));
+        uint16_t* PA = ws;                       // per instance [a2 | a0 | a1]
+        uint16_t* PB = PA + sa3 * na;            // per instance [b0 | b2 | b1]
+        float* T = (float*)((char*)ws + panel_bytes);
+        uint8_t* nz_a = (uint8_t*)T + t_bytes;
+        uint8_t* nz_b = nz_a + fa * na;
+        prof_mark(0, st);
+        split3_any(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, sa3, na,
+                   PA + Mp * Kp, PA + 2 * Mp * Kp, PA, nz_a, st);
+        split3_any(d.prec_b, B, d.ldb, d.trans_b == 0, N, K, Np, Kp, d.stride_b, sb3, nbb,
+                   PB, PB + 2 * Np * Kp, PB + Np * Kp, nz_b, st);
+        prof_mark(1, st);
+        static bool attr6[kNumX3] = {};
+        if (!attr6[vid]) {
+            CHAI_HIP_TRY(hipFuncSetAttribute((const void*)v.kernel,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds));
+            attr6[vid] = true;
+        }
+        const uint32_t tiles_m = (uint32_t)(Mp / v.BM), tiles_n = (uint32_t)(Np / v.BN);
+        dim3 grid(tiles_m * tiles_n, 1, (unsigned)nb);
+        GemmArgs g{};                            // pass 1: order 0 and 1 -> T
+        g.Ah = PA + Mp * Kp; g.Al = PA + 2 * Mp * Kp; g.Bh = PB; g.Bl = PB + 2 * Np * Kp;
+        g.a_kblk = Mp * 32; g.b_kblk = Np * 32;
+        g.a_nz = nz_a; g.b_nz = nz_b;
+        g.a_nz_bstride = d.same_a ? 0 : fa;
+        g.b_nz_bstride = d.same_b ? 0 : fb;
+        g.Kp = Kp; g.Np = Np;
+        g.pa_bstride = d.same_a ? 0 : sa3;
+        g.pb_bstride = d.same_b ? 0 : sb3;
+        g.C = T; g.c_bstride = M * N; g.ldc = N;
+        g.M = M; g.N = N;
+        g.tiles_m = tiles_m; g.tiles_n = tiles_n;
+        hipLaunchKernelGGL(v.kernel, grid, dim3(v.threads), v.lds, st, g);
+        hipError_t e = hipGetLastError();
+        GemmArgs h{};                            // pass 2: order 2 over 3 Kp, + T -> C
+        h.Ah = PA; h.Bh = PB;
+        h.a_kblk = Mp * 32; h.b_kblk = Np * 32;
+        h.Kp = 3 * Kp; h.Np = Np;
+        h.pa_bstride = d.same_a ? 0 : sa3;
+        h.pb_bstride = d.same_b ? 0 : sb3;
+        h.C = (float*)C; h.c_bstride = d.stride_c; h.ldc = d.ldc;
+        h.M = M; h.N = N;
+        h.tiles_m = tiles_m; h.tiles_n = tiles_n;
+        h.general = d.general; h.alpha = d.alpha; h.beta = d.beta;
+        set_epilogue(h, epi);
+        h.hi_only = 1;
+        h.add = T; h.ld_add = N; h.add_bstride = M * N;
+        if (e == hipSuccess) {
+            hipLaunchKernelGGL(v.kernel, grid, dim3(v.threads), v.lds, st, h);
+            e = hipGetLastError();
+        }
+        prof_mark(2, st);
+        CHAI_HIP_TRY(hipFreeAsync(ws, st));
+        if (e != hipSuccess) return hip_fail(e, "gemm_x6 launch");
+        return CHAI_OK;
+    }
     if (mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_F64) {
         const bool f64 = mode == CHAI_GEMM_F64;
         int dv = 0;
@@ -615,7 +737,10 @@ using namespace chai_hip;
 
 namespace chai_hip {
 
-int gemm_num_variants(int mode) { return mode == CHAI_GEMM_BF16X3 ? kNumX3 : mode == CHAI_GEMM_F64 ? kNumD : 1; }
+int gemm_num_variants(int mode)
+{
+    return (mode == CHAI_GEMM_BF16X3 || mode == CHAI_GEMM_BF16X6) ? kNumX3 : mode == CHAI_GEMM_F64 ? kNumD : 1;
+}
 
 int gemm_variant_params(int mode, int vid, int* p)
 {
@@ -624,7 +749,7 @@ int gemm_variant_params(int mode, int vid, int* p)
         p[0] = v.BM; p[1] = v.BN; p[2] = D_BK; p[3] = v.WGM; p[4] = v.WGN;
         return CHAI_OK;
     }
-    if (mode != CHAI_GEMM_BF16X3 || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;
+    if ((mode != CHAI_GEMM_BF16X3 && mode != CHAI_GEMM_BF16X6) || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;
     const X3Variant& v = kX3Variants[vid];
     p[0] = v.BM; p[1] = v.BN; p[2] = v.BK; p[3] = v.WGM; p[4] = v.WGN;
     return CHAI_OK;

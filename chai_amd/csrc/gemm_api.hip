@@ -23,12 +23,17 @@ extern "C" int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A,
     if (d.batch > 65535) { set_error("gemm: at most 65535 batch instances per call"); return CHAI_ERR_ARG; }
     const bool any_f64 = d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64;
     const bool ints_only = d.prec_a <= CHAI_I32 && d.prec_b <= CHAI_I32;
-    if (ints_only) {
-        set_error("gemm: integer x integer products are evaluated by the interpreter path");
-        return CHAI_ERR_ARG;
+    if (ints_only || d.mode == CHAI_GEMM_INT) {
+        if (!ints_only || (d.mode != CHAI_GEMM_AUTO && d.mode != CHAI_GEMM_INT)) {
+            set_error("gemm: CHAI_GEMM_INT takes integer x integer operands, and they no other family");
+            return CHAI_ERR_ARG;
+        }
+        if (epi && epi->kind != CHAI_EPI_NONE) { set_error("gemm: epilogues apply to f32 results only"); return CHAI_ERR_ARG; }
+        if (d.K == 0) { set_error("gemm: K == 0"); return CHAI_ERR_ARG; }
+        return igemm_launch(d, A, B, C, as_stream(stream));
     }
     int mode = d.mode;
-    if (mode == CHAI_GEMM_AUTO) mode = any_f64 ? CHAI_GEMM_F64 : CHAI_GEMM_BF16X3;
+    if (mode == CHAI_GEMM_AUTO) mode = gemm_auto_mode(d);
     if (any_f64 && mode != CHAI_GEMM_F64) {
         set_error("gemm: f64 operands need CHAI_GEMM_F64");
         return CHAI_ERR_ARG;
@@ -37,8 +42,8 @@ extern "C" int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A,
         set_error("gemm: CHAI_GEMM_F64 writes an f64 C");
         return CHAI_ERR_ARG;
     }
-    if ((mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_BF16X3) && d.prec_c != CHAI_F32) {
-        set_error("gemm: FMA / BF16X3 write an f32 C");
+    if ((mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_BF16X3 || mode == CHAI_GEMM_BF16X6) && d.prec_c != CHAI_F32) {
+        set_error("gemm: FMA / BF16X3 / BF16X6 write an f32 C");
         return CHAI_ERR_ARG;
     }
     if (d.K == 0) {
@@ -52,6 +57,22 @@ extern "C" int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A,
     const int variant = d.variant >= 0 ? d.variant : tuned_variant(d, mode);
     return gemm_launch(d, A, B, C, mode, variant, as_stream(stream), epi);
 }
+
+namespace chai_hip {
+
+// The family AUTO picks: FP64 MFMA for f64 operands, the exact integer path
+// for two integer operands; for f32 (and mixed int / f32) products the split-
+// BF16 form whose error against FP64 stays within twice the FP32-FMA kernel's
+// in max and rms (measured per K on cancelling uniform[-1, 1) data): three
+// products from K = 16384 on, six below.
+int gemm_auto_mode(const chai_gemm_desc& d)
+{
+    if (d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64) return CHAI_GEMM_F64;
+    if (d.prec_a <= CHAI_I32 && d.prec_b <= CHAI_I32) return CHAI_GEMM_INT;
+    return d.K >= 16384 ? CHAI_GEMM_BF16X3 : CHAI_GEMM_BF16X6;
+}
+
+}  // namespace chai_hip
 
 extern "C" int chai_hip_gemm(const chai_gemm_desc* desc, const void* A, const void* B, void* C,
                              void* stream)

@@ -35,6 +35,12 @@ struct GemmArgs {
     // epi == 1: c = max(c * epi_scale + epi_shift, epi_lo), each op rounded in f32
     int epi;
     float epi_scale, epi_shift, epi_lo;
+    // split-BF16x6 second pass: hi_only runs the hi x hi product alone (a
+    // plain bf16 GEMM over the concatenated small-term panels), and add (f32,
+    // row stride ld_add) is the first pass's result, added before alpha / beta
+    int hi_only;
+    const float* add;
+    uint64_t ld_add, add_bstride;
 };
 
 // variant < 0 selects the current default (set by the tuner) for the mode.
@@ -45,5 +51,9 @@ int gemm_variant_params(int mode, int vid, int* params);   // BM, BN, BK, WGM, W
 void gemm_set_default_variant(int mode, int vid);
 // best variant recorded by the tuner for this exogenous key, or -1
 int tuned_variant(const chai_gemm_desc& d, int mode);
+// The family CHAI_GEMM_AUTO resolves to for a descriptor (gemm_api.hip).
+int gemm_auto_mode(const chai_gemm_desc& d);
+// integer x integer products with the interpreter's semantics (igemm.hip)
+int igemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, hipStream_t st);
 
 }  // namespace chai_hip

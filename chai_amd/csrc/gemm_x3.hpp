@@ -153,7 +153,7 @@ __device__ __forceinline__ void x3_stage_mfma(const u32x4* lds, int cur, const i
 // lane % 32; register r -> row 8 (r / 4) + 4 (lane / 32) + r % 4).
 template <typename S>
 __device__ __forceinline__ void x3_store(const GemmArgs& a, float* C, uint64_t row_base, uint64_t col_base, int lane,
-                                         const floatx16 (&acc)[S::TI][S::TJ])
+                                         const floatx16 (&acc)[S::TI][S::TJ], const float* add)
 {
 #pragma unroll
     for (int i = 0; i < S::TI; i++)
@@ -163,7 +163,11 @@ __device__ __forceinline__ void x3_store(const GemmArgs& a, float* C, uint64_t r
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const uint64_t row = row_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row < a.M && col < a.N) store_c(C, row * a.ldc + col, acc[i][j][r], a);
+                if (row < a.M && col < a.N) {
+                    float v = acc[i][j][r];
+                    if (add) v = v + gload(add + row * a.ld_add + col);
+                    store_c(C, row * a.ldc + col, v, a);
+                }
             }
         }
 }
@@ -236,7 +240,8 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
         __syncthreads();    // ... every wave's share, and stage kt is no longer read
     }
 
-    x3_store<S>(a, C, (uint64_t)tm * BM + wm, (uint64_t)tn * BN + wn, lane, acc);
+    x3_store<S>(a, C, (uint64_t)tm * BM + wm, (uint64_t)tn * BN + wn, lane, acc,
+                a.add ? a.add + bz * a.add_bstride : nullptr);
 }
 
 // One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is
@@ -246,8 +251,8 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
 template <int BM, int BN, int BK, int WGM, int WGN>
 __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, uint64_t bz, u32x4* lds)
 {
-    int skip = 0;
-    if (a.a_nz) {
+    int skip = a.hi_only ? 3 : 0;
+    if (a.a_nz && !a.hi_only) {
         constexpr int THREADS = WGM * WGN * 64;
         const uint64_t kb64 = a.Kp / 64;
         const uint64_t na = (uint64_t)(BM / 64) * kb64, nb = (uint64_t)(BN / 64) * kb64;

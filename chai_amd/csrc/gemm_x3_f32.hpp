@@ -161,7 +161,7 @@ __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_
         if (more) x3f_store_stage<BM, BN, WGM, WGN>(lds + ((kt + 1) & 1) * S::STAGE, fa, fb);
         __syncthreads();
     }
-    x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc);
+    x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc, nullptr);
 }
 
 // The f32-fed family's tile shapes: X(BM, BN, waves_m, waves_n, min blocks per CU)

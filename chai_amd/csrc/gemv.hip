@@ -11,6 +11,8 @@
 //               row), blocks split the rows; per-chunk FP64 partials are
 //               summed in a fixed order by a second pass.
 #include "common.hpp"
+
+#include <cstring>
 #include "reduce_device.hpp"
 
 #include <type_traits>
@@ -205,8 +207,23 @@ extern "C" int chai_hip_gemv(int prec_a, int prec_x, int prec_y, int trans_a, si
         return CHAI_ERR_ARG;
     }
     if (prec_a <= CHAI_I32 && prec_x <= CHAI_I32) {
-        set_error("gemv: integer x integer products are evaluated by the interpreter path");
-        return CHAI_ERR_ARG;
+        // the interpreter's integer product semantics: the GEMM family's
+        // integer path with x as a one-column matrix
+        if (M == 0 || N == 0) return CHAI_OK;
+        if (!A || !x || !y) { set_error("gemv: null operand"); return CHAI_ERR_ARG; }
+        chai_gemm_desc d;
+        std::memset(&d, 0, sizeof d);
+        d.prec_a = prec_a; d.prec_b = prec_x; d.prec_c = prec_y;
+        d.trans_a = trans_a;
+        d.M = trans_a ? N : M; d.K = trans_a ? M : N; d.N = 1;
+        d.lda = lda; d.ldb = 1; d.ldc = 1;
+        d.general = general; d.alpha = alpha; d.beta = beta;
+        d.batch = batch ? batch : 1;
+        d.same_a = same_a;
+        d.stride_a = stride_a; d.stride_b = stride_x; d.stride_c = stride_y;
+        d.mode = CHAI_GEMM_INT;
+        d.variant = -1;
+        return chai_hip_gemm(&d, A, x, y, stream);
     }
     const int want_y = (prec_a == CHAI_F64 || prec_x == CHAI_F64) ? CHAI_F64 : CHAI_F32;
     if (prec_y != want_y) { set_error("gemv: y must have the promoted precision (%d)", want_y); return CHAI_ERR_ARG; }

@@ -80,11 +80,17 @@ __global__ __launch_bounds__(kRedThreads) void dot_partial(const TA* __restrict_
     if (threadIdx.x == 0) partial[(uint64_t)slot * blocks_per_slot + blockIdx.x] = r;
 }
 
+// The result in the element type, converted the way the interpreter's x86
+// build converts its double accumulator (cvttsd2si): to int32 a value outside
+// [-2^31, 2^31) gives INT_MIN; to uint32 the conversion goes through int64 and
+// keeps the low 32 bits.
 template <typename T>
 __device__ __forceinline__ T from_double(double v)
 {
-    if constexpr (std::is_same<T, uint32_t>::value) return (uint32_t)(long long)v;
-    else if constexpr (std::is_same<T, int32_t>::value) return (int32_t)(long long)v;
+    if constexpr (std::is_same<T, uint32_t>::value)
+        return (v != v || v >= 0x1p63 || v < -0x1p63) ? 0u : (uint32_t)(unsigned long long)(long long)v;
+    else if constexpr (std::is_same<T, int32_t>::value)
+        return (v != v || v >= 0x1p31 || v < -0x1p31) ? INT32_MIN : (int32_t)v;
     else return (T)v;
 }
 

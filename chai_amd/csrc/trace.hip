@@ -196,6 +196,7 @@ bool is_mm(uint32_t c) { return c == 120 || c == 121; }
 bool is_gather(uint32_t c) { return c == 40 || c == 41; }
 bool is_litdata(uint32_t c) { return c >= 68 && c <= 82; }
 bool is_cl(uint32_t c) { return c == 50; }          // kernel_from_opencl
+bool is_tr(uint32_t c) { return c == 122; }         // transpose
 // an inline OpenCL kernel statement: its last node is kernel_from_opencl
 bool is_cl_stmt(const chai_trace& t, const chai_trace_stmt& st)
 {
@@ -347,10 +348,14 @@ Dims dmax(const Dims& a, const Dims& b)
 }
 
 // result shape of matmul(A, B) with the operands' transpose flags, the
-// oracle's dispatch: matrix * vector, vector * matrix, matrix * matrix
+// oracle's dispatch: matrix * vector, vector * matrix, matrix * matrix.  With
+// slots, instance s multiplies A's instance s mod A.slots by B's instance
+// s mod B.slots (interp/BaseInterp.cpp:79-98, interp/InterpMatmulBase.cpp:
+// 113-118): max(A.slots, B.slots) results.
 Dims mm_dims(const Dims& a, const Dims& b, bool ta, bool tb)
 {
     Dims r;
+    r.S = a.S > b.S ? a.S : b.S;
     if (b.H == 1 && !tb && a.H > 1) {
         r.W = ta ? a.W : a.H;
     } else if (a.H == 1 && !ta && b.H > 1) {
@@ -368,7 +373,7 @@ int node_arity(uint32_t c)
     bool fp_only;
     int bkind;
     if (c == CHAI_TRACE_VAR || is_scalar(c) || is_rng(c) || is_index(c) || is_litdata(c)) return 0;
-    if (is_convert(c) || c == 131 || c == 132 || is_unary(c, fp_only)) return 1;
+    if (is_convert(c) || c == 131 || c == 132 || is_tr(c) || is_unary(c, fp_only)) return 1;
     if (c == 130 || c == 120 || c == 40 || is_binary(c, bkind)) return 2;
     if (c == 121 || c == 41 || is_ternary(c)) return 3;
     return -1;
@@ -415,7 +420,7 @@ int parse_nodes(const chai_trace& t, const chai_trace_node* nodes, uint32_t coun
             arity = 2;
         } else if (is_ternary(n.code)) {
             arity = 3;
-        } else if (n.code == 131 || n.code == 132) {
+        } else if (n.code == 131 || n.code == 132 || is_tr(n.code)) {
             arity = 1;
         } else if (n.code == 130) {
             arity = 2;
@@ -450,6 +455,7 @@ int parse_nodes(const chai_trace& t, const chai_trace_node* nodes, uint32_t coun
                 if (is_fp(k.prec)) return fail("bytecode %u needs an integer operand", n.code);
             }
             if (n.code == 131 || n.code == 132) { e.d.W = 1; e.d.H = 1; }
+            if (is_tr(n.code)) { e.d.W = k.d.H; e.d.H = k.d.W; }   // interp/InterpTranspose.cpp
         } else if (is_convert(n.code)) {
             e.d = s.nodes[e.kids[0]].d;
         } else if (arity == 2) {
@@ -558,11 +564,11 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
     const int rc = parse_nodes(t, t.nodes + cs.first, cs.count, s);
     if (rc) return rc;
     const ENode& r = s.nodes[s.root];
-    s.kind = is_red(r.code) ? 1 : is_mm(r.code) ? 2 : 0;
-    // reductions / matmul only at the root
+    s.kind = is_red(r.code) ? 1 : is_mm(r.code) ? 2 : is_tr(r.code) ? 4 : 0;
+    // reductions / matmul / transpose only at the root
     for (size_t i = 0; i + 1 < s.nodes.size(); i++)
-        if (is_red(s.nodes[i].code) || is_mm(s.nodes[i].code))
-            return fail("reduction or matmul below the root of a statement (split it first)");
+        if (is_red(s.nodes[i].code) || is_mm(s.nodes[i].code) || is_tr(s.nodes[i].code))
+            return fail("reduction, matmul or transpose below the root of a statement (split it first)");
     const chai_trace_var& lv = t.vars[s.lhs];
     if (s.kind == 0) {
         s.space = r.d;
@@ -574,10 +580,23 @@ int parse_stmt(const chai_trace& t, const chai_trace_stmt& cs, Stmt& s)
         if (r.code == 130) in = dmax(in, s.nodes[r.kids[1]].d);
         s.space = in;
         if (lv.W != 1 || lv.H != 1 || lv.slots != in.S) return fail("reduction result must be 1x1xslots");
+    } else if (s.kind == 4) {
+        const ENode& k = s.nodes[r.kids[0]];
+        if (k.code != CHAI_TRACE_VAR || (k.src->flags & 1)) return fail("transpose operand must be a variable");
+        if (r.d.W != lv.W || r.d.H != lv.H || r.d.S != lv.slots) return fail("transpose result has the wrong shape");
     } else {
         for (int k : r.kids)
             if (s.nodes[k].code != CHAI_TRACE_VAR) return fail("matmul operands must be variables");
-        if (lv.slots != 1) return fail("matmul of slotted arrays is not supported");
+        // slots: each operand has one instance (shared by every product, the
+        // reference's sameDataA / sameDataB, enqueue/EnqueueMatmulAuto.cpp:
+        // 57-68) or as many as the result
+        if (lv.slots != r.d.S) return fail("matmul result has %llu slots, its variable %llu",
+                                           (unsigned long long)r.d.S, (unsigned long long)lv.slots);
+        for (int k : r.kids) {
+            const uint64_t ks = s.nodes[k].d.S;
+            if (ks != 1 && ks != lv.slots) return fail("matmul operand with %llu of %llu slots",
+                                                      (unsigned long long)ks, (unsigned long long)lv.slots);
+        }
     }
     return CHAI_OK;
 }
@@ -834,7 +853,7 @@ static int split_trace(const chai_trace& in, Versioned& V, const chai_trace*& ou
         if (is_cl_stmt(in, st)) continue;
         for (uint32_t k = 0; k < st.count; k++) {
             const uint32_t c = in.nodes[st.first + k].code;
-            if (((is_red(c) || is_mm(c)) && k + 1 < st.count) || is_mm(c) || is_gather(c)) need = true;
+            if (((is_red(c) || is_mm(c)) && k + 1 < st.count) || is_mm(c) || is_gather(c) || is_tr(c)) need = true;
         }
     }
     if (!need) return CHAI_OK;
@@ -889,7 +908,7 @@ static int split_trace(const chai_trace& in, Versioned& V, const chai_trace*& ou
             if ((int)starts.size() < ar) return fail("stack underflow at bytecode %u", n.code);
             const size_t base = starts.size() - (size_t)ar;
             // operands that must be variables: all of a matmul's, a gather's data
-            const int nv = is_mm(n.code) ? ar : is_gather(n.code) ? 1 : 0;
+            const int nv = is_mm(n.code) ? ar : (is_gather(n.code) || is_tr(n.code)) ? 1 : 0;
             for (int j = nv - 1; j >= 0; j--) {
                 const size_t b = starts[base + j];
                 const size_t e = (size_t)j + 1 < (size_t)ar ? starts[base + j + 1] : cur.size();
@@ -903,7 +922,7 @@ static int split_trace(const chai_trace& in, Versioned& V, const chai_trace*& ou
             starts.resize(base);
             cur.push_back(n);
             starts.push_back(start);
-            if ((is_red(n.code) || is_mm(n.code)) && k + 1 < st.count) {
+            if ((is_red(n.code) || is_mm(n.code) || is_tr(n.code)) && k + 1 < st.count) {
                 chai_trace_node var;
                 const int rc = hoist(cur, start, cur.size(), var);
                 if (rc) return rc;
@@ -1006,13 +1025,13 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
             if (is_gather(e.code) && !P.groups.empty() &&
                 group_of_var[s.nodes[e.kids[0]].src->arg] == (int)P.groups.size() - 1)
                 reads_red = true;
-        const bool join = s.kind != 2 && s.kind != 3 && !P.groups.empty() && P.groups.back().kind == 0 &&
+        const bool join = s.kind != 2 && s.kind != 3 && s.kind != 4 && !P.groups.empty() && P.groups.back().kind == 0 &&
                           P.groups.back().space == s.space && !reads_red &&
                           (s.kind == 0 || std::count_if(P.groups.back().stmts.begin(), P.groups.back().stmts.end(),
                                                         [&](int q) { return P.stmts[q].kind == 1; }) < VM_MAX_ACC);
         if (!join) {
             Group g;
-            g.kind = s.kind == 2 || s.kind == 3 ? s.kind : 0;
+            g.kind = s.kind == 2 || s.kind == 3 || s.kind == 4 ? s.kind : 0;
             g.space = s.space;
             P.groups.push_back(g);
         }
@@ -1033,12 +1052,12 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
             const uint32_t v = e.src->arg;
             if (P.vars[v].role != CHAI_VAR_TEMP) continue;
             if (def_group[v] < 0) return fail("temp variable %u read before it is assigned", v);
-            if (def_group[v] != group_of_stmt[i] || s.kind == 2 || s.kind == 3) P.needs_mem[v] = true;
+            if (def_group[v] != group_of_stmt[i] || s.kind == 2 || s.kind == 3 || s.kind == 4) P.needs_mem[v] = true;
         }
         for (const auto& e : s.nodes)
             if (is_gather(e.code)) P.needs_mem[s.nodes[e.kids[0]].src->arg] = true;
         if (s.kind == 1 && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
-        if (s.kind == 2 && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
+        if ((s.kind == 2 || s.kind == 4) && P.vars[s.lhs].role == CHAI_VAR_TEMP) P.needs_mem[s.lhs] = true;
     }
     // VM code per group
     for (auto& G : P.groups) {
@@ -1176,26 +1195,32 @@ static int run_matmul(const TraceProgram& P, const Stmt& s, void* const* bufs, h
     const bool ta = s.nodes[r.kids[0]].src->flags & 1, tb = s.nodes[r.kids[1]].src->flags & 1;
     const chai_trace_var &A = P.vars[va], &B = P.vars[vb], &Cv = P.vars[s.lhs];
     const bool general = r.code == 121;
+    const uint64_t S = Cv.slots;                      // instances: A / B have 1 or S
     if (general && s.nodes[r.kids[2]].src->arg != s.lhs) {
         // C operand is a different variable: copy it into the output first
+        // (one instance of it serves every slot)
         const uint32_t vc = s.nodes[r.kids[2]].src->arg;
         const size_t bytes = prec_size((int)Cv.prec) * Cv.W * Cv.H;
         if (P.vars[vc].prec != Cv.prec) return fail("matmulG C operand precision differs from its result");
-        CHAI_HIP_TRY(hipMemcpyAsync(bufs[s.lhs], bufs[vc], bytes, hipMemcpyDeviceToDevice, st));
+        for (uint64_t k = 0; k < S; k++)
+            CHAI_HIP_TRY(hipMemcpyAsync((char*)bufs[s.lhs] + k * bytes,
+                                        (const char*)bufs[vc] + (P.vars[vc].slots == 1 ? 0 : k * bytes), bytes,
+                                        hipMemcpyDeviceToDevice, st));
     }
     // row-major arrays: W = columns, H = rows.  A vector has H == 1.
     const uint64_t am = ta ? A.W : A.H, ak = ta ? A.H : A.W;
     const uint64_t bk = tb ? B.W : B.H, bn = tb ? B.H : B.W;
     const int pc = (int)Cv.prec;
+    const uint64_t sa = A.slots == 1 ? 0 : A.W * A.H, sb = B.slots == 1 ? 0 : B.W * B.H, sc = Cv.W * Cv.H;
     if (B.H == 1 && !tb && A.H > 1) {                 // matrix * vector
         if (ak != B.W) return fail("matrix-vector inner dimension mismatch");
         return chai_hip_gemv((int)A.prec, (int)B.prec, pc, ta, ta ? A.H : A.H, A.W, bufs[va], A.W,
-                             bufs[vb], bufs[s.lhs], general, r.src->s0, r.src->s1, 1, 0, 0, 0, 0, st);
+                             bufs[vb], bufs[s.lhs], general, r.src->s0, r.src->s1, S, sa == 0, sa, sb, sc, st);
     }
     if (A.H == 1 && !ta && B.H > 1) {                 // vector * matrix = B^T x
         if (A.W != bk) return fail("vector-matrix inner dimension mismatch");
         return chai_hip_gemv((int)B.prec, (int)A.prec, pc, !tb, B.H, B.W, bufs[vb], B.W, bufs[va],
-                             bufs[s.lhs], general, r.src->s0, r.src->s1, 1, 0, 0, 0, 0, st);
+                             bufs[s.lhs], general, r.src->s0, r.src->s1, S, sb == 0, sb, sa, sc, st);
     }
     if (ak != bk) return fail("matmul inner dimension mismatch");
     if (Cv.H != am || Cv.W != bn) return fail("matmul result has the wrong shape");
@@ -1206,10 +1231,52 @@ static int run_matmul(const TraceProgram& P, const Stmt& s, void* const* bufs, h
     d.M = am; d.N = bn; d.K = ak;
     d.lda = A.W; d.ldb = B.W; d.ldc = Cv.W;
     d.general = general; d.alpha = r.src->s0; d.beta = r.src->s1;
-    d.batch = 1;
+    d.batch = S;
+    d.same_a = sa == 0 && S > 1; d.same_b = sb == 0 && S > 1;
+    d.stride_a = sa; d.stride_b = sb; d.stride_c = sc;
     d.mode = CHAI_GEMM_AUTO;
     d.variant = -1;
     return chai_hip_gemm(&d, bufs[va], bufs[vb], bufs[s.lhs], st);
+}
+
+// T = transpose(V) (interp/InterpTranspose.cpp:13-60): out[s][x][y] =
+// in[s][y][x].  HBM-bound: 64 x 64 tiles through LDS (one padded column, so
+// the transposed reads are conflict free), reads and writes coalesced.
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_kernel(const T* __restrict__ in, T* __restrict__ out,
+                                                        uint64_t W, uint64_t H)
+{
+    __shared__ T tile[64][65];
+    const uint64_t slot = blockIdx.z;
+    in += slot * W * H;
+    out += slot * W * H;
+    const uint64_t x0 = (uint64_t)blockIdx.x * 64, y0 = (uint64_t)blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    for (int r = ty; r < 64; r += 4) {
+        const uint64_t y = y0 + r, x = x0 + tx;
+        if (y < H && x < W) tile[r][tx] = in[y * W + x];
+    }
+    __syncthreads();
+    // output row = input column x0 + r, output column = input row y0 + tx
+    for (int r = ty; r < 64; r += 4) {
+        const uint64_t oy = x0 + r, ox = y0 + tx;
+        if (oy < W && ox < H) out[oy * H + ox] = tile[tx][r];
+    }
+}
+
+static int run_transpose(const TraceProgram& P, const Stmt& s, void* const* bufs, hipStream_t st)
+{
+    const ENode& r = s.nodes[s.root];
+    const uint32_t vi = s.nodes[r.kids[0]].src->arg;
+    const chai_trace_var& V = P.vars[vi];
+    const dim3 grid(ceil_div_u(V.W, 64), ceil_div_u(V.H, 64), (unsigned)V.slots);
+    if (grid.y > 65535) return fail("transpose of %llu rows", (unsigned long long)V.H);
+    if (V.prec == CHAI_F64)
+        transpose_kernel<uint64_t><<<grid, 256, 0, st>>>((const uint64_t*)bufs[vi], (uint64_t*)bufs[s.lhs], V.W, V.H);
+    else
+        transpose_kernel<uint32_t><<<grid, 256, 0, st>>>((const uint32_t*)bufs[vi], (uint32_t*)bufs[s.lhs], V.W, V.H);
+    CHAI_LAUNCH_CHECK("transpose");
+    return CHAI_OK;
 }
 
 // an inline OpenCL kernel statement on the trace's buffers
@@ -1624,6 +1691,11 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
             if (rc) break;
             continue;
         }
+        if (G.kind == 4) {
+            rc = run_transpose(P, P.stmts[G.stmts[0]], bufs.data(), st);
+            if (rc) break;
+            continue;
+        }
         VmLaunchArgs L;
         std::memset(&L, 0, sizeof L);
         L.prog = G.dprog;
@@ -1808,4 +1880,46 @@ extern "C" int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, c
     }
     copy_out(std::string(kJitPrelude) + jit_source(P, P.groups[kernel]), buf, len);
     return CHAI_OK;
+}
+
+extern "C" int chai_hip_trace_infer(chai_trace_var* vars, uint32_t num_vars, const chai_trace_stmt* stmts,
+                                    uint32_t num_stmts, const chai_trace_node* nodes, uint32_t num_nodes)
+{
+    if (!vars || (num_stmts && (!stmts || !nodes))) { set_error("trace_infer: null argument"); return CHAI_ERR_ARG; }
+    chai_trace t;
+    t.vars = vars; t.num_vars = num_vars;
+    t.stmts = stmts; t.num_stmts = num_stmts;
+    t.nodes = nodes; t.num_nodes = num_nodes;
+    for (uint32_t i = 0; i < num_stmts; i++) {
+        const chai_trace_stmt& cs = stmts[i];
+        if (cs.lhs >= num_vars) return fail("statement %u assigns variable %u of %u", i, cs.lhs, num_vars);
+        if ((uint64_t)cs.first + cs.count > num_nodes || cs.count == 0) return fail("statement %u: bad node range", i);
+        if (is_cl_stmt(t, cs) || vars[cs.lhs].W != 0) continue;
+        for (uint32_t k = cs.first; k < cs.first + cs.count; k++)
+            if (nodes[k].code == CHAI_TRACE_VAR && nodes[k].arg < num_vars && vars[nodes[k].arg].W == 0)
+                return fail("statement %u reads variable %u before its shape is known", i, nodes[k].arg);
+        Stmt s;
+        const int rc = parse_nodes(t, nodes + cs.first, cs.count, s);
+        if (rc) return rc;
+        const ENode& r = s.nodes[s.root];
+        chai_trace_var& v = vars[cs.lhs];
+        v.prec = (uint32_t)r.prec;
+        v.W = r.d.W; v.H = r.d.H; v.slots = r.d.S;
+    }
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_trace_check(const chai_trace* trace)
+{
+    if (!trace || !trace->num_stmts || !trace->vars || !trace->stmts || !trace->nodes) {
+        set_error("trace_check: empty trace");
+        return CHAI_ERR_ARG;
+    }
+    TraceProgram P;
+    Versioned spl, ver;
+    const chai_trace *ts = nullptr, *tv = nullptr;
+    int rc = split_trace(*trace, spl, ts);
+    if (rc == CHAI_OK) rc = version_trace(*ts, ver, tv);
+    if (rc == CHAI_OK) rc = compile_trace(*tv, P);
+    return rc;
 }

@@ -122,6 +122,7 @@ __global__ void count_mismatch(const T* a, const T* b, uint64_t n, unsigned int*
 // element of C must lie within the family's error bound of that value:
 //   FMA     |C - C64| <= (K + 2) 2^-24 S
 //   BF16X3  |C - C64| <= (3 2^-16 + (3K + 2) 2^-24) S
+//   BF16X6  |C - C64| <= (6K + 4) 2^-24 S
 //   F64     |C - C64| <= (3K + 4) 2^-53 S
 // with S = sum_k |a_ik b_kj| (plus K 2^-126 for underflow).  Non-finite
 // values must match (NaN where the FP64 value is NaN, the same infinity).
@@ -185,7 +186,8 @@ static int validate_output(const chai_gemm_desc& d, int mode, const void* A, con
     const double K = (double)d.K;
     s.rel = mode == CHAI_GEMM_F64 ? (3 * K + 4) * 0x1p-53
           : mode == CHAI_GEMM_FMA ? (K + 2) * 0x1p-24
-                                  : 3 * 0x1p-16 + (3 * K + 2) * 0x1p-24;
+          : mode == CHAI_GEMM_BF16X6 ? (6 * K + 4) * 0x1p-24
+                                     : 3 * 0x1p-16 + (3 * K + 2) * 0x1p-24;
     const uint64_t cand[4] = {0, d.M / 3, (2 * d.M) / 3, d.M - 1};
     unsigned nr = 0;
     for (uint64_t r : cand) {
@@ -236,8 +238,7 @@ std::string kernel_name(const chai_gemm_desc& d, int mode, int v)
 
 int mode_of(const chai_gemm_desc& d)
 {
-    if (d.mode != CHAI_GEMM_AUTO) return d.mode;
-    return (d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64) ? CHAI_GEMM_F64 : CHAI_GEMM_BF16X3;
+    return d.mode != CHAI_GEMM_AUTO ? d.mode : gemm_auto_mode(d);
 }
 
 }  // namespace

@@ -123,13 +123,23 @@ int chai_hip_reduce_noalloc(int op, int prec, const void* in, size_t n, size_t s
                             void* out, double* out_f64, void* stream);
 
 /* ------------------------------------------------------------------- GEMM */
-/* Kernel family selector.  AUTO picks BF16X3 for f32/int operands with an f32
- * result and F64 when any operand is f64. */
+/* Kernel family selector.  AUTO picks F64 when any operand is f64, INT for
+ * two integer operands, and BF16X3 otherwise (f32 or mixed int / f32). */
 enum {
     CHAI_GEMM_AUTO = 0,
     CHAI_GEMM_FMA = 1,      /* plain FP32 VALU FMA                        */
     CHAI_GEMM_BF16X3 = 2,   /* split-BF16 three-product MFMA, FP32 accum  */
-    CHAI_GEMM_F64 = 3       /* FP64 MFMA                                  */
+    CHAI_GEMM_F64 = 3,      /* FP64 MFMA                                  */
+    CHAI_GEMM_BF16X6 = 5,   /* split-BF16 with three pieces per operand, six
+                               MFMA products (all of order <= 2), FP32
+                               accumulate: FP32-FMA-level error at twice the
+                               BF16X3 MFMA work                              */
+    CHAI_GEMM_INT = 4       /* integer x integer, the interpreter's exact
+                               semantics (interp/InterpMatmulBase.hpp:49-76):
+                               products wrap in the promoted 32-bit type, sum
+                               in k order as a double accumulator would,
+                               x86 conversion to C's type; AUTO picks it for
+                               u32 / i32 operand pairs                       */
 };
 
 /* One GEMM call (or a batch of identical-shape instances).  Mirrors the
@@ -383,6 +393,22 @@ int chai_hip_trace_kernel_source(void* handle, int kernel, char* buf, size_t len
 /* Front end only (no device): the JIT source kernel k of a trace would get
  * (the same text as chai_hip_trace_kernel_source, produced without a GPU). */
 int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, char* buf, size_t len);
+
+/* Front end only (no device): fill in prec, W, H and slots of every
+ * variable that a statement assigns and whose W is 0, from the statement's
+ * expression, in statement order (the dimension rules of the reference's
+ * API, api/ArrayFun.cpp:45-118, and interpreter, interp/InterpMatmulBase.cpp:
+ * 79-118).  A trace lowered from a VectorTrace declares only its inputs; this
+ * gives every statement's result its shape before the trace is compiled. */
+int chai_hip_trace_infer(chai_trace_var* vars, uint32_t num_vars, const chai_trace_stmt* stmts,
+                         uint32_t num_stmts, const chai_trace_node* nodes, uint32_t num_nodes);
+
+/* Front end only (no device): the checks of chai_hip_trace_compile (parse,
+ * type, split, group) without compiling kernels; CHAI_OK when the GPU back
+ * end takes the trace, else CHAI_ERR_ARG and chai_hip_last_error() says why
+ * (the answer the reference's JIT gives by failing, which sends the trace to
+ * the interpreter, runtime/Scheduler.cpp:221-254). */
+int chai_hip_trace_check(const chai_trace* trace);
 
 /* ------------------------------------------------------- inline OpenCL C */
 /* The ProgramCL hook (api/ProgramCL.cpp:133-376, api/chai.h:703-771): user

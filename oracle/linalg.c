@@ -20,6 +20,7 @@
  * compiled into oracle/_ref (tests/test_oracle_ref.py).
  */
 #include <math.h>
+#include <stdint.h>
 #include <string.h>
 #include "chai_oracle.h"
 
@@ -44,20 +45,38 @@ static double promoted_product(int pa, int pb, const void* A, size_t ia, const v
         return (double)(a * b);
     }
     if (pa == 0 || pb == 0) {     /* unsigned arithmetic (uint x uint, uint x int) */
+        /* C's usual conversions make uint32 * int32 an unsigned product, which
+         * the double accumulator adds as a nonnegative value even when the
+         * result type is int32 (matmat<uint32_t, int32_t, int32_t>) */
         uint32_t a = pa == 0 ? ((const uint32_t*)A)[ia] : (uint32_t)((const int32_t*)A)[ia];
         uint32_t b = pb == 0 ? ((const uint32_t*)B)[ib] : (uint32_t)((const int32_t*)B)[ib];
-        uint32_t r = a * b;
-        return pp == 0 ? (double)r : (double)(int32_t)r;
+        return (double)(uint32_t)(a * b);
     }
     int32_t a = ((const int32_t*)A)[ia], b = ((const int32_t*)B)[ib];
     return (double)(int32_t)((uint32_t)a * (uint32_t)b);
 }
 
+/* double -> int32 / uint32 as the reference's x86-64 build converts them
+ * (g++ emits cvttsd2si): to int32 a NaN or a value outside [-2^31, 2^31)
+ * gives INT_MIN; to uint32 the conversion goes through int64 (cvttsd2si with
+ * a 64-bit destination) and keeps the low 32 bits, so a NaN or a value
+ * outside the int64 range gives 0. */
+static int32_t x86_to_i32(double v)
+{
+    return (v != v || v >= 2147483648.0 || v < -2147483648.0) ? INT32_MIN : (int32_t)v;
+}
+
+static uint32_t x86_to_u32(double v)
+{
+    return (v != v || v >= 9223372036854775808.0 || v < -9223372036854775808.0)
+               ? 0u : (uint32_t)(uint64_t)(int64_t)v;
+}
+
 static double round_to(int prec, double v)
 {
     switch (prec) {
-        case 0: return (double)(uint32_t)(int64_t)v;
-        case 1: return (double)(int32_t)(int64_t)v;
+        case 0: return (double)x86_to_u32(v);
+        case 1: return (double)x86_to_i32(v);
         case 4: return (double)(float)v;
         default: return v;
     }
@@ -66,8 +85,8 @@ static double round_to(int prec, double v)
 static void store(int prec, void* p, size_t i, double v)
 {
     switch (prec) {
-        case 0: ((uint32_t*)p)[i] = (uint32_t)(int64_t)v; break;
-        case 1: ((int32_t*)p)[i] = (int32_t)(int64_t)v; break;
+        case 0: ((uint32_t*)p)[i] = x86_to_u32(v); break;
+        case 1: ((int32_t*)p)[i] = x86_to_i32(v); break;
         case 4: ((float*)p)[i] = (float)v; break;
         default: ((double*)p)[i] = v; break;
     }

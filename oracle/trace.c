@@ -515,6 +515,16 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                     put(&r, i, (uint32_t)(int64_t)d, d);
                 }
                 free(a.d);
+            } else if (c == 122) {                         /* transpose (interp/InterpTranspose.cpp:13-60) */
+                Arr a = stk[--sp];
+                r = arr_new(a.prec, a.H, a.W, a.S);
+                const size_t e = esz(a.prec);
+                for (size_t s = 0; s < a.S; s++)
+                    for (size_t y = 0; y < a.H; y++)
+                        for (size_t x = 0; x < a.W; x++)
+                            memcpy((char*)r.d + ((s * a.W + x) * a.H + y) * e,
+                                   (const char*)a.d + ((s * a.H + y) * a.W + x) * e, e);
+                free(a.d);
             } else if (c == 131 || c == 132) {
                 Arr a = stk[--sp];
                 r = arr_new(a.prec, 1, 1, a.S);
@@ -549,24 +559,37 @@ int chai_or_trace_eval(const chai_trace* t, void* const* bufs)
                 const chai_trace_node* nb = &t->nodes[root[sp + 1]];
                 const int ta = na->flags & 1, tb = nb->flags & 1;
                 const int p = A.prec > B.prec ? A.prec : B.prec;
-                if (B.H == 1 && !tb && A.H > 1) {          /* matrix * vector */
-                    const size_t M = ta ? A.W : A.H, N = ta ? A.H : A.W;
-                    r = general ? C0 : arr_new(p, M, 1, 1);
-                    if (!general) C0.d = NULL;
-                    if (chai_or_matvec_interp(A.prec, B.prec, r.prec, ta, M, N, A.d, A.W, B.d, r.d,
-                                              general, n->s0, n->s1)) rc = -1;
-                } else if (A.H == 1 && !ta && B.H > 1) {   /* vector * matrix */
-                    const size_t M = tb ? B.H : B.W, N = tb ? B.W : B.H;
-                    r = general ? C0 : arr_new(p, M, 1, 1);
-                    if (!general) C0.d = NULL;
-                    if (chai_or_matvec_interp(B.prec, A.prec, r.prec, !tb, M, N, B.d, B.W, A.d, r.d,
-                                              general, n->s0, n->s1)) rc = -1;
+                /* slots (interp/InterpMatmulBase.cpp:113-118, BaseInterp.cpp:79-98):
+                 * max(A.S, B.S) products, C's count for matmulG; instance s
+                 * reads A's instance s mod A.S and B's s mod B.S */
+                const size_t S = general ? C0.S : maxz(A.S, B.S);
+                const size_t ea = A.W * A.H * esz(A.prec), eb = B.W * B.H * esz(B.prec);
+                size_t M, N, K = 0;
+                int kind;                                   /* 0 mat*vec, 1 vec*mat, 2 mat*mat */
+                if (B.H == 1 && !tb && A.H > 1) { kind = 0; M = ta ? A.W : A.H; N = ta ? A.H : A.W; }
+                else if (A.H == 1 && !ta && B.H > 1) { kind = 1; M = tb ? B.H : B.W; N = tb ? B.W : B.H; }
+                else { kind = 2; M = ta ? A.W : A.H; K = ta ? A.H : A.W; N = tb ? B.H : B.W; }
+                if (general) {
+                    r = C0;
                 } else {
-                    const size_t M = ta ? A.W : A.H, K = ta ? A.H : A.W, N = tb ? B.H : B.W;
-                    r = general ? C0 : arr_new(p, N, M, 1);
-                    if (!general) C0.d = NULL;
-                    if (chai_or_matmul_interp(A.prec, B.prec, r.prec, ta, tb, M, N, K, A.d, A.W, B.d,
-                                              B.W, r.d, N, general, n->s0, n->s1)) rc = -1;
+                    r = kind == 2 ? arr_new(p, N, M, S) : arr_new(p, M, 1, S);
+                    C0.d = NULL;
+                }
+                const size_t ec = r.W * r.H * esz(r.prec);
+                for (size_t q = 0; q < S && rc == 0; q++) {
+                    const char* pa = (const char*)A.d + (q % A.S) * ea;
+                    const char* pb = (const char*)B.d + (q % B.S) * eb;
+                    char* pcq = (char*)r.d + q * ec;
+                    if (kind == 0) {
+                        if (chai_or_matvec_interp(A.prec, B.prec, r.prec, ta, M, N, pa, A.W, pb, pcq,
+                                                  general, n->s0, n->s1)) rc = -1;
+                    } else if (kind == 1) {
+                        if (chai_or_matvec_interp(B.prec, A.prec, r.prec, !tb, M, N, pb, B.W, pa, pcq,
+                                                  general, n->s0, n->s1)) rc = -1;
+                    } else {
+                        if (chai_or_matmul_interp(A.prec, B.prec, r.prec, ta, tb, M, N, K, pa, A.W, pb,
+                                                  B.W, pcq, N, general, n->s0, n->s1)) rc = -1;
+                    }
                 }
                 free(A.d); free(B.d);
             } else if (c == 40 || c == 41) {

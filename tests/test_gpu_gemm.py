@@ -8,6 +8,9 @@ Tolerances (stated per element, deterministic error bounds):
           so |x - hi| <= 2^-8 |x| and |(x - hi) - lo| <= 2^-8 |x - hi|; the dropped
           lo*lo term is below 2^-16 |a b|; three
           products accumulated per k)
+  BF16X6  |C - C64| <= (6K + 4) 2^-24 * sum_k |a_k b_k|   (three pieces carry all 24 bits;
+          the dropped products of order 3 and 4 are below 2^-24 |a b|; 6K
+          accumulations in two passes and one final add)
   F64     |C - C64| <= 2 K 2^-53 * sum_k |a_k b_k|
 and the interpreter's own result (product rounded to f32, double sum) lies
 within u * sum|ab| + u |C| of C64, so GPU-vs-interpreter uses the sum of both.
@@ -34,6 +37,8 @@ def bound(mode, K):
         # three products per k accumulate into one FP32 sum: 3K roundings
         g3 = 3 * K * U / (1 - 3 * K * U)
         return 3 * 2.0 ** -16 + g3
+    if mode == "bf16x6":
+        return (6 * K + 4) * U
     return 2 * K * 2.0 ** -53
 
 
@@ -48,7 +53,7 @@ def check(mode, C, A, B, oracle, ta=False, tb=False):
     return err
 
 
-MODES = {"fma": 1, "bf16x3": 2, "f64": 3}
+MODES = {"fma": 1, "bf16x3": 2, "f64": 3, "bf16x6": 5}
 
 
 @pytest.mark.parametrize("mode", ["fma", "bf16x3"])
@@ -357,3 +362,166 @@ def test_tuner_skips_a_variant_journalled_bad_output(chai, tmp_path):
         assert chai.memo_status(name)[0] == -1
     finally:
         chai.memo_close()
+
+
+def _int_operand(rng, shape, kind):
+    if kind == "u32":
+        return rng.integers(0, 2**32, size=shape, dtype=np.uint64).astype(np.uint32)
+    return rng.integers(-2**31, 2**31, size=shape, dtype=np.int64).astype(np.int32)
+
+
+def _dev(a):
+    """device tensor of a numpy u32 / i32 array (u32 travels as int32 bits)"""
+    import torch
+    return torch.from_numpy(a.view(np.int32)).cuda()
+
+
+@pytest.mark.parametrize("ka,kb", [("u32", "u32"), ("i32", "i32"), ("u32", "i32"), ("i32", "u32")])
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True)])
+def test_gemm_int_exact_interpreter_semantics(chai, oracle, ka, kb, ta, tb):
+    """integer x integer GEMM (CHAI_GEMM_INT): the interpreter's semantics,
+    pinned against the reference's own interpreter (tests/test_frontend_cpu.py):
+    products wrap in the promoted type, the double sum converts to C the x86
+    way (int32 overflow gives INT_MIN).  Bit-exact against the oracle, on
+    full-range operands (most int32 sums overflow) and on small ones."""
+    import torch
+    rng = np.random.default_rng(hash((ka, kb, ta, tb)) & 0xffff)
+    M, N, K = 70, 45, 130
+    for small in (False, True):
+        A = _int_operand(rng, (K, M) if ta else (M, K), ka)
+        B = _int_operand(rng, (N, K) if tb else (K, N), kb)
+        if small:
+            A = (A % 97).astype(A.dtype)
+            B = (B % 89).astype(B.dtype)
+        ref = oracle.matmul_interp(A, B, trans_a=ta, trans_b=tb)
+        pa = chai.U32 if ka == "u32" else chai.I32
+        pb = chai.U32 if kb == "u32" else chai.I32
+        # the result type is the promoted one: u32 only for u32 x u32 (u32 travels as int32 bits)
+        C = chai.gemm(_dev(A), _dev(B), trans_a=ta, trans_b=tb, prec_a=pa, prec_b=pb,
+                      out_dtype=torch.int32, prec_c=max(pa, pb))
+        got = C.cpu().numpy().view(ref.dtype)
+        assert np.array_equal(got, ref), (ka, kb, ta, tb, small)
+
+
+def test_gemm_int_general_batched_and_long_k(chai, oracle):
+    """matmulG on integers (tmp in the promoted type, then alpha tmp + beta C
+    in double), a batch sharing A, and K past 2^21 where the kernel sums in
+    double in k order as the interpreter does."""
+    import torch
+    rng = np.random.default_rng(11)
+    M, N, K = 33, 40, 50
+    A = _int_operand(rng, (M, K), "i32") % 1000
+    A = A.astype(np.int32)
+    B = _int_operand(rng, (K, N), "i32") % 1000
+    B = B.astype(np.int32)
+    C0 = _int_operand(rng, (M, N), "i32") % 5000
+    C0 = C0.astype(np.int32)
+    ref = oracle.matmul_interp(A, B, C=C0, alpha=1.5, beta=-2.0)
+    Cd = _dev(C0.copy())
+    chai.gemm(_dev(A), _dev(B), C=Cd, general=True, alpha=1.5, beta=-2.0)
+    assert np.array_equal(Cd.cpu().numpy(), ref)
+    # batch of 3 instances of B against one A
+    Bs = np.stack([(_int_operand(rng, (K, N), "u32") % 7919).astype(np.uint32) for _ in range(3)])
+    Cb = chai.gemm(_dev(A), _dev(Bs.reshape(-1, N)).reshape(3, K, N), prec_b=chai.U32, out_dtype=torch.int32)
+    for i in range(3):
+        assert np.array_equal(Cb[i].cpu().numpy(), oracle.matmul_interp(A, Bs[i]))
+    # K = 2^21 + 96: the double accumulator rounds; emulated in k order
+    K2 = (1 << 21) + 96
+    A2 = _int_operand(rng, (2, K2), "u32")
+    B2 = _int_operand(rng, (K2, 3), "u32")
+    ref2 = oracle.matmul_interp(A2, B2)
+    C2 = chai.gemm(_dev(A2), _dev(B2), prec_a=chai.U32, prec_b=chai.U32, out_dtype=torch.int32,
+                   prec_c=chai.U32)
+    assert np.array_equal(C2.cpu().numpy().view(np.uint32), ref2)
+
+
+def test_gemv_int_exact(chai, oracle):
+    import torch
+    rng = np.random.default_rng(12)
+    M, N = 300, 257
+    A = _int_operand(rng, (M, N), "i32")
+    Ad = _dev(A)
+    for ta in (False, True):
+        xv = (_int_operand(rng, (M if ta else N,), "i32") % 1001).astype(np.int32)
+        ref = oracle.matvec_interp(A, xv, trans_a=ta)
+        y = chai.gemv(Ad, _dev(xv), trans_a=ta)
+        assert y.dtype == torch.int32
+        assert np.array_equal(y.cpu().numpy(), ref), ta
+
+
+def test_config1_against_the_interpreters_result(chai, oracle):
+    """config 1 (f32 256^3, A and B from Chai's Philox stream, seed 42): the
+    GPU's product against the reference interpreter's own result, kept in
+    tests/golden/interp_fixtures.npz.  The interpreter sums in double and
+    rounds once (within 2^-24 |C| of exact), so |C - C_interp| <=
+    (3 2^-16 + gamma_3K + 2^-23) sum|a b| for the split-BF16x3 family, and
+    gamma_(K+1) + 2^-23 for the FP32-FMA one."""
+    import os
+    import torch
+    fix = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "interp_fixtures.npz"))
+    ref = fix["mm_f32.out.C"].reshape(256, 256).astype(np.float64)
+    A = oracle.rng_fill(1230, 4, 4, 42, 256, 256, limits=(-1.0, 1.0)).reshape(256, 256)
+    B = oracle.rng_fill(1230, 4, 4, 42 + 65536, 256, 256, limits=(-1.0, 1.0)).reshape(256, 256)
+    mag = np.abs(A.astype(np.float64)) @ np.abs(B.astype(np.float64))
+    K = 256
+    for mode, coef in [(chai.GEMM_BF16X3, bound("bf16x3", K) + 2.0 ** -23),
+                       (chai.GEMM_FMA, bound("fma", K) + 2.0 ** -23)]:
+        C = chai.gemm(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), mode=mode).cpu().numpy()
+        assert np.all(np.abs(C - ref) <= coef * mag), mode
+
+
+@pytest.mark.parametrize("M,N,K,ta,tb", [(256, 256, 256, False, False), (300, 520, 200, True, False),
+                                         (129, 77, 1000, False, True), (512, 384, 4096, False, False)])
+def test_gemm_bf16x6_bound_and_variants(chai, oracle, M, N, K, ta, tb):
+    """Split-BF16x6: within (6K + 4) 2^-24 sum|ab| of the FP64 product, and every
+    tile variant gives the same C (same k order for each output)."""
+    A = unif(oracle, 21, K if ta else M, M if ta else K)
+    B = unif(oracle, 22, N if tb else K, K if tb else N)
+    At, Bt = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    A64 = (A.T if ta else A).astype(np.float64)
+    B64 = (B.T if tb else B).astype(np.float64)
+    C64, mag = A64 @ B64, np.abs(A64) @ np.abs(B64)
+    first = None
+    for v in range(len(chai.gemm_variants(chai.GEMM_BF16X6))):
+        C = chai.gemm(At, Bt, trans_a=ta, trans_b=tb, mode=chai.GEMM_BF16X6, variant=v).cpu().numpy()
+        assert np.all(np.abs(C - C64) <= bound("bf16x6", K) * mag), v
+        if first is None:
+            first = C
+        else:
+            assert np.array_equal(C, first), v
+
+
+def test_gemm_bf16x6_general_epilogue_and_batch(chai, oracle):
+    M, N, K = 200, 160, 320
+    A = unif(oracle, 31, M, K)
+    Bs = np.stack([unif(oracle, 32 + i, K, N) for i in range(3)])
+    C0 = unif(oracle, 40, M, N)
+    At, Bt = torch.from_numpy(A).cuda(), torch.from_numpy(Bs).cuda()
+    C = torch.from_numpy(np.stack([C0] * 3)).cuda()
+    chai.gemm(At, Bt, C=C, general=True, alpha=0.5, beta=-1.25, mode=chai.GEMM_BF16X6)
+    for i in range(3):
+        ref = 0.5 * (A.astype(np.float64) @ Bs[i].astype(np.float64)) - 1.25 * C0
+        mag = 0.5 * (np.abs(A.astype(np.float64)) @ np.abs(Bs[i].astype(np.float64))) + 1.25 * np.abs(C0)
+        assert np.all(np.abs(C[i].cpu().numpy() - ref) <= bound("bf16x6", K) * mag + U * np.abs(ref)), i
+    epi = chai.epilogue(0.5, 1.0, 0.0)
+    E = chai.gemm(At, Bt[0], mode=chai.GEMM_BF16X6, epi=epi).cpu().numpy()
+    P = chai.gemm(At, Bt[0], mode=chai.GEMM_BF16X6).cpu().numpy()
+    assert np.array_equal(E, np.maximum(P * np.float32(0.5) + np.float32(1.0), np.float32(0.0)))
+
+
+@pytest.mark.parametrize("lo", [-1.0, 0.0])
+@pytest.mark.parametrize("K", [256, 2048, 8192, 16384])
+def test_auto_family_within_twice_fp32_error(chai, oracle, lo, K):
+    """CHAI_GEMM_AUTO for f32: the family it picks keeps the error against FP64
+    within twice the FP32-FMA kernel's, in max and in rms (M = N = 256, every
+    element, operands uniform in [lo, 1) from Chai's stream)."""
+    M = N = 256
+    A = oracle.rng_fill(1230, 4, 4, 42, K, M, limits=(lo, 1.0)).reshape(M, K)
+    B = oracle.rng_fill(1230, 4, 4, 42 + M * K, N, K, limits=(lo, 1.0)).reshape(K, N)
+    At, Bt = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    C64 = A.astype(np.float64) @ B.astype(np.float64)
+    ea = np.abs(chai.gemm(At, Bt, mode=chai.GEMM_AUTO).cpu().numpy() - C64)
+    ef = np.abs(chai.gemm(At, Bt, mode=chai.GEMM_FMA).cpu().numpy() - C64)
+    rms = lambda e: float(np.sqrt((e ** 2).mean()))   # noqa: E731
+    print("K=%d lo=%g auto max %.3e rms %.3e fma max %.3e rms %.3e" % (K, lo, ea.max(), rms(ea), ef.max(), rms(ef)))
+    assert ea.max() <= 2 * ef.max() and rms(ea) <= 2 * rms(ef)
