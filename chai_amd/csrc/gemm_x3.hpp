@@ -12,11 +12,12 @@
 //   WGM x WGN waves per block; each wave owns a (BM/WGM) x (BN/WGN) tile
 // LDS holds two stages of the four panels Ah, Al, Bh, Bl; rows are 16-byte
 // chunks XOR-swizzled so that the rows one MFMA operand read touches fall
-// into distinct banks.  Stage k+1 is copied from global memory straight into
+// into distinct banks.  Stages are copied from global memory straight into
 // LDS (global_load_lds_dwordx4: lane i of a wave fills the i-th 16 bytes
 // after the wave's LDS base, so each lane fetches the source chunk that the
-// swizzle puts at its position) while the MFMAs of stage k run; one vmcnt
-// wait and one barrier close each stage.  No staging registers, no ds_write.
+// swizzle puts at its position): stage k+2 is issued right after the barrier
+// that frees its buffer and lands while stage k+1 is multiplied; one vmcnt
+// wait and one barrier per stage.  No staging registers, no ds_write.
 // The kernel body avoids lambdas and keeps every per-thread array fully
 // unrolled so that nothing lands in scratch.
 #pragma once
@@ -52,6 +53,9 @@ struct X3Shape {
     static constexpr int IA = PA / 64 / WAVES, IB = PB / 64 / WAVES;   // LDS-DMA issues per panel
     static constexpr int KB = BK / 16;
     static constexpr size_t LDS_BYTES = 2 * (size_t)STAGE * 16;
+    // LDS reads and MFMAs of one k block, without the zero panels (SKIP)
+    template <int SKIP> static constexpr int frag_reads() { return TI * ((SKIP & 1) ? 1 : 2) + TJ * ((SKIP & 2) ? 1 : 2); }
+    template <int SKIP> static constexpr int frag_mfmas() { return TI * TJ * (1 + !(SKIP & 1) + !(SKIP & 2)); }
     static_assert(WM % 32 == 0 && WN % 32 == 0, "wave tile must be a multiple of 32");
     static_assert(BK == 32 || BK == 64, "BK is 32 or 64");
     static_assert(PA % THREADS == 0 && PB % THREADS == 0, "panel chunks must split evenly");
@@ -107,45 +111,68 @@ __device__ __forceinline__ void x3_frag_offsets(int wm, int wn, int lane, int (&
     }
 }
 
-// The MFMAs of one LDS stage (at chunk offset cur): per 16-deep k block the
-// three products ah*bl, al*bh, ah*bh, minus those with a zero panel (SKIP).
+// One 16-deep k block's operand fragments of a wave: hi and lo of its TI A
+// tiles and TJ B tiles (a register slot of the fragment pipeline).
+template <typename S>
+struct X3Frag {
+    bf16x8 ah[S::TI], al[S::TI], bh[S::TJ], bl[S::TJ];
+};
+
+// Read k block kb of the LDS stage at chunk offset cur into f, minus the zero
+// panels (SKIP).
+template <typename S, int SKIP>
+__device__ __forceinline__ void x3_load_frag(const u32x4* lds, int cur, int kb, const int (&offA)[S::TI][S::KB],
+                                             const int (&offB)[S::TJ][S::KB], X3Frag<S>& f)
+{
+#pragma unroll
+    for (int i = 0; i < S::TI; i++) {
+        f.ah[i] = as_bf16x8(lds[cur + offA[i][kb]]);
+        if (!(SKIP & 1)) f.al[i] = as_bf16x8(lds[cur + S::PA + offA[i][kb]]);
+    }
+#pragma unroll
+    for (int j = 0; j < S::TJ; j++) {
+        f.bh[j] = as_bf16x8(lds[cur + offB[j][kb]]);
+        if (!(SKIP & 2)) f.bl[j] = as_bf16x8(lds[cur + S::PB + offB[j][kb]]);
+    }
+}
+
+// The MFMAs of one k block: the three products ah*bl, al*bh, ah*bh (minus
+// those with a zero panel), each as a pass over the tiles so that consecutive
+// MFMAs write different accumulators.
+template <typename S, int SKIP>
+__device__ __forceinline__ void x3_frag_mfma(const X3Frag<S>& f, floatx16 (&acc)[S::TI][S::TJ])
+{
+    if (!(SKIP & 2)) {
+#pragma unroll
+        for (int i = 0; i < S::TI; i++)
+#pragma unroll
+            for (int j = 0; j < S::TJ; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bl[j], acc[i][j], 0, 0, 0);
+    }
+    if (!(SKIP & 1)) {
+#pragma unroll
+        for (int i = 0; i < S::TI; i++)
+#pragma unroll
+            for (int j = 0; j < S::TJ; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.al[i], f.bh[j], acc[i][j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < S::TI; i++)
+#pragma unroll
+        for (int j = 0; j < S::TJ; j++)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f.ah[i], f.bh[j], acc[i][j], 0, 0, 0);
+}
+
+// All MFMAs of one LDS stage (at chunk offset cur), k block by k block.
 template <typename S, int SKIP>
 __device__ __forceinline__ void x3_stage_mfma(const u32x4* lds, int cur, const int (&offA)[S::TI][S::KB],
                                               const int (&offB)[S::TJ][S::KB], floatx16 (&acc)[S::TI][S::TJ])
 {
 #pragma unroll
     for (int kb = 0; kb < S::KB; kb++) {
-        bf16x8 ah[S::TI], al[S::TI], bh[S::TJ], bl[S::TJ];
-#pragma unroll
-        for (int i = 0; i < S::TI; i++) {
-            ah[i] = as_bf16x8(lds[cur + offA[i][kb]]);
-            if (!(SKIP & 1)) al[i] = as_bf16x8(lds[cur + S::PA + offA[i][kb]]);
-        }
-#pragma unroll
-        for (int j = 0; j < S::TJ; j++) {
-            bh[j] = as_bf16x8(lds[cur + offB[j][kb]]);
-            if (!(SKIP & 2)) bl[j] = as_bf16x8(lds[cur + S::PB + offB[j][kb]]);
-        }
-        // three passes over the tiles: consecutive MFMAs write different accumulators
-        if (!(SKIP & 2)) {
-#pragma unroll
-            for (int i = 0; i < S::TI; i++)
-#pragma unroll
-                for (int j = 0; j < S::TJ; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
-        }
-        if (!(SKIP & 1)) {
-#pragma unroll
-            for (int i = 0; i < S::TI; i++)
-#pragma unroll
-                for (int j = 0; j < S::TJ; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-        }
-#pragma unroll
-        for (int i = 0; i < S::TI; i++)
-#pragma unroll
-            for (int j = 0; j < S::TJ; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+        X3Frag<S> f;
+        x3_load_frag<S, SKIP>(lds, cur, kb, offA, offB, f);
+        x3_frag_mfma<S, SKIP>(f, acc);
     }
 }
 
@@ -218,27 +245,60 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
             for (int r = 0; r < 16; r++) acc[i][j][r] = 0.0f;
 
     const int nk = (int)(Kp / BK);
-    // prologue: stage 0 -> LDS buffer 0
+    // prologue: stage 0 -> LDS buffer 0, visible to all waves; stage 1 -> buffer 1
     x3_issue_stage<S, SKIP>(a.Ah + a0, a.Al + a0, a.Bh + b0, a.Bl + b0, srcA, srcB, hiA, hiB, a.a_kblk,
                             a.b_kblk, lds, wave);
     wait_lds_dma();
     __syncthreads();
+    {
+        const uint64_t k1 = nk > 1 ? 1 : 0;
+        x3_issue_stage<S, SKIP>(a.Ah + a0 + k1 * kstepA, a.Al + a0 + k1 * kstepA, a.Bh + b0 + k1 * kstepB,
+                                a.Bl + b0 + k1 * kstepB, srcA, srcB, hiA, hiB, a.a_kblk, a.b_kblk,
+                                lds + S::STAGE, wave);
+    }
 
     int offA[S::TI][S::KB], offB[S::TJ][S::KB];
     x3_frag_offsets<S>(wm, wn, lane, offA, offB);
 
+    // Fragment pipeline: the registers of k block kb + 1 are read while the
+    // MFMAs of k block kb run.  The last k block of stage kt is multiplied
+    // after the stage barrier, beside the reads of stage kt + 1's first k
+    // block and the copy of stage kt + 2, so the matrix pipe is fed across
+    // the barrier.  KB is even, so k block kb always sits in slot kb & 1.
+    // Scheduling fences keep each group of MFMAs on its side of the barrier.
+    // The copy is issued every step (stage min(kt + 2, nk - 1)): in the last
+    // two steps it refills a buffer no wave reads again, which keeps the
+    // step free of branches.
+    X3Frag<S> f[2];
+    x3_load_frag<S, SKIP>(lds, 0, 0, offA, offB, f[0]);
     for (int kt = 0; kt < nk; kt++) {
         const int cur = (kt & 1) * S::STAGE;
-        // buffer (kt + 1) & 1 was last read in step kt - 1, closed by its barrier
-        if (kt + 1 < nk) {
-            const uint64_t oa = a0 + (uint64_t)(kt + 1) * kstepA, ob = b0 + (uint64_t)(kt + 1) * kstepB;
-            x3_issue_stage<S, SKIP>(a.Ah + oa, a.Al + oa, a.Bh + ob, a.Bl + ob, srcA, srcB, hiA, hiB,
-                                    a.a_kblk, a.b_kblk, lds + ((kt + 1) & 1) * S::STAGE, wave);
+        // k block 0 has arrived (its reads went out behind the last step's
+        // MFMAs); waiting here leaves the reads below free to stay in flight
+        __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0)
+#pragma unroll
+        for (int kb = 0; kb + 1 < S::KB; kb++) {
+            x3_load_frag<S, SKIP>(lds, cur, kb + 1, offA, offB, f[(kb + 1) & 1]);
+            x3_frag_mfma<S, SKIP>(f[kb & 1], acc);
+            // all reads of k block kb + 1 go out before the MFMAs of kb
+            __builtin_amdgcn_sched_group_barrier(0x100, S::template frag_reads<SKIP>(), 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, S::template frag_mfmas<SKIP>(), 0);
         }
-        x3_stage_mfma<S, SKIP>(lds, cur, offA, offB, acc);
-        wait_lds_dma();     // stage kt + 1 has landed (this wave's share)
-        __syncthreads();    // ... every wave's share, and stage kt is no longer read
+        __builtin_amdgcn_sched_barrier(0);
+        // every wave's reads of buffer cur have completed (the barrier waits
+        // for them) and stage kt + 1 has landed in the other buffer
+        wait_lds_dma();
+        __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        const int kn = kt + 1 < nk ? kt + 1 : kt;
+        x3_load_frag<S, SKIP>(lds, (kn & 1) * S::STAGE, 0, offA, offB, f[0]);
+        const uint64_t k2 = (uint64_t)(kt + 2 < nk ? kt + 2 : nk - 1);
+        const uint64_t oa = a0 + k2 * kstepA, ob = b0 + k2 * kstepB;
+        x3_issue_stage<S, SKIP>(a.Ah + oa, a.Al + oa, a.Bh + ob, a.Bl + ob, srcA, srcB, hiA, hiB, a.a_kblk,
+                                a.b_kblk, lds + cur, wave);
+        x3_frag_mfma<S, SKIP>(f[(S::KB - 1) & 1], acc);
     }
+    wait_lds_dma();     // the refills of the last two steps have landed: LDS is free on exit
 
     x3_store<S>(a, C, (uint64_t)tm * BM + wm, (uint64_t)tn * BN + wn, lane, acc,
                 a.add ? a.add + bz * a.add_bstride : nullptr);
