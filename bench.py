@@ -601,6 +601,12 @@ def run_ragged(args, leg=False):
 
 # ----------------------------------------------------------------- autotune
 def run_autotune(args):
+    """The learned-autotune workload: the exhaustive timed search of every
+    tile variant of the four modelled families (BF16X3, FMA, F64, GEMV) on
+    the training shapes, one MLP trained on all of them, then per family on
+    the held-out shapes: the model's choice against the search's best
+    (throughput regret) and the wall time of one inference against the
+    search's (cold start)."""
     import numpy as np
     import torch
     import chai_amd as chai
@@ -609,51 +615,75 @@ def run_autotune(args):
     ws = tuner.Workspace()
     train = tuner.random_shapes(args.train_shapes, seed=2024)
     test = tuner.random_shapes(args.test_shapes, seed=2025)
-    t_train, t_test, cold_ex = [], [], []
+    fams = tuner.FAMILIES
+    t_train, t_test, cold_ex = {}, {}, {}
     t0 = time.perf_counter()
-    for i, s in enumerate(train):
-        t_train.append(tuner.time_variants(ws, s)[0])
-        if i % 250 == 0:
-            print("# timed %d/%d train shapes (%.0f s)" % (i, len(train), time.perf_counter() - t0),
-                  file=sys.stderr, flush=True)
-    for s in test:
-        ms, secs = tuner.time_variants(ws, s)
-        t_test.append(ms)
-        cold_ex.append(secs)
+    for f in fams:
+        name = tuner.FAMILY_NAMES[f]
+        t_train[f] = []
+        for i, s in enumerate(train):
+            t_train[f].append(tuner.time_variants(ws, s, family=f)[0])
+            if i % 250 == 0:
+                print("# %s: timed %d/%d train shapes (%.0f s)" % (name, i, len(train), time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
+        t_test[f], cold_ex[f] = [], []
+        for s in test:
+            ms, secs = tuner.time_variants(ws, s, family=f)
+            t_test[f].append(ms)
+            cold_ex[f].append(secs)
+        t_train[f], t_test[f] = np.array(t_train[f]), np.array(t_test[f])
     collect_s = time.perf_counter() - t0
-    t_train, t_test = np.array(t_train), np.array(t_test)
-    X, Y = tuner.dataset(train, t_train)
+    Xs, Ys = [], []
+    for f in fams:
+        X, Y = tuner.dataset(train, t_train[f], f)
+        Xs.append(X)
+        Ys.append(Y)
+    X, Y = np.concatenate(Xs), np.concatenate(Ys)
     mu, sd = float(Y.mean()), float(Y.std())
     m = tuner.MLP(tuner.NFEAT, seed=0)
     t1 = time.perf_counter()
     m.fit(torch.from_numpy(X).cuda(), torch.from_numpy((Y - mu) / sd).cuda(), epochs=args.steps * 20,
           batch=256, lr=2e-3)
     train_s = time.perf_counter() - t1
-    tuner.predict(m, test[0])
-    choice, cold_m = [], []
-    for s in test:
-        v, _, secs = tuner.predict(m, s)
-        choice.append(v)
-        cold_m.append(secs)
-    reg = tuner.regret(choice, t_test)
-    fixed = int(np.argmin(np.log(t_train).sum(0)))
-    reg_fixed = tuner.regret([fixed] * len(test), t_test)
-    reg_v0 = tuner.regret([0] * len(test), t_test)
-    perf_ratio = float(np.mean(t_test.min(1) / t_test[np.arange(len(test)), choice]))
     if args.save_model:
         tuner.save_model(m, args.save_model, mu, sd)
-    line = base_line(args, round(float(np.mean(cold_m)), 6), "s", round(float(np.mean(cold_m)) * 1e3, 4),
-                     "f32 (MLP 3x64)", {"workload": "learned autotune: %d train / %d held-out shapes, M,N,K "
-                                        "log-uniform [64,16384] step 16, bf16x3 tile variants"
-                                        % (len(train), len(test))}, higher=False)
+    per = {}
+    cold_all_m, cold_all_ex = [], []
+    for f in fams:
+        tuner.predict(m, test[0], f)          # one-time setup
+        choice, cold_m = [], []
+        for s in test:
+            v, _, secs = tuner.predict(m, s, f)
+            choice.append(v)
+            cold_m.append(secs)
+        tt = t_test[f]
+        reg = tuner.regret(choice, tt)
+        fixed = int(np.argmin(np.log(t_train[f]).sum(0)))
+        per[tuner.FAMILY_NAMES[f]] = {
+            "variants": int(tt.shape[1]),
+            "regret_mean": float(reg.mean()), "regret_median": float(np.median(reg)),
+            "regret_p90": float(np.quantile(reg, 0.9)),
+            "perf_vs_exhaustive_mean": float(np.mean(tt.min(1) / tt[np.arange(len(test)), choice])),
+            "regret_fixed_best_variant": float(tuner.regret([fixed] * len(test), tt).mean()),
+            "fixed_variant": fixed,
+            "regret_random_variant": float(np.mean([tuner.regret([v] * len(test), tt).mean()
+                                                    for v in range(tt.shape[1])])),
+            "cold_start_model_s_mean": float(np.mean(cold_m)),
+            "cold_start_exhaustive_s_mean": float(np.mean(cold_ex[f])),
+            "cold_start_speedup": float(np.mean(cold_ex[f]) / np.mean(cold_m))}
+        cold_all_m += cold_m
+        cold_all_ex += cold_ex[f]
+    cm = float(np.mean(cold_all_m))
+    line = base_line(args, round(cm, 6), "s", round(cm * 1e3, 4), "f32 (MLP 3x64)",
+                     {"workload": "learned autotune: %d train / %d held-out shapes, M,N,K log-uniform [64,16384] "
+                                  "step 16; BF16X3 / FMA / F64 / GEMV tile variants" % (len(train), len(test))},
+                     higher=False)
     line["metric"] = METRIC
     line["autotune"] = {
-        "cold_start_model_s_mean": float(np.mean(cold_m)), "cold_start_exhaustive_s_mean": float(np.mean(cold_ex)),
-        "cold_start_speedup": float(np.mean(cold_ex) / np.mean(cold_m)),
-        "regret_mean": float(reg.mean()), "regret_median": float(np.median(reg)),
-        "regret_p90": float(np.quantile(reg, 0.9)), "perf_vs_exhaustive_mean": perf_ratio,
-        "regret_fixed_best_variant": float(reg_fixed.mean()), "fixed_variant": fixed,
-        "regret_variant0": float(reg_v0.mean()),
+        "families": per,
+        "regret_mean_worst_family": max(v["regret_mean"] for v in per.values()),
+        "cold_start_model_s_mean": cm, "cold_start_exhaustive_s_mean": float(np.mean(cold_all_ex)),
+        "cold_start_speedup": float(np.mean(cold_all_ex) / cm),
         "train_rows": int(X.shape[0]), "train_s": round(train_s, 2), "collect_s": round(collect_s, 1)}
     emit(line)
 

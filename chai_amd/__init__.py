@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(_HERE, "libchai_mi355x.so")
 U32, I32, F32, F64 = 0, 1, 4, 8
 PHILOX, THREEFRY = 1230, 1231
 OP_DOT, OP_MEAN, OP_SUM = 130, 131, 132
-GEMM_AUTO, GEMM_FMA, GEMM_BF16X3, GEMM_F64, GEMM_INT, GEMM_BF16X6 = 0, 1, 2, 3, 4, 5
+GEMM_AUTO, GEMM_FMA, GEMM_BF16X3, GEMM_F64, GEMM_INT, GEMM_BF16X6, GEMM_GEMV = 0, 1, 2, 3, 4, 5, 6
 
 _lib = None
 _inited = False

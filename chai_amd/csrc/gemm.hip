@@ -13,7 +13,8 @@
 //   gemm_bf16x3     : 256x256x32 tiles, 8 waves (2 x 4), each wave 128x64 built
 //                     from 32x32x16 bf16 MFMAs; three products hi*lo + lo*hi +
 //                     hi*hi accumulate in FP32; double-buffered XOR-swizzled LDS.
-//   sgemm_fma       : plain FP32 VALU FMA, 128x128x16 tiles, 8x8 outputs/thread.
+//   sgemm_fma       : plain FP32 VALU FMA, BM x BN x 16 tiles (BM, BN in {64, 128}),
+//                     (BM/16) x (BN/16) outputs per thread.
 //   dgemm_mfma      : FP64 with 16x16x4 f64 MFMAs, 64 x 64 per wave, three tile
 //                     shapes (128x128, 256x128, 128x256), stages copied into LDS directly.
 #include "common.hpp"
@@ -183,52 +184,66 @@ static int g_x3_default = 0;
 // plain FP32 VALU FMA GEMM
 // =====================================================================
 
-constexpr int F_BM = 128, F_BN = 128, F_BK = 16;
+constexpr int F_BK = 16;
 
-// A packed [Mp][Kp] f32, B packed [Kp][Np] f32 (both zero padded).
-__global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
+// A packed [Mp][Kp] f32, B packed [Kp][Np] f32 (both zero padded).  256
+// threads as 16 x 16; each owns TM = BM / 16 rows and TN = BN / 16 columns of
+// the BM x BN tile, in groups of 4 spaced 64 apart.  Every output is one fmaf
+// chain over k in order, so all tile shapes give the same bits.
+template <int BM, int BN, int MINB>
+__global__ __launch_bounds__(256, MINB) void sgemm_fma(GemmArgs a)
 {
-    __shared__ float As[2][F_BK][F_BM + 4];
-    __shared__ float Bs[2][F_BK][F_BN + 4];
+    constexpr int TM = BM / 16, TN = BN / 16;
+    constexpr int LA = BM / 64, LB = BN / 64;        // 16-byte loads per thread per stage
+    constexpr int TPR = BN / 4, RPP = 256 / TPR;     // B: threads per k row, k rows per pass
+    static_assert(TM % 4 == 0 && TN % 4 == 0 && LA >= 1 && LB >= 1, "tile shape");
+    __shared__ float As[2][F_BK][BM + 4];
+    __shared__ float Bs[2][F_BK][BN + 4];
     uint32_t tm, tn;
     tile_coords<8>(blockIdx.x, a.tiles_m, a.tiles_n, tm, tn);
     const uint64_t bz = blockIdx.z;
     const uint64_t Kp = a.Kp, Np = a.Np;
-    const float* A = a.Af + bz * a.pa_bstride + (uint64_t)tm * F_BM * Kp;
-    const float* B = a.Bf + bz * a.pb_bstride + (uint64_t)tn * F_BN;
+    const float* A = a.Af + bz * a.pa_bstride + (uint64_t)tm * BM * Kp;
+    const float* B = a.Bf + bz * a.pb_bstride + (uint64_t)tn * BN;
     float* C = a.C + bz * a.c_bstride;
     const int t = threadIdx.x;
     const int tx = t & 15, ty = t >> 4;
 
-    float acc[8][8];
+    float acc[TM][TN];
 #pragma unroll
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int j = 0; j < 8; j++) acc[i][j] = 0.0f;
+        for (int j = 0; j < TN; j++) acc[i][j] = 0.0f;
 
-    // A tile 128 x 16: thread loads rows (t>>2) and (t>>2)+64, 4 consecutive k at (t&3)*4
-    // B tile 16 x 128: thread loads k rows (t>>5) and (t>>5)+8, 4 consecutive n at (t&31)*4
-    f32x4 ra[2], rb[2];
+    // A tile BM x 16: thread loads rows (t >> 2) + 64 h, 4 consecutive k at (t & 3) * 4
+    // B tile 16 x BN: thread loads k rows t / TPR + RPP h, 4 consecutive n at (t % TPR) * 4
+    f32x4 ra[LA], rb[LB];
     auto gload = [&](int kt) {
         const uint64_t k0 = (uint64_t)kt * F_BK;
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < LA; h++) {
             const int row = (t >> 2) + h * 64;
             ra[h] = *reinterpret_cast<const f32x4*>(A + (uint64_t)row * Kp + k0 + (t & 3) * 4);
-            const int kr = (t >> 5) + h * 8;
-            rb[h] = *reinterpret_cast<const f32x4*>(B + (k0 + kr) * Np + (t & 31) * 4);
+        }
+#pragma unroll
+        for (int h = 0; h < LB; h++) {
+            const int kr = t / TPR + h * RPP;
+            rb[h] = *reinterpret_cast<const f32x4*>(B + (k0 + kr) * Np + (t % TPR) * 4);
         }
     };
     auto lstore = [&](int buf) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
+        for (int h = 0; h < LA; h++) {
             const int row = (t >> 2) + h * 64, kc = (t & 3) * 4;
             As[buf][kc + 0][row] = ra[h].x;
             As[buf][kc + 1][row] = ra[h].y;
             As[buf][kc + 2][row] = ra[h].z;
             As[buf][kc + 3][row] = ra[h].w;
-            const int kr = (t >> 5) + h * 8;
-            *reinterpret_cast<f32x4*>(&Bs[buf][kr][(t & 31) * 4]) = rb[h];
+        }
+#pragma unroll
+        for (int h = 0; h < LB; h++) {
+            const int kr = t / TPR + h * RPP;
+            *reinterpret_cast<f32x4*>(&Bs[buf][kr][(t % TPR) * 4]) = rb[h];
         }
     };
     const int nk = (int)(Kp / F_BK);
@@ -240,31 +255,46 @@ __global__ __launch_bounds__(256, 2) void sgemm_fma(GemmArgs a)
         if (kt + 1 < nk) gload(kt + 1);
 #pragma unroll 2
         for (int k = 0; k < F_BK; k++) {
-            const f32x4 a0 = *reinterpret_cast<const f32x4*>(&As[buf][k][ty * 4]);
-            const f32x4 a1 = *reinterpret_cast<const f32x4*>(&As[buf][k][ty * 4 + 64]);
-            const f32x4 b0 = *reinterpret_cast<const f32x4*>(&Bs[buf][k][tx * 4]);
-            const f32x4 b1 = *reinterpret_cast<const f32x4*>(&Bs[buf][k][tx * 4 + 64]);
-            const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
-            const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+            float av[TM], bv[TN];
 #pragma unroll
-            for (int i = 0; i < 8; i++)
+            for (int g = 0; g < TM / 4; g++) {
+                const f32x4 v = *reinterpret_cast<const f32x4*>(&As[buf][k][ty * 4 + g * 64]);
+                av[4 * g] = v.x; av[4 * g + 1] = v.y; av[4 * g + 2] = v.z; av[4 * g + 3] = v.w;
+            }
 #pragma unroll
-                for (int j = 0; j < 8; j++) acc[i][j] = __builtin_fmaf(av[i], bv[j], acc[i][j]);
+            for (int g = 0; g < TN / 4; g++) {
+                const f32x4 v = *reinterpret_cast<const f32x4*>(&Bs[buf][k][tx * 4 + g * 64]);
+                bv[4 * g] = v.x; bv[4 * g + 1] = v.y; bv[4 * g + 2] = v.z; bv[4 * g + 3] = v.w;
+            }
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) acc[i][j] = __builtin_fmaf(av[i], bv[j], acc[i][j]);
         }
         if (kt + 1 < nk) lstore(buf ^ 1);
         __syncthreads();
     }
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
-        const uint64_t row = (uint64_t)tm * F_BM + ty * 4 + (i & 3) + (i >> 2) * 64;
+    for (int i = 0; i < TM; i++) {
+        const uint64_t row = (uint64_t)tm * BM + ty * 4 + (i & 3) + (i >> 2) * 64;
         if (row >= a.M) continue;
 #pragma unroll
-        for (int j = 0; j < 8; j++) {
-            const uint64_t col = (uint64_t)tn * F_BN + tx * 4 + (j & 3) + (j >> 2) * 64;
+        for (int j = 0; j < TN; j++) {
+            const uint64_t col = (uint64_t)tn * BN + tx * 4 + (j & 3) + (j >> 2) * 64;
             if (col < a.N) store_c(C, row * a.ldc + col, acc[i][j], a);
         }
     }
 }
+
+struct FVariant {
+    int BM, BN;
+    void (*kernel)(GemmArgs);
+};
+// FP32-FMA tile variants (the tuner's search space for mode FMA): BM x BN
+// with the resident blocks per CU the register budget is sized for
+static const FVariant kFVariants[] = {{128, 128, sgemm_fma<128, 128, 2>}, {64, 128, sgemm_fma<64, 128, 3>},
+                                      {128, 64, sgemm_fma<128, 64, 3>}, {64, 64, sgemm_fma<64, 64, 4>}};
+static constexpr int kNumF = sizeof(kFVariants) / sizeof(kFVariants[0]);
 
 // =====================================================================
 // FP64 MFMA GEMM
@@ -667,6 +697,13 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         if (e != hipSuccess) return hip_fail(e, "gemm_x6 launch");
         return CHAI_OK;
     }
+    if (mode == CHAI_GEMM_GEMV) {
+        prof_mark(0, st);
+        prof_mark(1, st);
+        const int rc = gemv_launch(d, A, B, C, variant, st);
+        prof_mark(2, st);
+        return rc;
+    }
     if (mode == CHAI_GEMM_FMA || mode == CHAI_GEMM_F64) {
         const bool f64 = mode == CHAI_GEMM_F64;
         int dv = 0;
@@ -675,7 +712,13 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
             dv = variant >= 0 ? variant : f64_default_variant(M, N, nb);
             if (dv < 0) return CHAI_ERR_HIP;
         }
-        const uint64_t BMx = f64 ? kDVariants[dv].BM : F_BM, BNx = f64 ? kDVariants[dv].BN : F_BN;
+        int fv = 0;
+        if (!f64) {
+            if (variant >= kNumF) { set_error("gemm: fma variant %d out of range", variant); return CHAI_ERR_ARG; }
+            fv = variant >= 0 ? variant : 0;
+        }
+        const uint64_t BMx = f64 ? kDVariants[dv].BM : kFVariants[fv].BM;
+        const uint64_t BNx = f64 ? kDVariants[dv].BN : kFVariants[fv].BN;
         const uint64_t BKx = f64 ? D_BK : F_BK;
         const uint64_t Mp = round_up(M, BMx), Np = round_up(N, BNx), Kp = round_up(K, 32);
         const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
@@ -719,7 +762,7 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
             hipLaunchKernelGGL(v.kernel, grid, dim3(v.WGM * v.WGN * 64), v.lds, st, g);
         } else {
             g.Af = (const float*)pa; g.Bf = (const float*)pb; g.C = (float*)C;
-            sgemm_fma<<<grid, 256, 0, st>>>(g);
+            hipLaunchKernelGGL(kFVariants[fv].kernel, grid, dim3(256), 0, st, g);
         }
         hipError_t e = hipGetLastError();
         prof_mark(2, st);
@@ -739,7 +782,11 @@ namespace chai_hip {
 
 int gemm_num_variants(int mode)
 {
-    return (mode == CHAI_GEMM_BF16X3 || mode == CHAI_GEMM_BF16X6) ? kNumX3 : mode == CHAI_GEMM_F64 ? kNumD : 1;
+    return (mode == CHAI_GEMM_BF16X3 || mode == CHAI_GEMM_BF16X6) ? kNumX3
+           : mode == CHAI_GEMM_F64                                   ? kNumD
+           : mode == CHAI_GEMM_FMA                                   ? kNumF
+           : mode == CHAI_GEMM_GEMV                                  ? gemv_num_variants()
+                                                                     : 1;
 }
 
 int gemm_variant_params(int mode, int vid, int* p)
@@ -747,6 +794,12 @@ int gemm_variant_params(int mode, int vid, int* p)
     if (mode == CHAI_GEMM_F64 && vid >= 0 && vid < kNumD) {
         const DVariant& v = kDVariants[vid];
         p[0] = v.BM; p[1] = v.BN; p[2] = D_BK; p[3] = v.WGM; p[4] = v.WGN;
+        return CHAI_OK;
+    }
+    if (mode == CHAI_GEMM_GEMV) return gemv_variant_params(vid, p);
+    if (mode == CHAI_GEMM_FMA && vid >= 0 && vid < kNumF) {
+        const FVariant& v = kFVariants[vid];
+        p[0] = v.BM; p[1] = v.BN; p[2] = F_BK; p[3] = 4; p[4] = 1;   // 4 waves, each 16 / 4 thread rows
         return CHAI_OK;
     }
     if ((mode != CHAI_GEMM_BF16X3 && mode != CHAI_GEMM_BF16X6) || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;

@@ -25,12 +25,26 @@ extern "C" int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A,
     const bool ints_only = d.prec_a <= CHAI_I32 && d.prec_b <= CHAI_I32;
     if (ints_only || d.mode == CHAI_GEMM_INT) {
         if (!ints_only || (d.mode != CHAI_GEMM_AUTO && d.mode != CHAI_GEMM_INT)) {
-            set_error("gemm: CHAI_GEMM_INT takes integer x integer operands, and they no other family");
+            set_error("gemm: integer x integer operands take CHAI_GEMM_INT (or AUTO), and CHAI_GEMM_INT takes no others");
             return CHAI_ERR_ARG;
         }
         if (epi && epi->kind != CHAI_EPI_NONE) { set_error("gemm: epilogues apply to f32 results only"); return CHAI_ERR_ARG; }
         if (d.K == 0) { set_error("gemm: K == 0"); return CHAI_ERR_ARG; }
         return igemm_launch(d, A, B, C, as_stream(stream));
+    }
+    if (d.mode == CHAI_GEMM_GEMV) {
+        const int want_c = any_f64 ? CHAI_F64 : CHAI_F32;
+        if (d.N != 1 || (!d.trans_b && d.ldb != 1 && d.K > 1) || (d.ldc != 1 && d.M > 1)) {
+            set_error("gemm: CHAI_GEMM_GEMV takes one contiguous column op(B) and writes one contiguous column C");
+            return CHAI_ERR_ARG;
+        }
+        if (d.prec_c != want_c) { set_error("gemm: CHAI_GEMM_GEMV writes the promoted precision (%d)", want_c); return CHAI_ERR_ARG; }
+        if (epi && epi->kind != CHAI_EPI_NONE) { set_error("gemm: CHAI_GEMM_GEMV has no epilogue"); return CHAI_ERR_ARG; }
+        if (d.K == 0) { set_error("gemm: K == 0"); return CHAI_ERR_ARG; }
+        if (d.variant >= gemv_num_variants()) { set_error("gemm: variant %d out of range for GEMV", d.variant); return CHAI_ERR_ARG; }
+        const int tv = d.variant >= 0 ? d.variant : tuned_variant(d, CHAI_GEMM_GEMV);
+        return gemm_launch(d, A, B, C, CHAI_GEMM_GEMV, tv >= 0 ? tv : gemv_default_variant(d), as_stream(stream),
+                           nullptr);
     }
     int mode = d.mode;
     if (mode == CHAI_GEMM_AUTO) mode = gemm_auto_mode(d);

@@ -55,5 +55,10 @@ int tuned_variant(const chai_gemm_desc& d, int mode);
 int gemm_auto_mode(const chai_gemm_desc& d);
 // integer x integer products with the interpreter's semantics (igemm.hip)
 int igemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, hipStream_t st);
+// the matrix-vector family (gemv.hip): variants, the untuned choice, launch
+int gemv_num_variants();
+int gemv_variant_params(int vid, int* p);
+int gemv_default_variant(const chai_gemm_desc& d);
+int gemv_launch(const chai_gemm_desc& d, const void* A, const void* x, void* y, int variant, hipStream_t st);
 
 }  // namespace chai_hip

@@ -234,38 +234,84 @@ static int mlp_run(Mlp* N, const float* X, const float* Y, size_t B, float* out,
 }
 
 // ------------------------------------------------------------ GEMM features
-// The model's input for (shape, variant): scaled so every feature is O(1).
-static void gemm_features(const chai_gemm_desc& d, int variant, float* f)
+// The families the model covers, in one-hot order.  BF16X6 runs the BF16X3
+// tiles (twice) and shares their slot.
+static int family_slot(int mode)
 {
-    int p[5] = {256, 256, 32, 2, 4};
-    (void)gemm_variant_params(CHAI_GEMM_BF16X3, variant, p);
-    const double BM = p[0], BN = p[1], BK = p[2], waves = p[3] * p[4];
+    switch (mode) {
+        case CHAI_GEMM_BF16X3:
+        case CHAI_GEMM_BF16X6: return 0;
+        case CHAI_GEMM_FMA: return 1;
+        case CHAI_GEMM_F64: return 2;
+        case CHAI_GEMM_GEMV: return 3;
+        default: return -1;
+    }
+}
+
+// The model's input for (shape, family, variant), every feature O(1): the
+// shape and operand types, the family, the variant's tile parameters, and a
+// crude occupancy model of the launch (resident blocks per CU from LDS and
+// waves, rounds of blocks over 256 CUs, padding waste).
+static void gemm_features(const chai_gemm_desc& d, int mode, int variant, float* f)
+{
+    const int fam = family_slot(mode);
+    int p[5] = {0, 0, 0, 0, 0};
+    (void)gemm_variant_params(mode == CHAI_GEMM_BF16X6 ? CHAI_GEMM_BF16X3 : mode, variant, p);
     const double M = (double)d.M, N = (double)d.N, K = (double)d.K;
-    const double tm = std::ceil(M / BM), tn = std::ceil(N / BN);
-    const double tiles = tm * tn;
-    // resident blocks per CU: LDS (two stages of four bf16 panels) and waves
-    const double lds = 2.0 * (2 * BM + 2 * BN) * BK * 2;
-    double per_cu = std::floor(160.0 * 1024 / lds);
-    per_cu = std::min(per_cu, std::floor(32.0 / waves));
-    per_cu = std::max(per_cu, 1.0);
-    const double rounds = tiles / (256.0 * per_cu);
-    const double Kp = std::ceil(K / 64) * 64;
+    const double nb = (double)(d.batch ? d.batch : 1);
+    for (int i = 0; i < CHAI_GEMM_FEATURES; i++) f[i] = 0.f;
     f[0] = (float)(std::log2(M) / 14);
     f[1] = (float)(std::log2(N) / 14);
     f[2] = (float)(std::log2(K) / 14);
-    f[3] = (float)(d.trans_a != 0 || d.trans_b != 0);   // (affects only the split pre-pass)
-    f[4] = (float)(std::log2((BM / p[3]) / (BN / p[4])) / 2);   // wave tile aspect log2(WM / WN) / 2
+    f[3] = (float)(d.trans_a != 0);
+    f[4] = (float)(d.trans_b != 0);
     f[5] = (float)(d.prec_a == CHAI_I32 || d.prec_a == CHAI_U32);
     f[6] = (float)(d.prec_b == CHAI_I32 || d.prec_b == CHAI_U32);
-    f[7] = (float)(std::log2(BM) / 8);
-    f[8] = (float)(std::log2(BN) / 8);
-    f[9] = (float)(BK / 64);
-    f[10] = (float)(waves / 8);
-    f[11] = (float)(std::log2(tiles) / 16);
-    f[12] = (float)(std::log2(rounds) / 8);
-    f[13] = (float)(rounds / std::ceil(rounds));        // last-round occupancy
-    f[14] = (float)(M * N / (tm * BM * tn * BN));       // useful fraction of the tile grid
-    f[15] = (float)(K / Kp);
+    f[7] = (float)(d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64);
+    if (fam >= 0) f[8 + fam] = 1.f;
+    double BM = 1, BN = 1, BK = 1, waves = 4, lds = 1024, kq = 1, aspect = 0, tiles = 1, used = 1;
+    if (fam == 3) {
+        const double split = p[0], lif = p[1];
+        if (!d.trans_a) {           // rows of A: 4 / split per block
+            BM = 4 / split; BN = 1; BK = 256 * lif * split;
+            tiles = std::ceil(M / BM) * nb;
+            used = M / (std::ceil(M / BM) * BM);
+        } else {                    // 1024 columns per block, row chunks for ~1024 split blocks
+            const double cb = std::ceil(M / 1024);
+            const double chunks = std::max(1.0, std::min(K, std::floor(1024 * split / (cb * nb))));
+            BM = 1024; BN = 1; BK = std::ceil(K / chunks);
+            tiles = cb * chunks * nb;
+            used = M / (cb * 1024);
+        }
+        waves = 4; lds = 64;
+        aspect = std::log2(split) / 2;
+        f[17] = (float)(std::log2(lif) / 3);
+    } else {
+        BM = p[0] > 0 ? p[0] : 128; BN = p[1] > 0 ? p[1] : 128; BK = p[2] > 0 ? p[2] : 16;
+        waves = std::max(1, p[3] * p[4]);
+        if (fam == 0) { lds = 2.0 * (2 * BM + 2 * BN) * BK * 2; kq = 64; }
+        else if (fam == 1) { lds = 2.0 * 16 * (BM + BN + 8) * 4; kq = 16; waves = 4; }
+        else { lds = 2.0 * 8 * (BM + BN + 32) * 8; kq = 32; }
+        aspect = std::log2((BM / std::max(1, p[3])) / (BN / std::max(1, p[4]))) / 2;
+        const double tm = std::ceil(M / BM), tn = std::ceil(N / BN);
+        tiles = tm * tn * nb;
+        used = M * N / (tm * BM * tn * BN);
+    }
+    double per_cu = std::floor(160.0 * 1024 / lds);
+    per_cu = std::max(1.0, std::min(per_cu, std::floor(32.0 / waves)));
+    const double rounds = tiles / (256.0 * per_cu);
+    const double Kp = std::ceil(K / kq) * kq;
+    f[12] = (float)(std::log2(BM) / 10);
+    f[13] = (float)(std::log2(BN) / 8);
+    f[14] = (float)(std::log2(BK) / 14);
+    f[15] = (float)(waves / 8);
+    f[16] = (float)aspect;
+    f[18] = (float)(std::log2(tiles) / 16);
+    f[19] = (float)(std::log2(rounds) / 8);
+    f[20] = (float)(rounds / std::ceil(rounds));       // last-round occupancy
+    f[21] = (float)used;                               // useful fraction of the tile grid
+    f[22] = (float)(K / Kp);
+    f[23] = (float)(std::log2(nb) / 8);
 }
 
 }  // namespace chai_hip
@@ -385,13 +431,21 @@ int chai_hip_mlp_release(void* mlp)
     return CHAI_OK;
 }
 
+// the family a descriptor's mode stands for (AUTO resolves as chai_hip_gemm does)
+static int model_mode(const chai_gemm_desc& d)
+{
+    return d.mode != CHAI_GEMM_AUTO ? d.mode : gemm_auto_mode(d);
+}
+
 int chai_hip_gemm_features(const chai_gemm_desc* d, int variant, float* out)
 {
-    if (!d || !out || variant < 0 || variant >= gemm_num_variants(CHAI_GEMM_BF16X3) || !d->M || !d->N || !d->K) {
-        set_error("gemm_features: bad arguments");
+    if (!d || !out || !d->M || !d->N || !d->K) { set_error("gemm_features: bad arguments"); return CHAI_ERR_ARG; }
+    const int mode = model_mode(*d);
+    if (family_slot(mode) < 0 || variant < 0 || variant >= gemm_num_variants(mode)) {
+        set_error("gemm_features: no tile variant %d in family %d", variant, mode);
         return CHAI_ERR_ARG;
     }
-    gemm_features(*d, variant, out);
+    gemm_features(*d, mode, variant, out);
     return CHAI_OK;
 }
 
@@ -399,10 +453,12 @@ int chai_hip_gemm_predict_variant(const chai_gemm_desc* d, void* mlp, int* varia
 {
     Mlp* N = (Mlp*)mlp;
     if (!d || !N || !variant || N->L.F != CHAI_GEMM_FEATURES) { set_error("gemm_predict_variant: bad arguments"); return CHAI_ERR_ARG; }
-    const int nv = gemm_num_variants(CHAI_GEMM_BF16X3);
     if (!d->M || !d->N || !d->K) { set_error("gemm_predict_variant: empty shape"); return CHAI_ERR_ARG; }
+    const int mode = model_mode(*d);
+    if (family_slot(mode) < 0) { set_error("gemm_predict_variant: family %d has no tile variants", mode); return CHAI_ERR_ARG; }
+    const int nv = gemm_num_variants(mode);
     std::vector<float> X((size_t)nv * CHAI_GEMM_FEATURES), y(nv);
-    for (int v = 0; v < nv; v++) gemm_features(*d, v, &X[(size_t)v * CHAI_GEMM_FEATURES]);
+    for (int v = 0; v < nv; v++) gemm_features(*d, mode, v, &X[(size_t)v * CHAI_GEMM_FEATURES]);
     hipStream_t st = as_stream(stream);
     float* dX = nullptr;
     CHAI_HIP_TRY(hipMallocAsync((void**)&dX, sizeof(float) * (X.size() + nv), st));

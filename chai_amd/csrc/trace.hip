@@ -1767,6 +1767,7 @@ struct CacheEntry {
 };
 static std::mutex g_cache_mu;
 static std::map<uint64_t, std::vector<CacheEntry>> g_cache;
+static uint64_t g_cache_hits = 0, g_cache_misses = 0, g_cache_entries = 0;
 
 struct Handle {
     std::shared_ptr<TraceProgram> prog;
@@ -1797,8 +1798,10 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
             for (auto& e : it->second)
                 if (e.key == key) {
                     *handle = new Handle{e.prog};
+                    g_cache_hits++;
                     return CHAI_OK;
                 }
+        g_cache_misses++;
     }
     auto prog = std::make_shared<TraceProgram>();
     prog->nuser = trace->num_vars;
@@ -1812,8 +1815,18 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
     {
         std::lock_guard<std::mutex> lk(g_cache_mu);
         g_cache[h].push_back(CacheEntry{prog, key});
+        g_cache_entries++;
     }
     *handle = new Handle{prog};
+    return CHAI_OK;
+}
+
+extern "C" int chai_hip_trace_cache_stats(uint64_t* hits, uint64_t* misses, uint64_t* entries)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if (hits) *hits = g_cache_hits;
+    if (misses) *misses = g_cache_misses;
+    if (entries) *entries = g_cache_entries;
     return CHAI_OK;
 }
 

@@ -124,6 +124,9 @@ __global__ void count_mismatch(const T* a, const T* b, uint64_t n, unsigned int*
 //   BF16X3  |C - C64| <= (3 2^-16 + (3K + 2) 2^-24) S
 //   BF16X6  |C - C64| <= (6K + 4) 2^-24 S
 //   F64     |C - C64| <= (3K + 4) 2^-53 S
+//   GEMV    |C - C64| <= (3 2^-24 + (K + 2) 2^-52) S, f64 operands (K + 2) 2^-53 S
+//           (each product rounded once in f32, the FP64 sum, one final
+//           rounding; an int operand's conversion to f32 adds one more)
 // with S = sum_k |a_ik b_kj| (plus K 2^-126 for underflow).  Non-finite
 // values must match (NaN where the FP64 value is NaN, the same infinity).
 template <typename T>
@@ -184,7 +187,9 @@ static int validate_output(const chai_gemm_desc& d, int mode, const void* A, con
     s.stride_a = d.stride_a; s.stride_b = d.stride_b; s.stride_c = d.stride_c;
     s.same_a = d.same_a; s.same_b = d.same_b;
     const double K = (double)d.K;
+    const bool f64 = d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64;
     s.rel = mode == CHAI_GEMM_F64 ? (3 * K + 4) * 0x1p-53
+          : mode == CHAI_GEMM_GEMV ? (f64 ? (K + 2) * 0x1p-53 : 3 * 0x1p-24 + (K + 2) * 0x1p-52)
           : mode == CHAI_GEMM_FMA ? (K + 2) * 0x1p-24
           : mode == CHAI_GEMM_BF16X6 ? (6 * K + 4) * 0x1p-24
                                      : 3 * 0x1p-16 + (3 * K + 2) * 0x1p-24;
@@ -363,8 +368,10 @@ int chai_hip_gemm_time_variant(const chai_gemm_desc* desc, const void* A, const 
 // BEFORE_CHECK -> AFTER_CHECK, or BEFORE_CHECK -> BAD_OUTPUT (blacklisted,
 // also after a reload).  The first variant run must pass the FP64 check of
 // sampled rows (validate_output); it then becomes the reference.  Every tile
-// variant of a mode accumulates each output element in the same k order, so
-// every later one must give the reference's C, compared by value.
+// variant of a GEMM mode accumulates each output element in the same k order,
+// so every later one must give the reference's C, compared by value.  The
+// GEMV variants split the FP64 sum differently, so each of them is held to
+// the FP64 check instead.
 // alpha A B + beta C reads C, which the timed runs overwrite, so general
 // GEMMs are not checked.
 struct OutputCheck {
@@ -445,10 +452,10 @@ int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B,
             const std::string name = kernel_name(d, mode, v);
             { std::lock_guard<std::mutex> lk(g_mu); g_memo.add(name, BEFORE_CHECK); }
             // the first variant that passes the FP64 check becomes the
-            // reference; every later one must equal it
+            // reference; every later one must equal it (GEMV: each passes its own check)
             int ok = validate_output(d, mode, A, B, C, st);
             if (ok < 0) return ok;
-            if (ok) {
+            if (ok && mode != CHAI_GEMM_GEMV) {   // the GEMV variants order the FP64 sum differently
                 ok = check_output(ck, *desc, C, st);
                 if (ok < 0) return ok;
             }

@@ -300,6 +300,16 @@ class Trace:
         return Program(self, h, c)
 
 
+def cache_stats():
+    """(hits, misses, entries) of the compiled-trace cache."""
+    init()
+    L = lib()
+    L.chai_hip_trace_cache_stats.argtypes = [ctypes.POINTER(ctypes.c_uint64)] * 3
+    h, m, e = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+    check(L.chai_hip_trace_cache_stats(ctypes.byref(h), ctypes.byref(m), ctypes.byref(e)), "trace_cache_stats")
+    return h.value, m.value, e.value
+
+
 class Program:
     def __init__(self, trace, handle, c):
         self.trace = trace

@@ -1,14 +1,17 @@
-"""Learned tile choice for the split-BF16x3 GEMM family.
+"""Learned tile choice for the GEMM and GEMV kernel families.
 
-The reference chooses GEMM kernel parameters by timed search only
+The reference chooses GEMM / GEMV kernel parameters by timed search only
 (autotune/StandardEM.cpp:69-121, autotune/AutoTuneEM.cpp:48-131,
 enqueue/EnqueueMatmulAuto.cpp:71-161).  Here the same search
 (chai_hip_gemm_time_variant / chai_hip_gemm_tune) produces the training data of a
 small MLP (chai_hip_mlp_*: 3 x 64 ReLU, FP32, trained by the library's HIP
-kernels) that predicts every variant's core-kernel time from the shape; a new
-shape then gets its variant from one inference (chai_hip_gemm_predict_variant)
-instead of a search, and the search remains the fallback and measures the
-model's regret.
+kernels) that predicts every variant's core-kernel time from the shape, the
+family and the variant's tile parameters; a new shape then gets its variant
+from one inference (chai_hip_gemm_predict_variant) instead of a search, and
+the search remains the fallback and measures the model's regret.  One model
+covers four families: the split-BF16x3 MFMA tiles (8 variants; BF16X6 runs the
+same tiles), the FP32-FMA tiles (4), the FP64 MFMA tiles (3) and the GEMV
+kernels (9: waves per row or row chunks x loads in flight).
 """
 import ctypes
 import os
@@ -16,9 +19,13 @@ import time
 
 import numpy as np
 
-from . import (F32, GEMM_BF16X3, I32, GemmDesc, _stream, check, gemm_variants, init, lib)
+from . import (F32, F64, GEMM_BF16X3, GEMM_F64, GEMM_FMA, GEMM_GEMV, I32, GemmDesc, _stream, check,  # noqa: F401
+               gemm_variants, init, lib)
 
-NFEAT = 16          # CHAI_GEMM_FEATURES
+# the families the model covers (the one-hot order of the features)
+FAMILIES = (GEMM_BF16X3, GEMM_FMA, GEMM_F64, GEMM_GEMV)
+FAMILY_NAMES = {GEMM_BF16X3: "bf16x3", GEMM_FMA: "fma", GEMM_F64: "f64", GEMM_GEMV: "gemv"}
+NFEAT = 24          # CHAI_GEMM_FEATURES
 _vp, _sz = ctypes.c_void_p, ctypes.c_size_t
 
 
@@ -126,32 +133,44 @@ def random_shapes(n, seed, lo=64, hi=16384, step=16):
     return out
 
 
-def desc_of(shape, variant=-1):
+def desc_of(shape, variant=-1, family=GEMM_BF16X3):
+    """The GEMM descriptor of a random shape in a family: f32 (or i32 A) x f32
+    for the BF16X3 and FMA tiles, f64 x f64 for the FP64 tiles, and for GEMV
+    op(A) (M x K) times a K-vector."""
     M, N, K, ta, tb, pa = shape
     d = GemmDesc()
     d.prec_a, d.prec_b, d.prec_c = pa, F32, F32
+    if family == GEMM_F64:
+        d.prec_a, d.prec_b, d.prec_c = F64, F64, F64
+    if family == GEMM_GEMV:
+        N, tb = 1, 0
     d.trans_a, d.trans_b = ta, tb
     d.M, d.N, d.K = M, N, K
     d.lda = M if ta else K
     d.ldb = K if tb else N
     d.ldc = N
     d.batch = 1
-    d.mode = GEMM_BF16X3
+    d.mode = family
     d.variant = variant
     return d
 
 
-def features(shape, variant):
+def num_variants(family):
+    return len(gemm_variants(family))
+
+
+def features(shape, variant, family=GEMM_BF16X3):
     f = np.empty(NFEAT, dtype=np.float32)
-    d = desc_of(shape)
+    d = desc_of(shape, family=family)
     check(_bind().chai_hip_gemm_features(ctypes.byref(d), variant, f.ctypes.data), "chai_hip_gemm_features")
     return f
 
 
 class Workspace:
-    """Operand buffers big enough for every shape up to hi x hi (uniform data)."""
+    """Operand buffers big enough for every shape up to hi x hi (uniform data),
+    in f32, i32 and f64."""
 
-    def __init__(self, hi=16384):
+    def __init__(self, hi=16384, f64=True):
         import torch
         from . import rng_fill
         init()
@@ -163,24 +182,32 @@ class Workspace:
         rng_fill(self.A, seed=42, limits=(-1.0, 1.0))
         rng_fill(self.B, seed=43, limits=(-1.0, 1.0))
         self.Ai.copy_((self.A * 127).to(torch.int32))
+        self.Ad = self.Bd = self.Cd = None
+        if f64:
+            self.Ad = self.A.to(torch.float64)
+            self.Bd = self.B.to(torch.float64)
+            self.Cd = torch.empty(n, dtype=torch.float64, device="cuda")
         torch.cuda.synchronize()
 
-    def ptrs(self, shape):
+    def ptrs(self, shape, family=GEMM_BF16X3):
+        if family == GEMM_F64:
+            return self.Ad.data_ptr(), self.Bd.data_ptr(), self.Cd.data_ptr()
         pa = shape[5]
         return (self.Ai if pa == I32 else self.A).data_ptr(), self.B.data_ptr(), self.C.data_ptr()
 
 
-def time_variants(ws, shape, trials=3):
-    """core ms of every variant (the exhaustive search) and its wall time (s)."""
+def time_variants(ws, shape, trials=3, family=GEMM_BF16X3):
+    """core ms of every variant of the family (the exhaustive search) and its
+    wall time (s)."""
     import torch
     L = _bind()
-    nv = len(gemm_variants())
-    a, b, c = ws.ptrs(shape)
+    nv = num_variants(family)
+    a, b, c = ws.ptrs(shape, family)
     ms = np.empty(nv)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for v in range(nv):
-        d = desc_of(shape, v)
+        d = desc_of(shape, v, family)
         out = ctypes.c_float()
         check(L.chai_hip_gemm_time_variant(ctypes.byref(d), a, b, c, v, trials, ctypes.byref(out), _stream()),
               "chai_hip_gemm_time_variant")
@@ -189,11 +216,11 @@ def time_variants(ws, shape, trials=3):
     return ms, time.perf_counter() - t0
 
 
-def predict(model, shape):
+def predict(model, shape, family=GEMM_BF16X3):
     """(variant, per-variant model outputs, wall seconds) of one inference."""
     L = _bind()
-    nv = len(gemm_variants())
-    d = desc_of(shape)
+    nv = num_variants(family)
+    d = desc_of(shape, family=family)
     v = ctypes.c_int()
     pred = np.empty(nv, dtype=np.float32)
     t0 = time.perf_counter()
@@ -202,15 +229,15 @@ def predict(model, shape):
     return v.value, pred, time.perf_counter() - t0
 
 
-def dataset(shapes, times):
+def dataset(shapes, times, family=GEMM_BF16X3):
     """rows (shape, variant) -> features, target log(core ms)"""
     X, Y = [], []
     for s, ms in zip(shapes, times):
         for v, t in enumerate(ms):
             if t > 0:
-                X.append(features(s, v))
+                X.append(features(s, v, family))
                 Y.append(np.log(t))
-    return np.array(X, dtype=np.float32), np.array(Y, dtype=np.float32)
+    return np.array(X, dtype=np.float32).reshape(-1, NFEAT), np.array(Y, dtype=np.float32)
 
 
 def regret(choice, times):

@@ -124,7 +124,9 @@ int chai_hip_reduce_noalloc(int op, int prec, const void* in, size_t n, size_t s
 
 /* ------------------------------------------------------------------- GEMM */
 /* Kernel family selector.  AUTO picks F64 when any operand is f64, INT for
- * two integer operands, and BF16X3 otherwise (f32 or mixed int / f32). */
+ * two integer operands, and for f32 or mixed int / f32 operands the split-
+ * BF16 form within twice the FP32-FMA kernel's error: BF16X6 below
+ * K = 16384, BF16X3 from there. */
 enum {
     CHAI_GEMM_AUTO = 0,
     CHAI_GEMM_FMA = 1,      /* plain FP32 VALU FMA                        */
@@ -134,12 +136,20 @@ enum {
                                MFMA products (all of order <= 2), FP32
                                accumulate: FP32-FMA-level error at twice the
                                BF16X3 MFMA work                              */
-    CHAI_GEMM_INT = 4       /* integer x integer, the interpreter's exact
+    CHAI_GEMM_INT = 4,      /* integer x integer, the interpreter's exact
                                semantics (interp/InterpMatmulBase.hpp:49-76):
                                products wrap in the promoted 32-bit type, sum
                                in k order as a double accumulator would,
                                x86 conversion to C's type; AUTO picks it for
                                u32 / i32 operand pairs                       */
+    CHAI_GEMM_GEMV = 6      /* the matrix-vector kernels (chai_hip_gemv) for
+                               N == 1 with contiguous x and y (ldb == 1 unless
+                               trans_b, ldc == 1): HBM streaming of A, each
+                               product in the operands' promoted type summed
+                               in FP64, as the interpreter's matvec
+                               (interp/InterpMatmulBase.hpp:37-46).  Its tile
+                               variants split each dot product over waves or
+                               row chunks and vary the loads in flight       */
 };
 
 /* One GEMM call (or a batch of identical-shape instances).  Mirrors the
@@ -162,7 +172,10 @@ typedef struct chai_gemm_desc {
 
 /* The tile-variant search space of a kernel family (autotune/KernelParams.hpp's
  * endogenous parameters: work-group shape, inner blocking).  params receives
- * {BM, BN, BK, waves_m, waves_n}. */
+ * {BM, BN, BK, waves_m, waves_n}; for CHAI_GEMM_GEMV {split, loads in flight,
+ * 0, 0, 0}: waves per row of A (for op(A) = A^T: row chunks sized so that
+ * about 1024 x split blocks run) and 16-byte loads in flight per lane (the
+ * MV generator's g / bW space, autotune/EvergreenMatmulMV.cpp:189-264). */
 int chai_hip_gemm_num_variants(int mode, int* count);
 int chai_hip_gemm_variant_params(int mode, int variant, int* params);
 
@@ -280,14 +293,15 @@ int chai_hip_gemm_tune(const chai_gemm_desc* desc, const void* A, const void* B,
 int chai_hip_gemm_set_variant(const chai_gemm_desc* desc, int variant);
 
 /* ------------------------------------------------- learned tile-cost model */
-/* A 3 x 64 ReLU MLP (FP32) that predicts the core-kernel time of a GEMM tile
- * variant from CHAI_GEMM_FEATURES features, trained on the tuner's own
+/* A 3 x 64 ReLU MLP (FP32) that predicts the core-kernel time of a tile
+ * variant of the BF16X3 (and BF16X6), FMA, F64 and GEMV families from
+ * CHAI_GEMM_FEATURES features, trained on the tuner's own
  * timings with the library's HIP forward / backward / update kernels.  It has
  * no counterpart in the reference, whose only parameter choice is the timed
  * search (autotune/StandardEM.cpp:69-121); that search stays the fallback and
  * measures the model's regret.  X is [B][inputs] row-major f32, Y / out are
  * [B] f32, all device pointers.  The loss is mean_b (out_b - Y_b)^2. */
-enum { CHAI_GEMM_FEATURES = 16 };
+enum { CHAI_GEMM_FEATURES = 24 };
 enum { CHAI_MLP_SGD = 0, CHAI_MLP_ADAM = 1 };
 int chai_hip_mlp_create(int inputs, uint64_t seed, void** mlp);
 int chai_hip_mlp_num_params(void* mlp, size_t* count);
@@ -302,7 +316,8 @@ int chai_hip_mlp_update(void* mlp, const float* grad, int kind, float lr, void* 
 int chai_hip_mlp_train_step(void* mlp, const float* X, const float* Y, size_t B, int kind, float lr,
                             void* stream);
 int chai_hip_mlp_release(void* mlp);
-/* The model input of (shape, BF16X3 tile variant): CHAI_GEMM_FEATURES floats. */
+/* The model input of (shape, tile variant) for the family desc->mode names
+ * (AUTO resolves as in chai_hip_gemm): CHAI_GEMM_FEATURES floats. */
 int chai_hip_gemm_features(const chai_gemm_desc* desc, int variant, float* out);
 /* One inference over every variant of the shape; the lowest predicted time
  * wins and is recorded for the shape's exogenous key (as
@@ -370,9 +385,15 @@ typedef struct chai_trace {
 
 /* Compile (validate, type, group, emit VM programs).  Replaces MakeStmt /
  * WorkTrace / Enqueue text emission and the per-trace JustInTimeMemo
- * (enqueue/JustInTimeMemo.cpp:31-334): compiled handles are cached by the
- * trace's TEA hash (misc/TEA.cpp:62-89) and shape. */
+ * (enqueue/JustInTimeMemo.cpp:31-334): compiled programs are cached for the
+ * life of the process, keyed by the trace's full description (variables with
+ * their shapes, statements, nodes) and the engine (VM or JIT); a repeated
+ * trace shares the compiled program of the first. */
 int chai_hip_trace_compile(const chai_trace* trace, void** handle);
+/* Counters of that cache: compiles answered from it, compiles that did not
+ * find their trace in it, programs held.  JustInTimeMemo's lookup-or-build
+ * (enqueue/JustInTimeMemo.cpp:59-88) made observable. */
+int chai_hip_trace_cache_stats(uint64_t* hits, uint64_t* misses, uint64_t* entries);
 /* buffers[v] = device pointer of variable v (INPUT and OUTPUT required;
  * TEMP may be NULL, the library allocates one when it must be stored). */
 int chai_hip_trace_launch(void* handle, void* const* buffers, void* stream);

@@ -18,7 +18,13 @@ ORACLE_BACKEND = os.path.join(BIN, "liboracle_backend.so")
 REAL_LIB = os.path.join(ROOT, "chai_amd", "libchai_mi355x.so")
 
 # the reference's sample programs built by tests/frontend/Makefile
-SAMPLES = ["md5", "sum", "sum_pth", "loopsum", "matmul", "matmul_pth", "filter", "index"]
+SAMPLES = ["md5", "sum", "sum_pth", "loopsum", "matmul", "matmul_pth", "filter", "index", "sum_vec", "matmul_vec",
+           "matmul64", "cg", "kirch", "mingle", "pi", "monte"]
+# samples that draw random arrays: the interpreter's generator is glibc
+# random_r seeded with the seed XOR the generator object's address
+# (interp/InterpRNG.cpp:11-23), so its values change from run to run; the
+# device draws Chai's Random123 stream (the JIT's), which is deterministic
+RNG_SAMPLES = ["pi", "monte"]
 
 
 def built():

@@ -46,9 +46,28 @@ def test_device_failure_falls_back_to_interpreter(case, tmp_path):
         assert np.array_equal(ref[name].view(np.uint8), got[name].view(np.uint8)), (case, name)
 
 
-@pytest.mark.parametrize("sample", fe.SAMPLES)
+@pytest.mark.parametrize("sample", [s for s in fe.SAMPLES if s not in fe.RNG_SAMPLES])
 def test_reference_samples_over_oracle(sample):
     ref, _ = fe.run_sample(sample)
     got, err = fe.run_sample(sample, device="oracle")
     assert not _refused(err), err
     assert ref == got and ref.strip()
+
+
+def test_rng_samples_over_oracle():
+    """pi.cpp and monte.cpp draw Philox streams (RNG_DEFAULT = RNG_PHILOX).
+    The interpreter's values change from run to run, so its output is no
+    reference; the device's (over the oracle) is the same on every run, and
+    pi.cpp's estimate, the mean of 10^6 Bernoulli draws with p = pi / 4, is
+    within 5 standard deviations (5 * 4 sqrt(p (1 - p) / 10^6)) of pi."""
+    a, err = fe.run_sample("pi", device="oracle")
+    b, _ = fe.run_sample("pi", device="oracle")
+    assert not _refused(err), err
+    assert a == b
+    est = float(a.split("=")[1])
+    p = np.pi / 4
+    assert abs(est - np.pi) <= 5 * 4 * np.sqrt(p * (1 - p) / 1e6)
+    m1, err = fe.run_sample("monte", device="oracle")
+    m2, _ = fe.run_sample("monte", device="oracle")
+    assert not _refused(err), err
+    assert m1 == m2 and m1.startswith("mean is ")

@@ -10,7 +10,8 @@
   instances, and within the split bound of the FP64 product on a few of them.
 - The learned tile model (chai_amd/data/tile_mlp.npz, trained on autotuner
   timings of 2000 shapes) on the 500 held-out shapes of the learned-autotune
-  workload (M, N, K log-uniform in [64, 16384] step 16): mean throughput
+  workload (M, N, K log-uniform in [64, 16384] step 16), for each family it
+  covers (BF16X3, FMA, F64 tiles and the GEMV kernels): mean throughput
   regret against the exhaustive timed search at most 5 %, cold start (one
   inference against the timed search) at least 10 times shorter.
 """
@@ -121,22 +122,25 @@ def test_learned_tuner_on_held_out_config_shapes(chai):
     m = tuner.load_model(model_file)
     ws = tuner.Workspace(hi=16384)
     test = tuner.random_shapes(500, seed=2025)
-    tuner.predict(m, test[0])                 # one-time module setup
-    times, cold_ex, choice, cold_m = [], [], [], []
-    for i, s in enumerate(test):
-        ms, secs = tuner.time_variants(ws, s, trials=2)
-        times.append(ms)
-        cold_ex.append(secs)
-        v, _, secs_m = tuner.predict(m, s)
-        choice.append(v)
-        cold_m.append(secs_m)
-        if i % 100 == 0:
-            print("held-out shape %d/500" % i, flush=True)
-    times = np.array(times)
-    assert np.all(times > 0)
-    reg = tuner.regret(choice, times)
-    speed = np.mean(cold_ex) / np.mean(cold_m)
-    print("mean regret %.4f, median %.4f, p90 %.4f; cold start %.2e s vs %.2e s (%.0fx)"
-          % (reg.mean(), np.median(reg), np.quantile(reg, 0.9), np.mean(cold_m), np.mean(cold_ex), speed))
-    assert reg.mean() <= 0.05
-    assert speed >= 10.0
+    for f in tuner.FAMILIES:
+        name = tuner.FAMILY_NAMES[f]
+        tuner.predict(m, test[0], f)             # one-time module setup
+        times, cold_ex, choice, cold_m = [], [], [], []
+        for i, s in enumerate(test):
+            ms, secs = tuner.time_variants(ws, s, trials=2, family=f)
+            times.append(ms)
+            cold_ex.append(secs)
+            v, _, secs_m = tuner.predict(m, s, f)
+            choice.append(v)
+            cold_m.append(secs_m)
+            if i % 100 == 0:
+                print("%s: held-out shape %d/500" % (name, i), flush=True)
+        times = np.array(times)
+        assert np.all(times > 0)
+        reg = tuner.regret(choice, times)
+        speed = np.mean(cold_ex) / np.mean(cold_m)
+        print("%s: mean regret %.4f, median %.4f, p90 %.4f; cold start %.2e s vs %.2e s (%.0fx)"
+              % (name, reg.mean(), np.median(reg), np.quantile(reg, 0.9), np.mean(cold_m), np.mean(cold_ex), speed),
+              flush=True)
+        assert reg.mean() <= 0.05, name
+        assert speed >= 10.0, name

@@ -137,9 +137,12 @@ _NUM = re.compile(r"-?\d+\.?\d*(?:e[-+]?\d+)?")
 def test_reference_samples(sample):
     """The reference's sample programs, unchanged: the printed results (6
     significant digits) of the GPU run equal the interpreter's, up to one unit
-    in the last printed digit for floating-point sums and products."""
+    in the last printed digit for floating-point sums and products.  The
+    random samples (pi, monte) are compared with the same lowering over the
+    CPU oracle instead: the interpreter's generator is not Chai's Random123
+    stream and changes from run to run."""
     print("reference sample %s" % sample, flush=True)
-    ref, _ = fe.run_sample(sample)
+    ref, _ = fe.run_sample(sample, device="oracle" if sample in fe.RNG_SAMPLES else None)
     got, err = fe.run_sample(sample, device="gpu")
     assert "not available" not in err and "refused" not in err, err
     if sample == "md5":

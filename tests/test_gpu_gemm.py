@@ -217,8 +217,10 @@ def test_gemm_rejects_bad_arguments(chai):
     with pytest.raises(ValueError):
         chai.gemm(A, torch.ones(5, 4, device="cuda"))  # inner dimension mismatch (binding check)
     Ai = torch.ones(4, 4, dtype=torch.int32, device="cuda")
-    with pytest.raises(chai.ChaiError):
-        chai.gemm(Ai, Ai, out_dtype=torch.float32)
+    with pytest.raises(chai.ChaiError):     # integer x integer runs on the integer GEMM only
+        chai.gemm(Ai, Ai, mode=chai.GEMM_BF16X3, out_dtype=torch.float32)
+    with pytest.raises(chai.ChaiError):     # and the integer GEMM takes integer operands only
+        chai.gemm(A, A, mode=chai.GEMM_INT)
 
 
 def _bf16_rne(x):
@@ -525,3 +527,20 @@ def test_auto_family_within_twice_fp32_error(chai, oracle, lo, K):
     rms = lambda e: float(np.sqrt((e ** 2).mean()))   # noqa: E731
     print("K=%d lo=%g auto max %.3e rms %.3e fma max %.3e rms %.3e" % (K, lo, ea.max(), rms(ea), ef.max(), rms(ef)))
     assert ea.max() <= 2 * ef.max() and rms(ea) <= 2 * rms(ef)
+
+
+@pytest.mark.parametrize("M,N,K", [(300, 257, 129), (1000, 64, 2048), (64, 1000, 33)])
+def test_fma_tile_variants_give_the_same_bits(chai, M, N, K):
+    """Every FP32-FMA tile variant forms each output as one fmaf chain over k
+    in order, so all of them agree bit for bit (and the tuner compares them
+    by value)."""
+    g = torch.Generator(device="cpu").manual_seed(M + N + K)
+    A = (torch.rand(M, K, generator=g) * 2 - 1).cuda()
+    B = (torch.rand(K, N, generator=g) * 2 - 1).cuda()
+    nv = len(chai.gemm_variants(chai.GEMM_FMA))
+    assert nv == 4
+    ref = chai.gemm(A, B, mode=chai.GEMM_FMA, variant=0)
+    for v in range(1, nv):
+        assert torch.equal(chai.gemm(A, B, mode=chai.GEMM_FMA, variant=v), ref), v
+    err = (ref.double() - A.double() @ B.double()).abs()
+    assert torch.all(err <= (K + 2) * 2.0 ** -24 * (A.double().abs() @ B.double().abs()))

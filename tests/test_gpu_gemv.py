@@ -53,3 +53,36 @@ def test_gemv_16384_headline(chai, oracle):
     xh = x.double().cpu().numpy()
     ref = Ah @ xh
     assert np.all(np.abs(y[rows] - ref) <= 2 * n * U * (np.abs(Ah) @ np.abs(xh)))
+
+
+@pytest.mark.parametrize("M,N", [(5, 70001), (300, 500), (4100, 1030), (1, 4099)])
+@pytest.mark.parametrize("trans", [False, True])
+def test_gemv_variants_match_the_interpreter(chai, oracle, M, N, trans):
+    """Every GEMV tile variant against the interpreter's matvec (each product
+    rounded in f32, summed in double): the FP64 sums differ only in order, so
+    |y - y_interp| <= 2^-24 |y_interp| + K 2^-52 sum|a x|."""
+    A = unif(oracle, 7, M, N)
+    xv = unif(oracle, 8, 1, M if trans else N).ravel()
+    At, xt = torch.from_numpy(A).cuda(), torch.from_numpy(xv).cuda()
+    o = oracle.matvec_interp(A, xv, trans_a=trans).astype(np.float64)
+    A64 = A.astype(np.float64).T if trans else A.astype(np.float64)
+    absp = np.abs(A64) @ np.abs(xv.astype(np.float64))
+    K = A64.shape[1]
+    nv = len(chai.gemm_variants(chai.GEMM_GEMV))
+    assert nv == 9
+    for v in range(nv):
+        y = chai.gemm(At, xt.view(-1, 1), trans_a=trans, mode=chai.GEMM_GEMV, variant=v).cpu().numpy().ravel()
+        assert np.all(np.abs(y - o) <= U * np.abs(o) + K * 2.0 ** -52 * absp + 1e-30), v
+
+
+def test_gemv_tune_checks_each_variant(chai, oracle):
+    """The tuner's search over the GEMV family: every variant passes its own
+    FP64 check (they order the sums differently) and the winner is used."""
+    A = torch.from_numpy(unif(oracle, 9, 2000, 3000)).cuda()
+    x = torch.from_numpy(unif(oracle, 10, 1, 3000).ravel()).cuda()
+    y = torch.empty(2000, 1, device="cuda")
+    v, ms = chai.gemm_tune(A, x.view(-1, 1), y, mode=chai.GEMM_GEMV, trials=2)
+    assert 0 <= v < 9 and ms > 0
+    y1 = chai.gemv(A, x)
+    y2 = chai.gemm(A, x.view(-1, 1), mode=chai.GEMM_GEMV, variant=v).view(-1)
+    assert torch.equal(y1, y2)

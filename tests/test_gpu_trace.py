@@ -191,9 +191,18 @@ def test_unsupported_trace_is_rejected(chai):
 
 
 def test_compile_cache_returns_same_program(chai):
-    t, m = mc_trace(1 << 10)
-    p1 = t.compile(); p2 = t.compile()
-    assert p1.info() == p2.info()
+    from chai_amd.trace import cache_stats
+    t, m = mc_trace(1000 + 7)                 # a size no other test compiles
+    h0, m0, e0 = cache_stats()
+    p1 = t.compile()
+    h1, m1, e1 = cache_stats()
+    assert (h1, m1, e1) == (h0, m0 + 1, e0 + 1)
+    p2 = t.compile()
+    h2, m2, e2 = cache_stats()
+    assert (h2, m2, e2) == (h1 + 1, m1, e1) and p1.info() == p2.info()
+    t3, _ = mc_trace(1000 + 9)                # another shape is another program
+    t3.compile()
+    assert cache_stats()[1:] == (m2 + 1, e2 + 1)
 
 
 # ops whose result is fixed by IEEE arithmetic (or by an exact definition such
