@@ -22,6 +22,7 @@
 #include "gemm_internal.hpp"
 #include "gemm_split.hpp"
 #include "gemm_x3.hpp"
+#include "gemm_x6.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -178,6 +179,12 @@ struct X3Variant {
 // index = variant id
 static const X3Variant kX3Variants[] = {CHAI_X3_VARIANTS(CHAI_X3V)};
 static constexpr int kNumX3 = sizeof(kX3Variants) / sizeof(kX3Variants[0]);
+
+#define CHAI_X6V(BM, BN, BK, WGM, WGN, MINB)                                              \
+    {BM, BN, BK, WGM, WGN, X6Shape<BM, BN, BK, WGM, WGN>::LDS_BYTES, WGM * WGN * 64,        \
+     gemm_x6_kernel<BM, BN, BK, WGM, WGN, MINB>},
+static const X3Variant kX6Variants[] = {CHAI_X6_VARIANTS(CHAI_X6V)};
+static constexpr int kNumX6 = sizeof(kX6Variants) / sizeof(kX6Variants[0]);
 static int g_x3_default = 0;
 
 // =====================================================================
@@ -623,75 +630,54 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
     }
     if (mode == CHAI_GEMM_BF16X6) {
         // Split-BF16x6: A = a0 + a1 + a2, B = b0 + b1 + b2 (three bf16 pieces,
-        // all 24 bits of an f32).  The products of order 0 and 1 (a0 b0, a0 b1,
-        // a1 b0) are the BF16x3 tile's; the order-2 ones (a2 b0 + a0 b2 + a1 b1)
-        // are one plain bf16 GEMM over panels concatenated along k:
-        // [a2 | a0 | a1] x [b0 | b2 | b1].  The first pass writes T; the second
-        // accumulates the small terms and adds T before alpha / beta and the
-        // epilogue.  The terms dropped (order >= 3) are below 2^-24 |a b|.
+        // all 24 bits of an f32); one pass of the six-panel tile (gemm_x6.hpp)
+        // over the split3 pre-pass's panels.  The terms dropped (order >= 3)
+        // are below 2^-24 |a b|.
         if (d.prec_c != CHAI_F32) { set_error("gemm: bf16x6 writes f32 C"); return CHAI_ERR_ARG; }
-        const int vid = variant < 0 ? g_x3_default : variant;
-        if (vid >= kNumX3) { set_error("gemm: bf16x6 variant %d out of range", vid); return CHAI_ERR_ARG; }
-        const X3Variant& v = kX3Variants[vid];
+        const int vid = variant < 0 ? 0 : variant;
+        if (vid >= kNumX6) { set_error("gemm: bf16x6 variant %d out of range", vid); return CHAI_ERR_ARG; }
+        const X3Variant& v = kX6Variants[vid];
         const uint64_t Mp = round_up(M, v.BM), Np = round_up(N, v.BN), Kp = round_up(K, 64);
         const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
         const uint64_t sa3 = 3 * Mp * Kp, sb3 = 3 * Np * Kp;        // elements per instance
         const size_t panel_bytes = sizeof(uint16_t) * (sa3 * na + sb3 * nbb);
         const uint64_t fa = (Mp / 64) * (Kp / 64), fb = (Np / 64) * (Kp / 64);
-        const size_t t_bytes = sizeof(float) * M * N * nb;
-        const size_t bytes = panel_bytes + t_bytes + fa * na + fb * nbb;
+        const size_t bytes = panel_bytes + fa * na + fb * nbb;
         uint16_t* ws = nullptr;
         CHAI_HIP_TRY(hipMallocAsync((void**)&ws, bytes, st));
-        uint16_t* PA = ws;                       // per instance [a2 | a0 | a1]
-        uint16_t* PB = PA + sa3 * na;            // per instance [b0 | b2 | b1]
-        float* T = (float*)((char*)ws + panel_bytes);
-        uint8_t* nz_a = (uint8_t*)T + t_bytes;
+        uint16_t* PA = ws;                       // per instance [a0 | a1 | a2]
+        uint16_t* PB = PA + sa3 * na;            // per instance [b0 | b1 | b2]
+        uint8_t* nz_a = (uint8_t*)ws + panel_bytes;
         uint8_t* nz_b = nz_a + fa * na;
         prof_mark(0, st);
         split3_any(d.prec_a, A, d.lda, d.trans_a != 0, M, K, Mp, Kp, d.stride_a, sa3, na,
-                   PA + Mp * Kp, PA + 2 * Mp * Kp, PA, nz_a, st);
+                   PA, PA + Mp * Kp, PA + 2 * Mp * Kp, nz_a, st);
         split3_any(d.prec_b, B, d.ldb, d.trans_b == 0, N, K, Np, Kp, d.stride_b, sb3, nbb,
-                   PB, PB + 2 * Np * Kp, PB + Np * Kp, nz_b, st);
+                   PB, PB + Np * Kp, PB + 2 * Np * Kp, nz_b, st);
         prof_mark(1, st);
-        static bool attr6[kNumX3] = {};
+        static bool attr6[kNumX6] = {};
         if (!attr6[vid]) {
             CHAI_HIP_TRY(hipFuncSetAttribute((const void*)v.kernel,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)v.lds));
             attr6[vid] = true;
         }
-        const uint32_t tiles_m = (uint32_t)(Mp / v.BM), tiles_n = (uint32_t)(Np / v.BN);
-        dim3 grid(tiles_m * tiles_n, 1, (unsigned)nb);
-        GemmArgs g{};                            // pass 1: order 0 and 1 -> T
-        g.Ah = PA + Mp * Kp; g.Al = PA + 2 * Mp * Kp; g.Bh = PB; g.Bl = PB + 2 * Np * Kp;
+        GemmArgs g{};
+        g.Ah = PA; g.Bh = PB;
         g.a_kblk = Mp * 32; g.b_kblk = Np * 32;
         g.a_nz = nz_a; g.b_nz = nz_b;
         g.a_nz_bstride = d.same_a ? 0 : fa;
         g.b_nz_bstride = d.same_b ? 0 : fb;
-        g.Kp = Kp; g.Np = Np;
+        g.Kp = Kp; g.Np = Np; g.Mp = Mp;
         g.pa_bstride = d.same_a ? 0 : sa3;
         g.pb_bstride = d.same_b ? 0 : sb3;
-        g.C = T; g.c_bstride = M * N; g.ldc = N;
+        g.C = (float*)C; g.c_bstride = d.stride_c; g.ldc = d.ldc;
         g.M = M; g.N = N;
-        g.tiles_m = tiles_m; g.tiles_n = tiles_n;
+        g.tiles_m = (uint32_t)(Mp / v.BM); g.tiles_n = (uint32_t)(Np / v.BN);
+        g.general = d.general; g.alpha = d.alpha; g.beta = d.beta;
+        set_epilogue(g, epi);
+        dim3 grid(g.tiles_m * g.tiles_n, 1, (unsigned)nb);
         hipLaunchKernelGGL(v.kernel, grid, dim3(v.threads), v.lds, st, g);
         hipError_t e = hipGetLastError();
-        GemmArgs h{};                            // pass 2: order 2 over 3 Kp, + T -> C
-        h.Ah = PA; h.Bh = PB;
-        h.a_kblk = Mp * 32; h.b_kblk = Np * 32;
-        h.Kp = 3 * Kp; h.Np = Np;
-        h.pa_bstride = d.same_a ? 0 : sa3;
-        h.pb_bstride = d.same_b ? 0 : sb3;
-        h.C = (float*)C; h.c_bstride = d.stride_c; h.ldc = d.ldc;
-        h.M = M; h.N = N;
-        h.tiles_m = tiles_m; h.tiles_n = tiles_n;
-        h.general = d.general; h.alpha = d.alpha; h.beta = d.beta;
-        set_epilogue(h, epi);
-        h.hi_only = 1;
-        h.add = T; h.ld_add = N; h.add_bstride = M * N;
-        if (e == hipSuccess) {
-            hipLaunchKernelGGL(v.kernel, grid, dim3(v.threads), v.lds, st, h);
-            e = hipGetLastError();
-        }
         prof_mark(2, st);
         CHAI_HIP_TRY(hipFreeAsync(ws, st));
         if (e != hipSuccess) return hip_fail(e, "gemm_x6 launch");
@@ -782,7 +768,8 @@ namespace chai_hip {
 
 int gemm_num_variants(int mode)
 {
-    return (mode == CHAI_GEMM_BF16X3 || mode == CHAI_GEMM_BF16X6) ? kNumX3
+    return mode == CHAI_GEMM_BF16X3                                 ? kNumX3
+           : mode == CHAI_GEMM_BF16X6                                ? kNumX6
            : mode == CHAI_GEMM_F64                                   ? kNumD
            : mode == CHAI_GEMM_FMA                                   ? kNumF
            : mode == CHAI_GEMM_GEMV                                  ? gemv_num_variants()
@@ -802,7 +789,12 @@ int gemm_variant_params(int mode, int vid, int* p)
         p[0] = v.BM; p[1] = v.BN; p[2] = F_BK; p[3] = 4; p[4] = 1;   // 4 waves, each 16 / 4 thread rows
         return CHAI_OK;
     }
-    if ((mode != CHAI_GEMM_BF16X3 && mode != CHAI_GEMM_BF16X6) || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;
+    if (mode == CHAI_GEMM_BF16X6 && vid >= 0 && vid < kNumX6) {
+        const X3Variant& v = kX6Variants[vid];
+        p[0] = v.BM; p[1] = v.BN; p[2] = v.BK; p[3] = v.WGM; p[4] = v.WGN;
+        return CHAI_OK;
+    }
+    if (mode != CHAI_GEMM_BF16X3 || vid < 0 || vid >= kNumX3) return CHAI_ERR_ARG;
     const X3Variant& v = kX3Variants[vid];
     p[0] = v.BM; p[1] = v.BN; p[2] = v.BK; p[3] = v.WGM; p[4] = v.WGN;
     return CHAI_OK;

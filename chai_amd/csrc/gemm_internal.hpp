@@ -35,12 +35,6 @@ struct GemmArgs {
     // epi == 1: c = max(c * epi_scale + epi_shift, epi_lo), each op rounded in f32
     int epi;
     float epi_scale, epi_shift, epi_lo;
-    // split-BF16x6 second pass: hi_only runs the hi x hi product alone (a
-    // plain bf16 GEMM over the concatenated small-term panels), and add (f32,
-    // row stride ld_add) is the first pass's result, added before alpha / beta
-    int hi_only;
-    const float* add;
-    uint64_t ld_add, add_bstride;
 };
 
 // variant < 0 selects the current default (set by the tuner) for the mode.

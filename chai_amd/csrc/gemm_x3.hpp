@@ -177,11 +177,15 @@ __device__ __forceinline__ void x3_stage_mfma(const u32x4* lds, int cur, const i
 }
 
 // Store the wave's accumulators (32x32x16 output layout: lane -> column
-// lane % 32; register r -> row 8 (r / 4) + 4 (lane / 32) + r % 4).
-template <typename S>
-__device__ __forceinline__ void x3_store(const GemmArgs& a, float* C, uint64_t row_base, uint64_t col_base, int lane,
-                                         const floatx16 (&acc)[S::TI][S::TJ], const float* add)
+// lane % 32; register r -> row 8 (r / 4) + 4 (lane / 32) + r % 4).  The
+// general (alpha / beta) and epilogue forms are separate loops, so each
+// fully unrolled loop stays small enough to keep the accumulators in
+// registers.
+template <typename S, bool GEN, bool EPI>
+__device__ __forceinline__ void x3_store_as(const GemmArgs& a, float* C, uint64_t row_base, uint64_t col_base,
+                                            int lane, const floatx16 (&acc)[S::TI][S::TJ])
 {
+    const bool full = row_base + S::TI * 32 <= a.M && col_base + S::TJ * 32 <= a.N;
 #pragma unroll
     for (int i = 0; i < S::TI; i++)
 #pragma unroll
@@ -190,13 +194,28 @@ __device__ __forceinline__ void x3_store(const GemmArgs& a, float* C, uint64_t r
 #pragma unroll
             for (int r = 0; r < 16; r++) {
                 const uint64_t row = row_base + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row < a.M && col < a.N) {
+                if (full || (row < a.M && col < a.N)) {
                     float v = acc[i][j][r];
-                    if (add) v = v + gload(add + row * a.ld_add + col);
-                    store_c(C, row * a.ldc + col, v, a);
+                    float* p = C + row * a.ldc + col;
+                    if (GEN) v = (float)(a.alpha * (double)v + a.beta * (double)gload(p));
+                    if (EPI) v = fmaxf(v * a.epi_scale + a.epi_shift, a.epi_lo);   // no contraction (-ffp-contract=off)
+                    gstore(p, v);
                 }
             }
         }
+}
+
+template <typename S>
+__device__ __forceinline__ void x3_store(const GemmArgs& a, float* C, uint64_t row_base, uint64_t col_base, int lane,
+                                         const floatx16 (&acc)[S::TI][S::TJ])
+{
+    if (a.general) {
+        if (a.epi) x3_store_as<S, true, true>(a, C, row_base, col_base, lane, acc);
+        else x3_store_as<S, true, false>(a, C, row_base, col_base, lane, acc);
+    } else {
+        if (a.epi) x3_store_as<S, false, true>(a, C, row_base, col_base, lane, acc);
+        else x3_store_as<S, false, false>(a, C, row_base, col_base, lane, acc);
+    }
 }
 
 // The tile body.  SKIP bit 0: the tile's Al panel is zero, bit 1: its Bl
@@ -300,8 +319,7 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
     }
     wait_lds_dma();     // the refills of the last two steps have landed: LDS is free on exit
 
-    x3_store<S>(a, C, (uint64_t)tm * BM + wm, (uint64_t)tn * BN + wn, lane, acc,
-                a.add ? a.add + bz * a.add_bstride : nullptr);
+    x3_store<S>(a, C, (uint64_t)tm * BM + wm, (uint64_t)tn * BN + wn, lane, acc);
 }
 
 // One BM x BN output tile (tm, tn) of batch instance bz.  The block's LDS is
@@ -311,8 +329,8 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
 template <int BM, int BN, int BK, int WGM, int WGN>
 __device__ __forceinline__ void x3_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, uint64_t bz, u32x4* lds)
 {
-    int skip = a.hi_only ? 3 : 0;
-    if (a.a_nz && !a.hi_only) {
+    int skip = 0;
+    if (a.a_nz) {
         constexpr int THREADS = WGM * WGN * 64;
         const uint64_t kb64 = a.Kp / 64;
         const uint64_t na = (uint64_t)(BM / 64) * kb64, nb = (uint64_t)(BN / 64) * kb64;

@@ -125,7 +125,8 @@ __device__ __forceinline__ void x3f_store_stage(u32x4* st, const float (&fa)[X3F
 
 // One BM x BN output tile (tm, tn) from f32 A (a.Af, a.lda) and B (a.Bf,
 // a.ldb); a.Kp = K rounded up to 64.  LDS is free on entry and on exit.
-template <int BM, int BN, int WGM, int WGN>
+// PIPE: software-pipeline the MFMA operand fragments across the barrier.
+template <int BM, int BN, int WGM, int WGN, bool PIPE>
 __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, u32x4* lds)
 {
     using F = X3F32Shape<BM, BN, WGM, WGN>;
@@ -152,16 +153,42 @@ __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_
     int offA[S::TI][S::KB], offB[S::TJ][S::KB];
     x3_frag_offsets<S>(wm, wn, lane, offA, offB);
 
+    if constexpr (!PIPE) {
+        for (int kt = 0; kt < nk; kt++) {
+            const int cur = (kt & 1) * S::STAGE;
+            const bool more = kt + 1 < nk;
+            if (more) x3f_load<BM, BN, WGM, WGN>(a, r0, c0, (uint64_t)(kt + 1) * 32, avec, fa, fb);
+            x3_stage_mfma<S, 0>(lds, cur, offA, offB, acc);
+            // the other buffer was last read in step kt - 1, closed by its barrier
+            if (more) x3f_store_stage<BM, BN, WGM, WGN>(lds + ((kt + 1) & 1) * S::STAGE, fa, fb);
+            __syncthreads();
+        }
+        x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc);
+        return;
+    }
+    // The panel tile's fragment pipeline (gemm_x3.hpp): k block 1 of stage kt
+    // is read before the barrier and multiplied after it, beside the reads of
+    // stage kt + 1's k block 0.  It holds two k blocks of fragments besides
+    // the staged operands, so it is used where a block's waves have a SIMD's
+    // registers to themselves.
+    static_assert(S::KB == 2, "32-deep stages");
+    X3Frag<S> f[2];
+    x3_load_frag<S, 0>(lds, 0, 0, offA, offB, f[0]);
     for (int kt = 0; kt < nk; kt++) {
         const int cur = (kt & 1) * S::STAGE;
         const bool more = kt + 1 < nk;
         if (more) x3f_load<BM, BN, WGM, WGN>(a, r0, c0, (uint64_t)(kt + 1) * 32, avec, fa, fb);
-        x3_stage_mfma<S, 0>(lds, cur, offA, offB, acc);
-        // the other buffer was last read in step kt - 1, closed by its barrier
+        x3_load_frag<S, 0>(lds, cur, 1, offA, offB, f[1]);
+        x3_frag_mfma<S, 0>(f[0], acc);
+        __builtin_amdgcn_sched_barrier(0);
+        // the other buffer was last read before the previous step's barrier
         if (more) x3f_store_stage<BM, BN, WGM, WGN>(lds + ((kt + 1) & 1) * S::STAGE, fa, fb);
         __syncthreads();
+        __builtin_amdgcn_sched_barrier(0);
+        if (more) x3_load_frag<S, 0>(lds, S::STAGE - cur, 0, offA, offB, f[0]);
+        x3_frag_mfma<S, 0>(f[1], acc);
     }
-    x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc, nullptr);
+    x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc);
 }
 
 // The f32-fed family's tile shapes: X(BM, BN, waves_m, waves_n, min blocks per CU)

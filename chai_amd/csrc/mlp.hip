@@ -234,16 +234,15 @@ static int mlp_run(Mlp* N, const float* X, const float* Y, size_t B, float* out,
 }
 
 // ------------------------------------------------------------ GEMM features
-// The families the model covers, in one-hot order.  BF16X6 runs the BF16X3
-// tiles (twice) and shares their slot.
+// The families the model covers, in one-hot order.
 static int family_slot(int mode)
 {
     switch (mode) {
-        case CHAI_GEMM_BF16X3:
-        case CHAI_GEMM_BF16X6: return 0;
+        case CHAI_GEMM_BF16X3: return 0;
         case CHAI_GEMM_FMA: return 1;
         case CHAI_GEMM_F64: return 2;
         case CHAI_GEMM_GEMV: return 3;
+        case CHAI_GEMM_BF16X6: return 4;
         default: return -1;
     }
 }
@@ -256,7 +255,7 @@ static void gemm_features(const chai_gemm_desc& d, int mode, int variant, float*
 {
     const int fam = family_slot(mode);
     int p[5] = {0, 0, 0, 0, 0};
-    (void)gemm_variant_params(mode == CHAI_GEMM_BF16X6 ? CHAI_GEMM_BF16X3 : mode, variant, p);
+    (void)gemm_variant_params(mode, variant, p);
     const double M = (double)d.M, N = (double)d.N, K = (double)d.K;
     const double nb = (double)(d.batch ? d.batch : 1);
     for (int i = 0; i < CHAI_GEMM_FEATURES; i++) f[i] = 0.f;
@@ -285,11 +284,12 @@ static void gemm_features(const chai_gemm_desc& d, int mode, int variant, float*
         }
         waves = 4; lds = 64;
         aspect = std::log2(split) / 2;
-        f[17] = (float)(std::log2(lif) / 3);
+        f[18] = (float)(std::log2(lif) / 3);
     } else {
         BM = p[0] > 0 ? p[0] : 128; BN = p[1] > 0 ? p[1] : 128; BK = p[2] > 0 ? p[2] : 16;
         waves = std::max(1, p[3] * p[4]);
         if (fam == 0) { lds = 2.0 * (2 * BM + 2 * BN) * BK * 2; kq = 64; }
+        else if (fam == 4) { lds = 2.0 * (3 * BM + 3 * BN) * BK * 2; kq = 64; }
         else if (fam == 1) { lds = 2.0 * 16 * (BM + BN + 8) * 4; kq = 16; waves = 4; }
         else { lds = 2.0 * 8 * (BM + BN + 32) * 8; kq = 32; }
         aspect = std::log2((BM / std::max(1, p[3])) / (BN / std::max(1, p[4]))) / 2;
@@ -301,17 +301,17 @@ static void gemm_features(const chai_gemm_desc& d, int mode, int variant, float*
     per_cu = std::max(1.0, std::min(per_cu, std::floor(32.0 / waves)));
     const double rounds = tiles / (256.0 * per_cu);
     const double Kp = std::ceil(K / kq) * kq;
-    f[12] = (float)(std::log2(BM) / 10);
-    f[13] = (float)(std::log2(BN) / 8);
-    f[14] = (float)(std::log2(BK) / 14);
-    f[15] = (float)(waves / 8);
-    f[16] = (float)aspect;
-    f[18] = (float)(std::log2(tiles) / 16);
-    f[19] = (float)(std::log2(rounds) / 8);
-    f[20] = (float)(rounds / std::ceil(rounds));       // last-round occupancy
-    f[21] = (float)used;                               // useful fraction of the tile grid
-    f[22] = (float)(K / Kp);
-    f[23] = (float)(std::log2(nb) / 8);
+    f[13] = (float)(std::log2(BM) / 10);
+    f[14] = (float)(std::log2(BN) / 8);
+    f[15] = (float)(std::log2(BK) / 14);
+    f[16] = (float)(waves / 8);
+    f[17] = (float)aspect;
+    f[19] = (float)(std::log2(tiles) / 16);
+    f[20] = (float)(std::log2(rounds) / 8);
+    f[21] = (float)(rounds / std::ceil(rounds));       // last-round occupancy
+    f[22] = (float)used;                               // useful fraction of the tile grid
+    f[23] = (float)(K / Kp);
+    f[24] = (float)(std::log2(nb) / 8);
 }
 
 }  // namespace chai_hip

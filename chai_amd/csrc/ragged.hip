@@ -146,7 +146,7 @@ __global__ __launch_bounds__(WGM * WGN * 64, MINB) void ragged_x3f_kernel(
             g.C = I.C; g.ldc = I.ldc;
             g.M = I.M; g.N = I.N;
             g.epi = epi.on; g.epi_scale = epi.scale; g.epi_shift = epi.shift; g.epi_lo = epi.lo;
-            x3f_tile<BM, BN, WGM, WGN>(g, w.y >> 16, w.y & 0xffffu, lds);
+            x3f_tile<BM, BN, WGM, WGN, MINB == 1>(g, w.y >> 16, w.y & 0xffffu, lds);
         }
     }
 }

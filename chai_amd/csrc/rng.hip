@@ -23,8 +23,10 @@ struct RngGeom {
     int has_limits, lo_nonzero;
 };
 
-// vector v of the stream -> its elements in `out`
-template <typename T, int VARIANT, int VL, int NORMAL>
+// vector v of the stream -> its elements in `out`.  LIM (f32 uniforms): 0 the
+// unit interval, 1 scaled by d_f, 2 scaled and shifted by lo_f (a template
+// argument, so the launch picks the arithmetic instead of every element)
+template <typename T, int VARIANT, int VL, int NORMAL, int LIM = 0>
 __device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out, uint64_t v)
 {
     // rows without padding (W a multiple of VL) are contiguous in the vector
@@ -48,10 +50,8 @@ __device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out
     } else {
         for (int i = 0; i < 4; i++) {
             float u = unit_f32(w[i]);
-            if (g.has_limits) {
-                u = u * g.d_f;
-                if (g.lo_nonzero) u = u + g.lo_f;
-            }
+            if (LIM >= 1) u = u * g.d_f;
+            if (LIM == 2) u = u + g.lo_f;
             val[i] = u;
         }
     }
@@ -67,12 +67,12 @@ __device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out
 
 // Grid-stride over the vectors: a bounded grid keeps every wave busy for many
 // vectors (one 1 KiB store per wave per step) instead of one vector per thread.
-template <typename T, int VARIANT, int VL, int NORMAL>
+template <typename T, int VARIANT, int VL, int NORMAL, int LIM>
 __global__ __launch_bounds__(256) void rng_fill_32(RngGeom g, T* __restrict__ out)
 {
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
     for (uint64_t v = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; v < g.nvec; v += stride)
-        rng_vec_32<T, VARIANT, VL, NORMAL>(g, out, v);
+        rng_vec_32<T, VARIANT, VL, NORMAL, LIM>(g, out, v);
 }
 
 template <int VARIANT, int VL, int NORMAL>
@@ -120,13 +120,24 @@ static unsigned rng_blocks(uint64_t nvec)
     return (unsigned)(b < 4096 ? b : 4096);
 }
 
+template <typename T, int VARIANT, int NORMAL, int LIM>
+static void launch32_lim(int vl, const RngGeom& g, void* out, hipStream_t st)
+{
+    const unsigned blocks = rng_blocks(g.nvec);
+    if (vl == 4) rng_fill_32<T, VARIANT, 4, NORMAL, LIM><<<blocks, 256, 0, st>>>(g, (T*)out);
+    else if (vl == 2) rng_fill_32<T, VARIANT, 2, NORMAL, LIM><<<blocks, 256, 0, st>>>(g, (T*)out);
+    else rng_fill_32<T, VARIANT, 1, NORMAL, LIM><<<blocks, 256, 0, st>>>(g, (T*)out);
+}
+
 template <typename T, int VARIANT, int NORMAL>
 static void launch32(int vl, const RngGeom& g, void* out, hipStream_t st)
 {
-    const unsigned blocks = rng_blocks(g.nvec);
-    if (vl == 4) rng_fill_32<T, VARIANT, 4, NORMAL><<<blocks, 256, 0, st>>>(g, (T*)out);
-    else if (vl == 2) rng_fill_32<T, VARIANT, 2, NORMAL><<<blocks, 256, 0, st>>>(g, (T*)out);
-    else rng_fill_32<T, VARIANT, 1, NORMAL><<<blocks, 256, 0, st>>>(g, (T*)out);
+    if (std::is_floating_point<T>::value && !NORMAL && g.has_limits) {
+        if (g.lo_nonzero) launch32_lim<T, VARIANT, NORMAL, 2>(vl, g, out, st);
+        else launch32_lim<T, VARIANT, NORMAL, 1>(vl, g, out, st);
+    } else {
+        launch32_lim<T, VARIANT, NORMAL, 0>(vl, g, out, st);
+    }
 }
 
 template <int VARIANT>

@@ -991,7 +991,10 @@ static int version_trace(const chai_trace& in, Versioned& V, const chai_trace*& 
 }
 
 // Statement-to-kernel grouping and VM code generation.
-static int compile_trace(const chai_trace& t, TraceProgram& P)
+// stmt_cap (empty, or one entry per statement): a fused kernel whose first
+// statement has a nonzero cap takes statements only while their nodes total
+// at most the cap (see chai_hip_trace_compile)
+static int compile_trace(const chai_trace& t, TraceProgram& P, const std::vector<size_t>& stmt_cap)
 {
     P.vars.assign(t.vars, t.vars + t.num_vars);
     for (const auto& v : P.vars) {
@@ -1012,6 +1015,7 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
     std::vector<int> group_of_stmt(t.num_stmts, -1);
     std::vector<int> red_group_of_var(t.num_vars, -1);
     std::vector<int> group_of_var(t.num_vars, -1);
+    size_t group_nodes = 0, group_cap = 0;
     for (uint32_t i = 0; i < t.num_stmts; i++) {
         const Stmt& s = P.stmts[i];
         bool reads_red = false;
@@ -1028,13 +1032,17 @@ static int compile_trace(const chai_trace& t, TraceProgram& P)
         const bool join = s.kind != 2 && s.kind != 3 && s.kind != 4 && !P.groups.empty() && P.groups.back().kind == 0 &&
                           P.groups.back().space == s.space && !reads_red &&
                           (s.kind == 0 || std::count_if(P.groups.back().stmts.begin(), P.groups.back().stmts.end(),
-                                                        [&](int q) { return P.stmts[q].kind == 1; }) < VM_MAX_ACC);
+                                                        [&](int q) { return P.stmts[q].kind == 1; }) < VM_MAX_ACC) &&
+                          (group_cap == 0 || group_nodes + s.nodes.size() <= group_cap);
         if (!join) {
             Group g;
             g.kind = s.kind == 2 || s.kind == 3 || s.kind == 4 ? s.kind : 0;
             g.space = s.space;
             P.groups.push_back(g);
+            group_nodes = 0;
+            group_cap = stmt_cap.empty() ? 0 : stmt_cap[i];
         }
+        group_nodes += s.nodes.size();
         P.groups.back().stmts.push_back((int)i);
         group_of_stmt[i] = (int)P.groups.size() - 1;
         group_of_var[s.lhs] = (int)P.groups.size() - 1;
@@ -1753,6 +1761,38 @@ static int launch_program(const TraceProgram& P, void* const* user, hipStream_t 
     return rc;
 }
 
+// A fused kernel longer than the JIT limit (after loop rolling) would run on
+// the trace VM, an order of magnitude slower than compiled code.  When such a
+// kernel holds several statements, its statements are regrouped into shorter
+// kernels (cap on their nodes, halved until each fits or holds one
+// statement); the variables that then cross kernels get memory as any other
+// crossing temporary.  Kernels of one statement stay whole.
+static int fit_jit_limit(const chai_trace& t, TraceProgram& P)
+{
+    std::vector<size_t> cap;
+    for (int attempt = 0; attempt < 12; attempt++) {
+        bool over = false;
+        std::vector<size_t> next = cap.empty() ? std::vector<size_t>(t.num_stmts, 0) : cap;
+        for (const Group& G : P.groups) {
+            if (G.kind != 0 || G.stmts.size() < 2 || jit_rolled_size(G.prog) <= jit_max_instructions()) continue;
+            size_t nodes = 0;
+            for (int si : G.stmts) nodes += P.stmts[si].nodes.size();
+            const size_t c = std::max<size_t>(1, (cap.empty() || !cap[G.stmts[0]] ? nodes : cap[G.stmts[0]]) / 2);
+            for (int si : G.stmts) next[si] = c;
+            over = true;
+        }
+        if (!over) return CHAI_OK;
+        cap = next;
+        P.stmts.clear();
+        P.groups.clear();
+        P.needs_mem.clear();
+        P.ninstr = 0;
+        const int rc = compile_trace(t, P, cap);
+        if (rc) return rc;
+    }
+    return CHAI_OK;
+}
+
 // ------------------------------------------------ compiled-trace cache (JIT memo)
 static uint64_t fnv_hash(const void* p, size_t n, uint64_t h)
 {
@@ -1809,7 +1849,8 @@ extern "C" int chai_hip_trace_compile(const chai_trace* trace, void** handle)
     const chai_trace *ts = nullptr, *tv = nullptr;
     int rc = split_trace(prog->own.t, prog->spl, ts);
     if (rc == CHAI_OK) rc = version_trace(*ts, prog->ver, tv);
-    if (rc == CHAI_OK) rc = compile_trace(*tv, *prog);
+    if (rc == CHAI_OK) rc = compile_trace(*tv, *prog, {});
+    if (rc == CHAI_OK && jit_enabled()) rc = fit_jit_limit(*tv, *prog);
     if (rc == CHAI_OK) rc = materialize(*prog);
     if (rc) return rc;
     {
@@ -1885,7 +1926,8 @@ extern "C" int chai_hip_trace_emit_source(const chai_trace* trace, int kernel, c
     const chai_trace *ts = nullptr, *tv = nullptr;
     int rc = split_trace(*trace, spl, ts);
     if (rc == CHAI_OK) rc = version_trace(*ts, ver, tv);
-    if (rc == CHAI_OK) rc = compile_trace(*tv, P);
+    if (rc == CHAI_OK) rc = compile_trace(*tv, P, {});
+    if (rc == CHAI_OK) rc = fit_jit_limit(*tv, P);    // the kernels the JIT engine compiles
     if (rc) return rc;
     if (kernel < 0 || kernel >= (int)P.groups.size() || P.groups[kernel].kind != 0) {
         set_error("trace_emit_source: kernel %d is not a fused VM kernel", kernel);
@@ -1933,6 +1975,6 @@ extern "C" int chai_hip_trace_check(const chai_trace* trace)
     const chai_trace *ts = nullptr, *tv = nullptr;
     int rc = split_trace(*trace, spl, ts);
     if (rc == CHAI_OK) rc = version_trace(*ts, ver, tv);
-    if (rc == CHAI_OK) rc = compile_trace(*tv, P);
+    if (rc == CHAI_OK) rc = compile_trace(*tv, P, {});
     return rc;
 }

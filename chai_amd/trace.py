@@ -234,9 +234,17 @@ class Trace:
         self.stmts.append((first.index, nodes))
 
     def _flatten(self, e, out):
-        for k in e.kids:
-            self._flatten(k, out)
-        out.append((e.code, dict(e.node)))
+        """postfix order (children, then the node), without recursion so that
+        deep expressions (long traced loops) flatten too"""
+        stack = [(e, False)]
+        while stack:
+            x, done = stack.pop()
+            if done:
+                out.append((x.code, dict(x.node)))
+                continue
+            stack.append((x, True))
+            for k in reversed(x.kids):
+                stack.append((k, False))
 
     # data creation leaves (api/RngFun.cpp, IndexFun.cpp)
     @staticmethod

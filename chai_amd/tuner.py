@@ -9,9 +9,9 @@ kernels) that predicts every variant's core-kernel time from the shape, the
 family and the variant's tile parameters; a new shape then gets its variant
 from one inference (chai_hip_gemm_predict_variant) instead of a search, and
 the search remains the fallback and measures the model's regret.  One model
-covers four families: the split-BF16x3 MFMA tiles (8 variants; BF16X6 runs the
-same tiles), the FP32-FMA tiles (4), the FP64 MFMA tiles (3) and the GEMV
-kernels (9: waves per row or row chunks x loads in flight).
+covers five families: the split-BF16x3 MFMA tiles (8 variants), the
+split-BF16x6 tiles (5), the FP32-FMA tiles (4), the FP64 MFMA tiles (3) and
+the GEMV kernels (9: waves per row or row chunks x loads in flight).
 """
 import ctypes
 import os
@@ -19,13 +19,13 @@ import time
 
 import numpy as np
 
-from . import (F32, F64, GEMM_BF16X3, GEMM_F64, GEMM_FMA, GEMM_GEMV, I32, GemmDesc, _stream, check,  # noqa: F401
-               gemm_variants, init, lib)
+from . import (F32, F64, GEMM_BF16X3, GEMM_BF16X6, GEMM_F64, GEMM_FMA, GEMM_GEMV, I32, GemmDesc,  # noqa: F401
+               _stream, check, gemm_variants, init, lib)
 
 # the families the model covers (the one-hot order of the features)
-FAMILIES = (GEMM_BF16X3, GEMM_FMA, GEMM_F64, GEMM_GEMV)
-FAMILY_NAMES = {GEMM_BF16X3: "bf16x3", GEMM_FMA: "fma", GEMM_F64: "f64", GEMM_GEMV: "gemv"}
-NFEAT = 24          # CHAI_GEMM_FEATURES
+FAMILIES = (GEMM_BF16X3, GEMM_FMA, GEMM_F64, GEMM_GEMV, GEMM_BF16X6)
+FAMILY_NAMES = {GEMM_BF16X3: "bf16x3", GEMM_FMA: "fma", GEMM_F64: "f64", GEMM_GEMV: "gemv", GEMM_BF16X6: "bf16x6"}
+NFEAT = 25          # CHAI_GEMM_FEATURES
 _vp, _sz = ctypes.c_void_p, ctypes.c_size_t
 
 
@@ -135,8 +135,8 @@ def random_shapes(n, seed, lo=64, hi=16384, step=16):
 
 def desc_of(shape, variant=-1, family=GEMM_BF16X3):
     """The GEMM descriptor of a random shape in a family: f32 (or i32 A) x f32
-    for the BF16X3 and FMA tiles, f64 x f64 for the FP64 tiles, and for GEMV
-    op(A) (M x K) times a K-vector."""
+    for the BF16X3, BF16X6 and FMA tiles, f64 x f64 for the FP64 tiles, and for
+    GEMV op(A) (M x K) times a K-vector."""
     M, N, K, ta, tb, pa = shape
     d = GemmDesc()
     d.prec_a, d.prec_b, d.prec_c = pa, F32, F32

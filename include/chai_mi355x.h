@@ -294,14 +294,14 @@ int chai_hip_gemm_set_variant(const chai_gemm_desc* desc, int variant);
 
 /* ------------------------------------------------- learned tile-cost model */
 /* A 3 x 64 ReLU MLP (FP32) that predicts the core-kernel time of a tile
- * variant of the BF16X3 (and BF16X6), FMA, F64 and GEMV families from
+ * variant of the BF16X3, BF16X6, FMA, F64 and GEMV families from
  * CHAI_GEMM_FEATURES features, trained on the tuner's own
  * timings with the library's HIP forward / backward / update kernels.  It has
  * no counterpart in the reference, whose only parameter choice is the timed
  * search (autotune/StandardEM.cpp:69-121); that search stays the fallback and
  * measures the model's regret.  X is [B][inputs] row-major f32, Y / out are
  * [B] f32, all device pointers.  The loss is mean_b (out_b - Y_b)^2. */
-enum { CHAI_GEMM_FEATURES = 24 };
+enum { CHAI_GEMM_FEATURES = 25 };
 enum { CHAI_MLP_SGD = 0, CHAI_MLP_ADAM = 1 };
 int chai_hip_mlp_create(int inputs, uint64_t seed, void** mlp);
 int chai_hip_mlp_num_params(void* mlp, size_t* count);

@@ -356,15 +356,20 @@ def test_gather_broadcast_reduction_and_second_kernel(chai, oracle, engine):
 @pytest.mark.parametrize("NT,N", [(12, 12), (1024, 64)])
 def test_kirchhoff_reassignment_bit_exact(chai, oracle, engine, NT, N):
     """sample/kirch.cpp: modl assigned N + 1 times (versions in registers),
-    N gather2_floor reads, one kernel; bit-exact against the oracle and the
-    numpy restatement"""
+    N gather2_floor reads, one kernel on the VM; under the JIT, a kernel
+    longer than the JIT limit is regrouped into shorter compiled ones.
+    Bit-exact against the oracle and the numpy restatement either way"""
     from tests.trace_cases import kirchhoff_expected, kirchhoff_trace
     t, data, modl, consts = kirchhoff_trace(NT, N)
     d = np.array([[1.0 / (1 + i * j) for j in range(NT)] for i in range(N)], dtype=np.float32)
     r, prog = run_gpu(t, {data.index: d})
     assert np.array_equal(r[modl.index], oracle.trace_eval(t, {data.index: d})[modl.index])
     assert np.array_equal(r[modl.index][0], kirchhoff_expected(d, NT, N, consts))
-    assert prog.info()[0] == 1
+    nk = prog.info()[0]
+    if engine == "vm" or N == 12:
+        assert nk == 1
+    else:
+        assert nk > 1 and all(prog.source(k)[1] for k in range(nk))
 
 
 def test_reassigned_temp_gathered_across_kernels(chai, oracle, engine):

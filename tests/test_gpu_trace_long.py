@@ -1,8 +1,9 @@
 """Long fused kernels: repeated statements (a traced loop) are emitted as a loop and
 compiled; a kernel still longer than the JIT limit after that (CHAI_TRACE_JIT_MAX, 1024 VM
-instructions by default) runs on the trace VM instead of being compiled by hipRTC, whose
-compile time grows faster than linearly with the kernel's length.  All give the
-step-by-step f32 result bit for bit."""
+instructions by default; hipRTC's compile time grows faster than linearly with the
+kernel's length) is regrouped into shorter compiled kernels, the intermediate arrays
+stored between them; a single statement longer than the limit runs on the trace VM.
+All give the step-by-step f32 result bit for bit."""
 import numpy as np
 import pytest
 import torch
@@ -27,21 +28,55 @@ def loop_trace(reps, n, varying=False):
 
 
 @pytest.mark.parametrize("reps,varying", [(8, False), (400, False), (400, True)])
-def test_long_kernels_roll_or_run_on_the_vm(chai, reps, varying):
-    """400 repetitions of one statement roll into a loop and are compiled; 400
-    different statements exceed the JIT limit and run on the VM"""
+def test_long_kernels_roll_or_split(chai, reps, varying):
+    """400 repetitions of one statement roll into a loop and are compiled as one
+    kernel; 400 different statements exceed the JIT limit as one kernel and are
+    regrouped into several compiled kernels"""
     n = 4096
     tr.set_jit(True)
     t, A, B, C, consts = loop_trace(reps, n, varying)
     prog = t.compile()
+    nk = prog.info()[0]
+    print("reps %d varying %s: %d kernels" % (reps, varying, nk), flush=True)
+    if varying:
+        assert nk > 1
+    else:
+        assert nk == 1
+    for k in range(nk):
+        src, jitted = prog.source(k)
+        assert jitted, src[:300]
+    if reps == 400 and not varying:
+        assert "for (int it = 0; it < 400;" in prog.source(0)[0]
+    rng = np.random.default_rng(reps)
+    a = rng.standard_normal(n).astype(np.float32)
+    b = rng.standard_normal(n).astype(np.float32)
+    c = torch.empty(n, device="cuda")
+    prog.run({A: torch.from_numpy(a).cuda(), B: torch.from_numpy(b).cuda(), C: c})
+    torch.cuda.synchronize()
+    ref = a + np.float32(0.0)
+    for k in consts:
+        ref = (ref * k).astype(np.float32) + b
+    assert np.array_equal(c.cpu().numpy(), ref)
+
+
+def test_one_statement_past_the_limit_runs_on_the_vm(chai):
+    """A single statement whose kernel exceeds the JIT limit cannot be split:
+    C = ((A * c_0 + B) * c_1 + B) ... as one expression, 600 distinct c_i"""
+    n = 4096
+    tr.set_jit(True)
+    t = tr.Trace()
+    A, B = t.input(F32, n), t.input(F32, n)
+    C = t.output(F32, n)
+    consts = [np.float32(0.999 - i * 1e-4) for i in range(600)]
+    e = A + tr.f32(0.0)
+    for c in consts:
+        e = e * tr.f32(float(c)) + B
+    t.assign(C, e)
+    prog = t.compile()
     assert prog.info()[0] == 1
     src, jitted = prog.source(0)
-    assert jitted == (not varying)
-    if varying:
-        assert "exceed the JIT limit" in src
-    elif reps == 400:
-        assert "for (int it = 0; it < 400;" in src
-    rng = np.random.default_rng(reps)
+    assert not jitted and "exceed the JIT limit" in src
+    rng = np.random.default_rng(3)
     a = rng.standard_normal(n).astype(np.float32)
     b = rng.standard_normal(n).astype(np.float32)
     c = torch.empty(n, device="cuda")
