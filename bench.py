@@ -229,7 +229,8 @@ def profile_run_ragged(args):
 
 def profile_run(args):
     """Short program for rocprofv3: calibration kernels of known traffic, then
-    the tile search and three bf16x3 GEMMs (8192^3) with the variant it picks."""
+    the tile search and three bf16x3 GEMMs (8192^3) with the variant it picks,
+    then the same for bf16x6."""
     import torch
     import chai_amd as chai
     chai.init(0)
@@ -243,6 +244,9 @@ def profile_run(args):
     chai.gemm_tune(A, B, C, trials=1)     # the variant the benchmark runs
     for _ in range(3):
         chai.gemm(A, B, C=C, mode=chai.GEMM_BF16X3)
+    chai.gemm_tune(A, B, C, mode=chai.GEMM_BF16X6, trials=1)
+    for _ in range(3):
+        chai.gemm(A, B, C=C, mode=chai.GEMM_BF16X6)
     torch.cuda.synchronize()
     print("profile run done", flush=True)
 
@@ -488,6 +492,14 @@ def run_sgemm(args):
                       "tile_variant": {"exhaustive": v6, "params": chai.gemm_variants(chai.GEMM_BF16X6)[v6]},
                       "note": "three bf16 pieces per operand, six MFMA products in one pass"}
     line["auto_family"] = "bf16x3" if n >= 16384 else "bf16x6"
+    # the target: a split-BF16 form at >= 2x the FP32-FMA kernel with <= 2x its
+    # FP64-referenced error, both in max and rms over the sampled rows
+    line["fp32_level_target"] = {
+        name: {"speedup_vs_fp32": round(sp, 3), "err_ratio_max": round(rmx, 3), "err_ratio_rms": round(rrms, 3),
+               "met": bool(sp >= 2.0 and rmx <= 2.0 and rrms <= 2.0)}
+        for name, sp, rmx, rrms in (("bf16x3", tflops_x3 / tflops_f32, err["ratio_max"], err["ratio_rms"]),
+                                    ("bf16x6", tflops_x6 / tflops_f32, err["bf16x6_ratio_max"],
+                                     err["bf16x6_ratio_rms"]))}
     line["split_prepass_ms"] = round(prep_ms, 4)
     line["errors_vs_fp64"] = err
     line["autotune_cold_start_s"] = cold
@@ -715,6 +727,8 @@ def run_mc(args, leg=False):
     N = 16384
     A, x = uniform((N, N), 42), uniform((N,), 42 + N * N)
     y = torch.empty(N, device="cuda")
+    # the GEMV family's tile search for this shape (chai.gemv then uses its choice)
+    gv, _ = chai.gemm_tune(A, x.view(-1, 1), y.view(-1, 1), mode=chai.GEMM_GEMV, trials=3)
     ms_gemv = timed(lambda: chai.gemv(A, x, y=y), args.steps, args.warmup)
     gemv_bytes = 4.0 * (N * N + 2 * N)
     gbs = gemv_bytes / (ms_gemv * 1e-3) / 1e9
@@ -737,6 +751,7 @@ def run_mc(args, leg=False):
     line["torch_fill_2^28_GBs"] = round(4.0 * 2 ** 28 / (ms_fill * 1e-3) / 1e9, 1)
     del big
     line["gemv_ms"] = round(ms_gemv, 4)
+    line["gemv_variant"] = {"tuned": gv, "params": chai.gemm_variants(chai.GEMM_GEMV)[gv][:2]}
     line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "gemv_rows"}
     if not args.no_cpu:

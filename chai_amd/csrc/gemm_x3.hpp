@@ -77,15 +77,18 @@ __device__ __forceinline__ void x3_issue_stage(const uint16_t* pAh, const uint16
     for (int s = 0; s < S::IA; s++) {
         const int c0 = (wave * S::IA + s) * 64;
         const uint64_t o = srcA[s] + (S::CPR == 8 && ((hiA >> s) & 1u) ? a_kblk : 0);
-        lds_dma16(pAh + o, st + c0);
-        if (!(SKIP & 1)) lds_dma16(pAl + o, st + S::PA + c0);
+        // panel base (wave-uniform) + the lane's 32-bit byte offset
+        const uint32_t ob = (uint32_t)(o * 2);
+        lds_dma16(reinterpret_cast<const char*>(pAh) + ob, st + c0);
+        if (!(SKIP & 1)) lds_dma16(reinterpret_cast<const char*>(pAl) + ob, st + S::PA + c0);
     }
 #pragma unroll
     for (int s = 0; s < S::IB; s++) {
         const int c0 = (wave * S::IB + s) * 64;
         const uint64_t o = srcB[s] + (S::CPR == 8 && ((hiB >> s) & 1u) ? b_kblk : 0);
-        lds_dma16(pBh + o, st + 2 * S::PA + c0);
-        if (!(SKIP & 2)) lds_dma16(pBl + o, st + 2 * S::PA + S::PB + c0);
+        const uint32_t ob = (uint32_t)(o * 2);
+        lds_dma16(reinterpret_cast<const char*>(pBh) + ob, st + 2 * S::PA + c0);
+        if (!(SKIP & 2)) lds_dma16(reinterpret_cast<const char*>(pBl) + ob, st + 2 * S::PA + S::PB + c0);
     }
 }
 
@@ -228,7 +231,9 @@ __device__ __forceinline__ void x3_tile_body(const GemmArgs& a, uint32_t tm, uin
     const uint64_t Kp = a.Kp;
     float* C = a.C + bz * a.c_bstride;
 
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // the wave index in a scalar register: the LDS destinations of the
+    // stage copies (M0) and the panel offsets derived from it stay scalar
+    const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wm = (wave / WGN) * S::WM;
     const int wn = (wave % WGN) * S::WN;
 

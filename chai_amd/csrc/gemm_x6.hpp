@@ -128,7 +128,8 @@ __device__ __forceinline__ void x6_tile_body(const GemmArgs& a, uint32_t tm, uin
     using S = X6Shape<BM, BN, BK, WGM, WGN>;
     using S3 = X3Shape<BM, BN, BK, WGM, WGN>;     // same fragment geometry and store layout
     float* C = a.C + bz * a.c_bstride;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // wave index in a scalar register: the copies' LDS destinations stay scalar
+    const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wm = (wave / WGN) * S::WM;
     const int wn = (wave % WGN) * S::WN;
 

@@ -23,24 +23,14 @@ struct RngGeom {
     int has_limits, lo_nonzero;
 };
 
-// vector v of the stream -> its elements in `out`.  LIM (f32 uniforms): 0 the
-// unit interval, 1 scaled by d_f, 2 scaled and shifted by lo_f (a template
+// the elements of vector v of the stream.  LIM (f32 uniforms): 0 the unit
+// interval, 1 scaled by d_f, 2 scaled and shifted by lo_f (a template
 // argument, so the launch picks the arithmetic instead of every element)
-template <typename T, int VARIANT, int VL, int NORMAL, int LIM = 0>
-__device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out, uint64_t v)
+template <typename T, int VARIANT, int VL, int NORMAL, int LIM>
+__device__ __forceinline__ void rng_vals_32(const RngGeom& g, uint64_t v, T (&val)[4])
 {
-    // rows without padding (W a multiple of VL) are contiguous in the vector
-    // index: element offset v * VL, no division
-    uint64_t row = 0, x0 = v * VL, n = VL;
-    if (g.W != g.vpr * VL) {
-        row = v / g.vpr;                       // slot * H + y
-        x0 = (v - row * g.vpr) * VL;
-        if (x0 >= g.W) return;                 // a vector entirely in the padding
-        n = (g.W - x0) < (uint64_t)VL ? (g.W - x0) : (uint64_t)VL;
-    }
     uint32_t w[4];
     rng_words32<VARIANT, VL>((uint32_t)v, (uint32_t)g.seed, w);
-    T val[4];
     if constexpr (std::is_integral<T>::value) {
         for (int i = 0; i < 4; i++) val[i] = (T)w[i];
     } else if constexpr (NORMAL) {
@@ -55,6 +45,23 @@ __device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out
             val[i] = u;
         }
     }
+}
+
+// vector v of the stream -> its elements in `out`
+template <typename T, int VARIANT, int VL, int NORMAL, int LIM = 0>
+__device__ __forceinline__ void rng_vec_32(const RngGeom& g, T* __restrict__ out, uint64_t v)
+{
+    // rows without padding (W a multiple of VL) are contiguous in the vector
+    // index: element offset v * VL, no division
+    uint64_t row = 0, x0 = v * VL, n = VL;
+    if (g.W != g.vpr * VL) {
+        row = v / g.vpr;                       // slot * H + y
+        x0 = (v - row * g.vpr) * VL;
+        if (x0 >= g.W) return;                 // a vector entirely in the padding
+        n = (g.W - x0) < (uint64_t)VL ? (g.W - x0) : (uint64_t)VL;
+    }
+    T val[4];
+    rng_vals_32<T, VARIANT, VL, NORMAL, LIM>(g, v, val);
     T* dst = out + row * g.W + x0;
     if (VL == 4 && n == 4 && (g.W & 3) == 0 && ((uintptr_t)out & 15) == 0) {
         typedef T T4 __attribute__((ext_vector_type(4)));

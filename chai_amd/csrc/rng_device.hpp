@@ -23,21 +23,6 @@
 namespace chai_hip {
 
 // ------------------------------------------------------------ block functions
-// a ^ b ^ c as one instruction: gfx950's three-input bitwise op with the
-// truth table 0x96 (= 0xF0 ^ 0xCC ^ 0xAA, the XOR of its three inputs).  The
-// compiler emits two v_xor_b32; the RNG kernels are bound by VALU issue, so
-// one instruction is cheaper.  Host builds of this file use the expression.
-__device__ __forceinline__ uint32_t xor3_u32(uint32_t a, uint32_t b, uint32_t c)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-    uint32_t r;
-    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(a), "v"(b), "v"(c));
-    return r;
-#else
-    return a ^ b ^ c;
-#endif
-}
-
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
@@ -47,10 +32,7 @@ __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32
         const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
         const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
         const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-        // round 0 of Chai's stream multiplies its uniform counter words: the
-        // compiler evaluates those products on the scalar unit
-        const uint32_t n0 = r ? xor3_u32(hi1, c[1], k0) : hi1 ^ c[1] ^ k0;
-        const uint32_t n2 = r ? xor3_u32(hi0, c[3], k1) : hi0 ^ c[3] ^ k1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
     }
 }
@@ -62,7 +44,7 @@ __device__ __forceinline__ void philox2x32_10(uint32_t c[2], uint32_t k0)
         if (r) k0 += 0x9E3779B9u;
         const uint64_t p = (uint64_t)0xd256d193u * c[0];
         const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
-        c[0] = r ? xor3_u32(hi, k0, c[1]) : hi ^ k0 ^ c[1];
+        c[0] = hi ^ k0 ^ c[1];
         c[1] = lo;
     }
 }
