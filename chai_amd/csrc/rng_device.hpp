@@ -23,16 +23,23 @@
 namespace chai_hip {
 
 // ------------------------------------------------------------ block functions
+// a ^ b ^ c in one v_bitop3_b32 (truth table 0x96); the two-XOR form costs two
+// full-rate issue slots, this one about one and a quarter
+__device__ __forceinline__ uint32_t xor3_u32(uint32_t a, uint32_t b, uint32_t c)
+{
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+
 __device__ __forceinline__ void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
 {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
         if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32)
+        // one 32 x 32 -> 64-bit product per multiplier (v_mad_u64_u32, half rate)
         const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
         const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
         const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
-        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        const uint32_t n0 = xor3_u32(hi1, c[1], k0), n2 = xor3_u32(hi0, c[3], k1);
         c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
     }
 }
@@ -44,7 +51,7 @@ __device__ __forceinline__ void philox2x32_10(uint32_t c[2], uint32_t k0)
         if (r) k0 += 0x9E3779B9u;
         const uint64_t p = (uint64_t)0xd256d193u * c[0];
         const uint32_t lo = (uint32_t)p, hi = (uint32_t)(p >> 32);
-        c[0] = hi ^ k0 ^ c[1];
+        c[0] = xor3_u32(hi, k0, c[1]);
         c[1] = lo;
     }
 }
@@ -121,18 +128,17 @@ __device__ __forceinline__ void threefry2x64_20(uint64_t X[2], uint64_t k0, uint
 }
 
 // ------------------------------------------------------- uniform conversions
+// (float)(int32)w * 2^-32 + 0.5 as one fma: the product by a power of two is
+// exact (|w| >= 1 stays normal), so the fused form rounds once, where the
+// two-step form rounds only at the add -- the same value
 __device__ __forceinline__ float unit_f32(uint32_t w)
 {
-    float t = (float)(int32_t)w;
-    t = t * 0x1p-32f;
-    return t + 0.5f;
+    return __builtin_fmaf((float)(int32_t)w, 0x1p-32f, 0.5f);
 }
 
 __device__ __forceinline__ double unit_f64(uint64_t w)
 {
-    double t = (double)(int64_t)w;
-    t = t * 0x1p-64;
-    return t + 0.5;
+    return __builtin_fma((double)(int64_t)w, 0x1p-64, 0.5);
 }
 
 // ------------------------------------------------------ Box-Muller math (f64)

@@ -24,3 +24,12 @@ inline int __mulhi(int a, int b) { return (int)(((int64_t)a * b) >> 32); }
 enum { CHAI_U32 = 0, CHAI_I32 = 1, CHAI_F32 = 4, CHAI_F64 = 8, CHAI_RNG_PHILOX = 1230, CHAI_RNG_THREEFRY = 1231 };
 inline int __clz(int x) { return x ? __builtin_clz((unsigned)x) : 32; }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
+// v_bitop3_b32: bit i of the result is bit (a_i b_i c_i) of the truth table imm
+inline uint32_t emu_bitop3(uint32_t a, uint32_t b, uint32_t c, unsigned imm)
+{
+    uint32_t r = 0;
+    for (unsigned t = 0; t < 8; t++)
+        if ((imm >> t) & 1u) r |= ((t & 4) ? a : ~a) & ((t & 2) ? b : ~b) & ((t & 1) ? c : ~c);
+    return r;
+}
+#define __builtin_amdgcn_bitop3_b32(a, b, c, imm) emu_bitop3((a), (b), (c), (imm))
