@@ -35,9 +35,21 @@ namespace chai_hip {
 // split / pack pre-pass
 // =====================================================================
 
+// Block grid of the split kernels: consecutive blocks walk the source's
+// contiguous index, k for a K-contiguous source (grid (Kp / 64, Rp / 64, nb)),
+// r for a transposed one (grid (Rp / 64, Kp / 64, nb)); for 8192^2 f32 the
+// transposed split takes 0.104 ms in r order against 0.119 ms in k order.
+template <bool TRANS>
+__device__ __forceinline__ void split_block(uint32_t& rb, uint32_t& kb, uint32_t& nkb)
+{
+    rb = TRANS ? blockIdx.x : blockIdx.y;
+    kb = TRANS ? blockIdx.y : blockIdx.x;
+    nkb = TRANS ? gridDim.y : gridDim.x;
+}
+
 template <typename T, bool TRANS>
-// Block (kb, rb, z) splits rows [64 rb, +64) x k [64 kb, +64) of instance z
-// and writes nz[(z * gridDim.y + rb) * gridDim.x + kb] = 1 if any of its lo
+// Block (rb, kb, z) splits rows [64 rb, +64) x k [64 kb, +64) of instance z
+// and writes nz[(z * (Rp / 64) + rb) * (Kp / 64) + kb] = 1 if any of its lo
 // values is nonzero (the GEMM tile skips products with all-zero lo panels).
 __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, uint64_t ld,
                                                     uint64_t R, uint64_t K, uint64_t Rp, uint64_t Kp,
@@ -46,14 +58,16 @@ __global__ __launch_bounds__(256) void split_kernel(const T* __restrict__ src, u
                                                     uint16_t* __restrict__ lo, uint8_t* __restrict__ nz)
 {
     __shared__ float tile[64][65];
+    uint32_t rb, kb, nkb;
+    split_block<TRANS>(rb, kb, nkb);
     const uint64_t dst_bstride = Rp * Kp;
     const uint32_t lo_or =
         split_tile<T, TRANS>(src + (uint64_t)blockIdx.z * src_bstride, ld, R, K, Kp, Rp,
                              hi + (uint64_t)blockIdx.z * dst_bstride, lo + (uint64_t)blockIdx.z * dst_bstride,
-                             (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64, tile);
+                             (uint64_t)rb * 64, (uint64_t)kb * 64, tile);
     const int any = __syncthreads_or(lo_or != 0u);
     if (threadIdx.x == 0)
-        nz[((uint64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (uint8_t)(any != 0);
+        nz[((uint64_t)blockIdx.z * (Rp / 64) + rb) * nkb + kb] = (uint8_t)(any != 0);
 }
 
 // The three-piece split of the BF16x6 family: p0 = hi, p1 = lo, p2 = the
@@ -68,13 +82,15 @@ __global__ __launch_bounds__(256) void split3_kernel(const T* __restrict__ src, 
 {
     __shared__ float tile[64][65];
     const uint64_t z = blockIdx.z;
+    uint32_t rb, kb, nkb;
+    split_block<TRANS>(rb, kb, nkb);
     const uint32_t lo_or =
         split_tile<T, TRANS, true>(src + z * src_bstride, ld, R, K, Kp, Rp, p0 + z * dst_bstride,
-                                   p1 + z * dst_bstride, (uint64_t)blockIdx.y * 64, (uint64_t)blockIdx.x * 64,
+                                   p1 + z * dst_bstride, (uint64_t)rb * 64, (uint64_t)kb * 64,
                                    tile, p2 + z * dst_bstride);
     const int any = __syncthreads_or(lo_or != 0u);
     if (threadIdx.x == 0)
-        nz[(z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = (uint8_t)(any != 0);
+        nz[(z * (Rp / 64) + rb) * nkb + kb] = (uint8_t)(any != 0);
 }
 
 // Pack op(X) into a dense zero-padded [Rp][Cp] array of TO (element (r, c) of
@@ -449,18 +465,20 @@ static void prof_mark(int i, hipStream_t st)
     if (i == 2) t_prof.done++;
 }
 
-// Kp and Rp are multiples of 64: the grid is exactly (Kp / 64, Rp / 64, nb)
-// and nz receives (Rp / 64) * (Kp / 64) flags per instance.
+// Kp and Rp are multiples of 64: the grid is exactly (Kp / 64, Rp / 64, nb),
+// or (Rp / 64, Kp / 64, nb) for a transposed source (split_block), and nz
+// receives (Rp / 64) * (Kp / 64) flags per instance.
 template <typename T>
 static void launch_split(const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
                          uint64_t Rp, uint64_t Kp, uint64_t src_bstride, uint64_t nb,
                          uint16_t* hi, uint16_t* lo, uint8_t* nz, hipStream_t st)
 {
-    dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64), (unsigned)nb);
     if (trans)
-        split_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo, nz);
+        split_kernel<T, true><<<dim3((unsigned)(Rp / 64), (unsigned)(Kp / 64), (unsigned)nb), 256, 0, st>>>(
+            (const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo, nz);
     else
-        split_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo, nz);
+        split_kernel<T, false><<<dim3((unsigned)(Kp / 64), (unsigned)(Rp / 64), (unsigned)nb), 256, 0, st>>>(
+            (const T*)src, ld, R, K, Rp, Kp, src_bstride, hi, lo, nz);
 }
 
 static void split_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K,
@@ -479,13 +497,12 @@ static void launch_split3(const void* src, uint64_t ld, bool trans, uint64_t R, 
                           uint64_t Kp, uint64_t src_bstride, uint64_t dst_bstride, uint64_t nb, uint16_t* p0,
                           uint16_t* p1, uint16_t* p2, uint8_t* nz, hipStream_t st)
 {
-    dim3 grid((unsigned)(Kp / 64), (unsigned)(Rp / 64), (unsigned)nb);
     if (trans)
-        split3_kernel<T, true><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, dst_bstride,
-                                                     p0, p1, p2, nz);
+        split3_kernel<T, true><<<dim3((unsigned)(Rp / 64), (unsigned)(Kp / 64), (unsigned)nb), 256, 0, st>>>(
+            (const T*)src, ld, R, K, Rp, Kp, src_bstride, dst_bstride, p0, p1, p2, nz);
     else
-        split3_kernel<T, false><<<grid, 256, 0, st>>>((const T*)src, ld, R, K, Rp, Kp, src_bstride, dst_bstride,
-                                                      p0, p1, p2, nz);
+        split3_kernel<T, false><<<dim3((unsigned)(Kp / 64), (unsigned)(Rp / 64), (unsigned)nb), 256, 0, st>>>(
+            (const T*)src, ld, R, K, Rp, Kp, src_bstride, dst_bstride, p0, p1, p2, nz);
 }
 
 static void split3_any(int prec, const void* src, uint64_t ld, bool trans, uint64_t R, uint64_t K, uint64_t Rp,
