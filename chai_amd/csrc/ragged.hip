@@ -484,6 +484,13 @@ int chai_hip_ragged_release(void* plan)
     return CHAI_OK;
 }
 
+int chai_hip_ragged_num_variants(int* n)
+{
+    if (!n) { set_error("ragged_num_variants: null output"); return CHAI_ERR_ARG; }
+    *n = kNumRagged;
+    return CHAI_OK;
+}
+
 int chai_hip_ragged_tune(const chai_ragged_item* items, size_t count, const chai_epilogue* epi,
                          int trials, int* best_variant, float* best_ms, void* stream)
 {

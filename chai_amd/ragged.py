@@ -100,6 +100,13 @@ class Plan:
             pass
 
 
+def num_variants():
+    """Number of ragged tile variants (0..7 panel tiles after a split pre-pass, 8..9 f32-fed)."""
+    n = ctypes.c_int()
+    check(lib().chai_hip_ragged_num_variants(ctypes.byref(n)), "chai_hip_ragged_num_variants")
+    return n.value
+
+
 def tune(items, epi=None, trials=3):
     """Time every tile variant on this batch; returns (best_variant, best_ms)."""
     init()

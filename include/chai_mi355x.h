@@ -236,6 +236,8 @@ int chai_hip_ragged_info(void* plan, uint64_t* info);
  * CHAI_ERR_STATE). */
 int chai_hip_ragged_last_timing(void* plan, float* split_ms, float* core_ms);
 int chai_hip_ragged_release(void* plan);
+/* Number of ragged tile variants (valid variant arguments are 0 .. n - 1). */
+int chai_hip_ragged_num_variants(int* n);
 /* Time every tile variant on this batch (one warm-up and `trials` runs each)
  * and remember the fastest for chai_hip_ragged_plan(..., -1, ...). */
 int chai_hip_ragged_tune(const chai_ragged_item* items, size_t count, const chai_epilogue* epi,
