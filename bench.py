@@ -385,6 +385,22 @@ def kernel_stats(d, title, command):
                 lines.append("%-100.100s | %5s | %10.3f | %10.3f | %5s" % (
                     r["Name"], r["Calls"], float(r["TotalDurationNs"]) / 1e6, float(r["AverageNs"]) / 1e3,
                     r.get("Percentage", "")))
+    # one kernel name can run several shapes (the 8192^3 and 4096^3 GEMMs share
+    # a tile): durations per launch grid, the median beside the mean
+    shapes = {}
+    for fn in glob.glob(os.path.join(d, "**", "*.db"), recursive=True):
+        con = sqlite3.connect(fn)
+        for name, dur, gx, gy, gz in con.execute("select name, duration, grid_x, grid_y, grid_z from kernels"):
+            shapes.setdefault((name, (gx, gy, gz)), []).append(dur / 1e3)
+        con.close()
+    if shapes:
+        lines.append("# per launch grid (work-items x, y, z), the 30 longest by total time: "
+                     "kernel | grid | calls | mean us | median us | min us")
+        top = sorted(shapes.items(), key=lambda kv: -sum(kv[1]))[:30]
+        for (name, g), v in top:
+            v = sorted(v)
+            lines.append("%-80.80s | %s | %5d | %10.3f | %10.3f | %10.3f" % (
+                name, "x".join(str(x) for x in g), len(v), sum(v) / len(v), v[len(v) // 2], v[0]))
     print("\n".join(lines))
 
 
