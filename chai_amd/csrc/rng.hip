@@ -127,12 +127,13 @@ __global__ __launch_bounds__(256) void rng_fill_64(RngGeom g, double* __restrict
 // time then write one compact stretch of the array; a grid of a few thousand
 // blocks striding over it (every wave writing 1 KiB per step, 16 MiB apart
 // between steps) stored 2^28 floats at 4.2-4.5 TB/s against 7.0 TB/s for this
-// grid, with the Philox arithmetic alone taking 0.11 ms of the 0.15 ms.  The
-// kernels still stride past 2^32 vectors.
+// grid, with the Philox arithmetic alone taking 0.11 ms of the 0.15 ms.  A
+// launch holds at most 2^31 work-items (the dispatch counts them in 32 bits);
+// the kernels stride past that.
 static unsigned rng_blocks(uint64_t nvec)
 {
     const uint64_t b = (nvec + 255) / 256;
-    return (unsigned)(b < (1ull << 24) ? b : (1ull << 24));
+    return (unsigned)(b < (1ull << 23) ? b : (1ull << 23));
 }
 
 template <typename T, int VARIANT, int NORMAL, int LIM>
