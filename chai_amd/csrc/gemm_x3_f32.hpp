@@ -16,6 +16,7 @@
 // columns of one k row).  Each thread then writes two 16-byte chunks per
 // panel: 8 consecutive k of one row, the layout the MFMA fragments read.
 #pragma once
+#include <type_traits>
 #include "gemm_x3.hpp"
 
 namespace chai_hip {
@@ -47,55 +48,91 @@ __device__ __forceinline__ void x3f_write16(const float* v, u32x4* lds, int o0, 
     lds[o1] = h1; lds[o1 + plane] = l1;
 }
 
-template <int BM, int BN, int WGM, int WGN>
-__device__ __forceinline__ void x3f_load(const GemmArgs& a, uint64_t r0, uint64_t c0, uint64_t k0, bool avec,
+// Stage k0 of the tile's operands into registers: thread t holds 16
+// consecutive k of A row t / 2 (+ p RP) and 16 consecutive k of B column
+// t % RP (+ p RP).  The loads go through raw buffer descriptors, which are
+// wave-uniform: A's over the 32 rows a wave reads ((rows - 1) lda + K floats
+// from its first row), B's over the stage's k rows ((rows - 1) ldb + N floats
+// from row k0).  A load past a descriptor's extent returns zero and touches
+// no memory, which zeroes rows of A past M and k past K; each lane computes
+// one offset per A row / B column and B's k rows are scalar offsets, so there
+// is no per-element address arithmetic.  Columns of B past N are read at
+// column N - 1 and zeroed.  Extents and offsets stay below 2^31 bytes for
+// row strides below 2^23 (the ragged plan checks it).  FULL: the stage's 32 k
+// lie below K (no k tests); AVEC: A's rows allow 16-byte loads (lda a
+// multiple of 4, A 16-byte aligned).  Both are template arguments so that a
+// loop of full stages is straight-line code whose load waits the compiler
+// can count.
+template <int BM, int BN, int WGM, int WGN, bool FULL, bool AVEC>
+__device__ __forceinline__ void x3f_load(const GemmArgs& a, uint64_t r0, uint64_t c0, uint64_t k0,
                                          float (&fa)[X3F32Shape<BM, BN, WGM, WGN>::NPA][16],
                                          float (&fb)[X3F32Shape<BM, BN, WGM, WGN>::NPB][16])
 {
     using F = X3F32Shape<BM, BN, WGM, WGN>;
+    static_assert(F::RP % 64 == 0, "B's k half is wave-uniform");
     const int t = threadIdx.x;
-    // A stage whose 32 k all lie below K (every stage but the last of a
-    // K % 32 != 0 operand) needs one row / column test per lane instead of
-    // one per element; kfull is block-uniform.
-    const bool kfull = k0 + 32 <= a.K;
-    const uint64_t kh = k0 + (uint64_t)(t & 1) * 16;          // A: 16 k of row t / 2
+    const uint32_t wave = __builtin_amdgcn_readfirstlane((uint32_t)t >> 6);
+    const bool kfull = FULL || k0 + 32 <= a.K;         // block-uniform
+    // A: 32 rows per wave and pass
+    const uint32_t kh = (uint32_t)(k0 + (uint64_t)(t & 1) * 16), rl = (uint32_t)(t & 63) >> 1;
 #pragma unroll
     for (int p = 0; p < F::NPA; p++) {
-        const uint64_t row = r0 + (uint64_t)(t >> 1) + (uint64_t)p * F::RP;
-        const float* src = a.Af + row * a.lda + kh;
-        if (kfull && avec) {
-            if (row < a.M) {
+        const uint64_t wr = r0 + (uint64_t)wave * 32 + (uint64_t)p * F::RP;   // the wave's first row
+        const uint64_t rows = wr < a.M ? (a.M - wr < 32 ? a.M - wr : 32) : 0;
+        const __amdgpu_buffer_rsrc_t ra = __builtin_amdgcn_make_buffer_rsrc(
+            (void*)(a.Af + (wr < a.M ? wr : 0) * a.lda), (short)0,
+            rows ? (int)(((rows - 1) * a.lda + a.K) * 4) : 0, 0x00020000);
+        const uint32_t off = rl < rows ? (uint32_t)(rl * a.lda + kh) * 4 : 0x80000000u;
+        if (AVEC && kfull) {
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const f32x4 x = gload(reinterpret_cast<const f32x4*>(src) + q);
-                    fa[p][4 * q] = x[0]; fa[p][4 * q + 1] = x[1]; fa[p][4 * q + 2] = x[2]; fa[p][4 * q + 3] = x[3];
-                }
-            } else {
+            for (int q = 0; q < 4; q++) {
+                const u32x4 x = __builtin_amdgcn_raw_buffer_load_b128(ra, off + 16 * q, 0, 0);
 #pragma unroll
-                for (int j = 0; j < 16; j++) fa[p][j] = 0.0f;
+                for (int e = 0; e < 4; e++) fa[p][4 * q + e] = __uint_as_float(x[e]);
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < 16; j++) fa[p][j] = (row < a.M && kh + j < a.K) ? gload(src + j) : 0.0f;
+            for (int j = 0; j < 16; j++) {
+                const uint32_t o = kh + j < a.K ? off + 4 * j : 0x80000000u;
+                fa[p][j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ra, o, 0, 0));
+            }
         }
     }
-    const uint64_t kb = k0 + (uint64_t)(t / F::RP) * 16;       // B: 16 k of column t % RP
+    // B: the stage's k rows (none for a stage entirely past K)
+    if (!FULL && k0 >= a.K) {
 #pragma unroll
-    for (int p = 0; p < F::NPB; p++) {
-        const uint64_t col = c0 + (uint64_t)(t % F::RP) + (uint64_t)p * F::RP;
-        const float* src = a.Bf + kb * a.ldb + col;
-        if (kfull) {
-            if (col < a.N) {
+        for (int p = 0; p < F::NPB; p++)
 #pragma unroll
-                for (int j = 0; j < 16; j++) fb[p][j] = gload(src + (uint64_t)j * a.ldb);
-            } else {
+            for (int j = 0; j < 16; j++) fb[p][j] = 0.0f;
+        return;
+    }
+    const uint32_t kbl = (uint32_t)(t / F::RP) * 16;   // wave-uniform
+    const uint64_t krows = a.K - k0 < 32 ? a.K - k0 : 32;
+    const __amdgpu_buffer_rsrc_t rb = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(a.Bf + k0 * a.ldb), (short)0, (int)(((krows - 1) * a.ldb + a.N) * 4), 0x00020000);
+    if (c0 + (uint64_t)BN <= a.N) {
 #pragma unroll
-                for (int j = 0; j < 16; j++) fb[p][j] = 0.0f;
-            }
-        } else {
+        for (int p = 0; p < F::NPB; p++) {
+            const uint32_t off = (uint32_t)(kbl * a.ldb + c0 + (uint64_t)(t % F::RP) + (uint64_t)p * F::RP) * 4;
 #pragma unroll
             for (int j = 0; j < 16; j++)
-                fb[p][j] = (col < a.N && kb + j < a.K) ? gload(src + (uint64_t)j * a.ldb) : 0.0f;
+                fb[p][j] = kfull || kbl + j < krows
+                               ? __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rb, off, (int)(j * a.ldb * 4), 0))
+                               : 0.0f;
+        }
+    } else {
+#pragma unroll
+        for (int p = 0; p < F::NPB; p++) {
+            const uint64_t col = c0 + (uint64_t)(t % F::RP) + (uint64_t)p * F::RP;
+            const bool ok = col < a.N;
+            const uint32_t off = (uint32_t)(kbl * a.ldb + (ok ? col : a.N - 1)) * 4;
+#pragma unroll
+            for (int j = 0; j < 16; j++) {
+                float v = 0.0f;
+                if (kfull || kbl + j < krows)
+                    v = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rb, off, (int)(j * a.ldb * 4), 0));
+                fb[p][j] = ok ? v : 0.0f;
+            }
         }
     }
 }
@@ -123,18 +160,15 @@ __device__ __forceinline__ void x3f_store_stage(u32x4* st, const float (&fa)[X3F
     }
 }
 
-// One BM x BN output tile (tm, tn) from f32 A (a.Af, a.lda) and B (a.Bf,
-// a.ldb); a.Kp = K rounded up to 64.  LDS is free on entry and on exit.
-// PIPE: software-pipeline the MFMA operand fragments across the barrier.
-template <int BM, int BN, int WGM, int WGN, bool PIPE>
-__device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, u32x4* lds)
+// x3f_tile for A rows that do (AVEC) or do not allow 16-byte loads.
+template <int BM, int BN, int WGM, int WGN, bool PIPE, bool AVEC>
+__device__ __forceinline__ void x3f_tile_body(const GemmArgs& a, uint32_t tm, uint32_t tn, u32x4* lds)
 {
     using F = X3F32Shape<BM, BN, WGM, WGN>;
     using S = typename F::S;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = (wave / WGN) * S::WM, wn = (wave % WGN) * S::WN;
     const uint64_t r0 = (uint64_t)tm * BM, c0 = (uint64_t)tn * BN;
-    const bool avec = ((a.lda & 3) == 0) && (((uintptr_t)a.Af & 15) == 0);
 
     floatx16 acc[S::TI][S::TJ];
 #pragma unroll
@@ -146,7 +180,7 @@ __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_
 
     float fa[F::NPA][16], fb[F::NPB][16];
     const int nk = (int)(a.Kp / 32);
-    x3f_load<BM, BN, WGM, WGN>(a, r0, c0, 0, avec, fa, fb);
+    x3f_load<BM, BN, WGM, WGN, false, AVEC>(a, r0, c0, 0, fa, fb);
     x3f_store_stage<BM, BN, WGM, WGN>(lds, fa, fb);
     __syncthreads();
 
@@ -154,15 +188,34 @@ __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_
     x3_frag_offsets<S>(wm, wn, lane, offA, offB);
 
     if constexpr (!PIPE) {
-        for (int kt = 0; kt < nk; kt++) {
-            const int cur = (kt & 1) * S::STAGE;
-            const bool more = kt + 1 < nk;
-            if (more) x3f_load<BM, BN, WGM, WGN>(a, r0, c0, (uint64_t)(kt + 1) * 32, avec, fa, fb);
-            x3_stage_mfma<S, 0>(lds, cur, offA, offB, acc);
-            // the other buffer was last read in step kt - 1, closed by its barrier
-            if (more) x3f_store_stage<BM, BN, WGM, WGN>(lds + ((kt + 1) & 1) * S::STAGE, fa, fb);
+        // Two f32 stages in registers: stage s sits in slot s & 1, is loaded
+        // at step s - 3 and split into LDS at step s - 1, so each load has two
+        // steps of MFMAs to arrive.  nk is even (Kp is a multiple of 64).
+        float ga[F::NPA][16], gb[F::NPB][16];            // slot 1 (fa, fb: slot 0)
+        x3f_load<BM, BN, WGM, WGN, false, AVEC>(a, r0, c0, 32, ga, gb);
+        if (2 < nk) x3f_load<BM, BN, WGM, WGN, false, AVEC>(a, r0, c0, 64, fa, fb);
+        // steps kt, kt + 1; the loads use x3f_load<FULL>
+        auto two_steps = [&](int kt, auto full) {
+            constexpr bool FULL = decltype(full)::value;
+            // step kt: stage kt from buffer 0; stage kt + 1 (slot 1) into buffer 1
+            x3_stage_mfma<S, 0>(lds, 0, offA, offB, acc);
+            x3f_store_stage<BM, BN, WGM, WGN>(lds + S::STAGE, ga, gb);
+            if (FULL || kt + 3 < nk) x3f_load<BM, BN, WGM, WGN, FULL, AVEC>(a, r0, c0, (uint64_t)(kt + 3) * 32, ga, gb);
             __syncthreads();
-        }
+            // step kt + 1: stage kt + 1 from buffer 1; stage kt + 2 (slot 0) into buffer 0
+            x3_stage_mfma<S, 0>(lds, S::STAGE, offA, offB, acc);
+            if (FULL || kt + 2 < nk) {
+                x3f_store_stage<BM, BN, WGM, WGN>(lds, fa, fb);
+                if (FULL || kt + 4 < nk)
+                    x3f_load<BM, BN, WGM, WGN, FULL, AVEC>(a, r0, c0, (uint64_t)(kt + 4) * 32, fa, fb);
+            }
+            __syncthreads();
+        };
+        // while both loads of a pair of steps are full stages, then the rest
+        const int kfs = (int)(a.K / 32);
+        int kt = 0;
+        for (; kt + 4 < kfs; kt += 2) two_steps(kt, std::true_type{});
+        for (; kt < nk; kt += 2) two_steps(kt, std::false_type{});
         x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc);
         return;
     }
@@ -177,7 +230,7 @@ __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_
     for (int kt = 0; kt < nk; kt++) {
         const int cur = (kt & 1) * S::STAGE;
         const bool more = kt + 1 < nk;
-        if (more) x3f_load<BM, BN, WGM, WGN>(a, r0, c0, (uint64_t)(kt + 1) * 32, avec, fa, fb);
+        if (more) x3f_load<BM, BN, WGM, WGN, false, AVEC>(a, r0, c0, (uint64_t)(kt + 1) * 32, fa, fb);
         x3_load_frag<S, 0>(lds, cur, 1, offA, offB, f[1]);
         x3_frag_mfma<S, 0>(f[0], acc);
         __builtin_amdgcn_sched_barrier(0);
@@ -189,6 +242,18 @@ __device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_
         x3_frag_mfma<S, 0>(f[1], acc);
     }
     x3_store<S>(a, a.C, r0 + wm, c0 + wn, lane, acc);
+}
+
+// One BM x BN output tile (tm, tn) from f32 A (a.Af, a.lda) and B (a.Bf,
+// a.ldb); a.Kp = K rounded up to 64.  LDS is free on entry and on exit.
+// PIPE: software-pipeline the MFMA operand fragments across the barrier.
+template <int BM, int BN, int WGM, int WGN, bool PIPE>
+__device__ __forceinline__ void x3f_tile(const GemmArgs& a, uint32_t tm, uint32_t tn, u32x4* lds)
+{
+    if (((a.lda & 3) == 0) && (((uintptr_t)a.Af & 15) == 0))
+        x3f_tile_body<BM, BN, WGM, WGN, PIPE, true>(a, tm, tn, lds);
+    else
+        x3f_tile_body<BM, BN, WGM, WGN, PIPE, false>(a, tm, tn, lds);
 }
 
 // The f32-fed family's tile shapes: X(BM, BN, waves_m, waves_n, min blocks per CU)

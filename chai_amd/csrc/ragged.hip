@@ -252,6 +252,12 @@ static int ragged_build(const chai_ragged_item* items, size_t count, const chai_
             delete P;
             return CHAI_ERR_ARG;
         }
+        // the f32-fed tiles address a wave's rows through 31-bit buffer offsets
+        if (v.fused && (it.lda >= (1ull << 23) || it.ldb >= (1ull << 23))) {
+            set_error("ragged_plan: item %zu: the f32-fed tile variants need row strides below 2^23", i);
+            delete P;
+            return CHAI_ERR_ARG;
+        }
         idx.push_back(i);
     }
     const size_t n = idx.size();
@@ -501,6 +507,7 @@ int chai_hip_ragged_tune(const chai_ragged_item* items, size_t count, const chai
     for (int v = 0; v < kNumRagged; v++) {
         RaggedPlan* P = nullptr;
         int rc = ragged_build(items, count, epi, v, &P);
+        if (rc == CHAI_ERR_ARG && v > 0 && kRaggedVariants[v].fused) continue;   // strides too large for it
         if (rc != CHAI_OK) return rc;
         float best = 1e30f;
         for (int t = 0; t <= trials && rc == CHAI_OK; t++) {   // run 0 is the warm-up

@@ -205,9 +205,10 @@ int chai_hip_gemm_epilogue(const chai_gemm_desc* desc, const void* A, const void
  * C = A B + epilogue with DIFFERENT shapes run as one persistent
  * split-BF16x3 launch: tile variants 0..7 (the dense path's shapes) after one
  * split pre-pass launch, variants 8..9 reading the f32 operands directly with
- * no pre-pass (the split happens inside each tile).  Results are identical
- * for every variant.  The persistent kernel pulls output
- * tiles from eight per-XCD queues, each filled largest-instance-first
+ * no pre-pass (the split happens inside each tile; they need row strides
+ * lda, ldb below 2^23).  Results are identical for every variant.  The
+ * persistent kernel pulls output tiles from eight per-XCD queues, each
+ * filled largest-instance-first
  * (LPT by M N K) and balanced by work; a block drains its own XCD's queue, then
  * steals from the others.  f32 operands, row-major, no transposes. */
 typedef struct chai_ragged_item {

@@ -172,6 +172,24 @@ def test_ragged_edge_cases(chai, oracle):
         ragged.Plan([ragged.item(A, B, C)], variant=99)
 
 
+def test_ragged_f32_fed_row_stride_limit(chai, oracle):
+    """The f32-fed tiles address rows through 31-bit buffer offsets: a row
+    stride of 2^23 floats is refused for them, accepted by the panel tiles."""
+    from chai_amd import ragged
+    big = torch.zeros(2 * (1 << 23) + 64, device="cuda")
+    A = big.as_strided((2, 16), (1 << 23, 1))
+    B = torch.ones(16, 16, device="cuda")
+    C = torch.empty(2, 16, device="cuda")
+    for v in (8, 9):
+        with pytest.raises(chai.ChaiError):
+            ragged.Plan([ragged.item(A, B, C)], variant=v)
+    p = ragged.Plan([ragged.item(A, B, C)], variant=2)
+    p.run()
+    torch.cuda.synchronize()
+    assert torch.equal(C, torch.zeros(2, 16, device="cuda"))
+    p.release()
+
+
 def test_ragged_tune_picks_a_variant(chai, oracle):
     from chai_amd import ragged
     batch = make_batch(oracle, SHAPES, seed0=900)
