@@ -999,6 +999,25 @@ def run_stream(args, leg=False):
     line["roofline"] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                         "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": read_traffic(PMC_SUMMARY_STREAM),
                         "kernel": "chai_trace_kernel"}
+    # a fused reduction trace over the same arrays: s = sum(A * B) (8 B read per
+    # element, FP64 block partials combined in a fixed order), beside the
+    # library's plain sum and torch's
+    tr_ = tr.Trace()
+    Ar, Br = tr_.input(chai.F32, n), tr_.input(chai.F32, n)
+    Sr = tr_.output(chai.F32, 1)
+    tr_.assign(Sr, tr.sum_(Ar * Br))
+    prog_r = tr_.compile()
+    s_out = torch.empty(1, device="cuda")
+    ms_r = timed(lambda: prog_r.run({Ar: a, Br: b, Sr: s_out}), args.steps, args.warmup)
+    ref = float(torch.dot(a.double(), b.double()))
+    ms_sum = timed(lambda: chai.reduce(a, "sum"), args.steps, args.warmup)
+    ms_tsum = timed(lambda: torch.sum(a), args.steps, args.warmup)
+    line["reduction_trace_2^28"] = {
+        "trace": "s = sum(A * B)", "ms": round(ms_r, 4), "GBs": round(8.0 * n / (ms_r * 1e-3) / 1e9, 1),
+        "frac_hbm": round(8.0 * n / (ms_r * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+        "rel_err_vs_fp64": abs(float(s_out.item()) - ref) / abs(ref),
+        "chai_reduce_sum_GBs": round(4.0 * n / (ms_sum * 1e-3) / 1e9, 1),
+        "torch_sum_GBs": round(4.0 * n / (ms_tsum * 1e-3) / 1e9, 1)}
     if not args.no_cpu:
         from oracle import oracle as o
         kc = 1 << 22
