@@ -599,6 +599,13 @@ int gemm_launch(const chai_gemm_desc& d, const void* A, const void* B, void* C, 
         if (vid >= kNumX3) { set_error("gemm: bf16x3 variant %d out of range", vid); return CHAI_ERR_ARG; }
         const X3Variant& v = kX3Variants[vid];
         const uint64_t Mp = round_up(M, v.BM), Np = round_up(N, v.BN), Kp = round_up(K, 64);
+        // the tile's stage copies address a panel's k block pair with 32-bit
+        // byte offsets (2 x 32 x Mp bytes per k block)
+        if (Mp > kX3MaxRows || Np > kX3MaxRows) {
+            set_error("gemm: bf16x3 takes at most %llu rows of op(A) and columns of op(B) per call",
+                      (unsigned long long)kX3MaxRows);
+            return CHAI_ERR_ARG;
+        }
         const uint64_t na = d.same_a ? 1 : nb, nbb = d.same_b ? 1 : nb;
         const size_t panel_bytes = 2 * sizeof(uint16_t) * (Mp * Kp * na + Np * Kp * nbb);
         const uint64_t fa = (Mp / 64) * (Kp / 64), fb = (Np / 64) * (Kp / 64);   // split flags per instance

@@ -83,7 +83,9 @@ int gemm_auto_mode(const chai_gemm_desc& d)
 {
     if (d.prec_a == CHAI_F64 || d.prec_b == CHAI_F64) return CHAI_GEMM_F64;
     if (d.prec_a <= CHAI_I32 && d.prec_b <= CHAI_I32) return CHAI_GEMM_INT;
-    return d.K >= 16384 ? CHAI_GEMM_BF16X3 : CHAI_GEMM_BF16X6;
+    if (d.K < 16384) return CHAI_GEMM_BF16X6;
+    // bf16x3's panels hold at most kX3MaxRows padded rows (256-row tiles)
+    return d.M + 256 <= kX3MaxRows && d.N + 256 <= kX3MaxRows ? CHAI_GEMM_BF16X3 : CHAI_GEMM_BF16X6;
 }
 
 }  // namespace chai_hip

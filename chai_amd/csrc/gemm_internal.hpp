@@ -45,6 +45,9 @@ int gemm_variant_params(int mode, int vid, int* params);   // BM, BN, BK, WGM, W
 void gemm_set_default_variant(int mode, int vid);
 // best variant recorded by the tuner for this exogenous key, or -1
 int tuned_variant(const chai_gemm_desc& d, int mode);
+// Padded rows of op(A) (columns of op(B)) the split-BF16x3 tile addresses
+// with its 32-bit stage-copy offsets.
+constexpr uint64_t kX3MaxRows = 1ull << 25;
 // The family CHAI_GEMM_AUTO resolves to for a descriptor (gemm_api.hip).
 int gemm_auto_mode(const chai_gemm_desc& d);
 // integer x integer products with the interpreter's semantics (igemm.hip)

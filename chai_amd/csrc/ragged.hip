@@ -252,6 +252,12 @@ static int ragged_build(const chai_ragged_item* items, size_t count, const chai_
             delete P;
             return CHAI_ERR_ARG;
         }
+        // the panel tiles address a panel's k block pair with 32-bit offsets
+        if (!v.fused && (round_up(it.M, v.BM) > kX3MaxRows || round_up(it.N, v.BN) > kX3MaxRows)) {
+            set_error("ragged_plan: item %zu: the panel tile variants take at most 2^25 padded rows / columns", i);
+            delete P;
+            return CHAI_ERR_ARG;
+        }
         // the f32-fed tiles address a wave's rows through 31-bit buffer offsets
         if (v.fused && (it.lda >= (1ull << 23) || it.ldb >= (1ull << 23))) {
             set_error("ragged_plan: item %zu: the f32-fed tile variants need row strides below 2^23", i);

@@ -126,7 +126,9 @@ int chai_hip_reduce_noalloc(int op, int prec, const void* in, size_t n, size_t s
 /* Kernel family selector.  AUTO picks F64 when any operand is f64, INT for
  * two integer operands, and for f32 or mixed int / f32 operands the split-
  * BF16 form within twice the FP32-FMA kernel's error: BF16X6 below
- * K = 16384, BF16X3 from there. */
+ * K = 16384, BF16X3 from there (BF16X6 also where op(A) has more than
+ * 2^25 - 256 rows or op(B) more columns: BF16X3 takes at most 2^25 padded
+ * rows / columns per call and refuses larger ones). */
 enum {
     CHAI_GEMM_AUTO = 0,
     CHAI_GEMM_FMA = 1,      /* plain FP32 VALU FMA                        */

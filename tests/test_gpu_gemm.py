@@ -221,6 +221,11 @@ def test_gemm_rejects_bad_arguments(chai):
         chai.gemm(Ai, Ai, mode=chai.GEMM_BF16X3, out_dtype=torch.float32)
     with pytest.raises(chai.ChaiError):     # and the integer GEMM takes integer operands only
         chai.gemm(A, A, mode=chai.GEMM_INT)
+    # bf16x3 panels hold at most 2^25 padded rows: refused before any allocation
+    tall = torch.ones((1 << 25) + 1, 1, device="cuda")
+    with pytest.raises(chai.ChaiError):
+        chai.gemm(tall, torch.ones(1, 1, device="cuda"), mode=chai.GEMM_BF16X3)
+    del tall
 
 
 def _bf16_rne(x):
