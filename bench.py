@@ -328,7 +328,7 @@ def pmc_summary(fetch_dir, write_dir, workload="sgemm"):
                 best = k
         return (best, sum(rows[best]) / len(rows[best])) if best else (None, None)
     _, f_cal = pick(fe, r"reduce_partial")
-    _, w_cal = pick(wr, r"rng_fill_32")
+    _, w_cal = pick(wr, r"rng_fill_32<float, 1230, 4, 0, 0>")   # the unit fill of 2^30 B (not the [-1, 1) operand fills)
     trace_wl = workload in ("gather", "stream")
     pat = r"chai_trace_kernel" if trace_wl else r"gemm_x3_kernel"
     kf, f_gemm = pick(fe, pat)
