@@ -273,13 +273,70 @@ __device__ __forceinline__ void bm_sincos2pif(float u, float& s, float& c)
     pc = __builtin_fmaf(pc, t2, 0x1.03c1f0p+6f);
     pc = __builtin_fmaf(pc, t2, -0x1.3bd3ccp+4f);
     const float cs = __builtin_fmaf(t2, pc, 1.0f);
-    s = qi == 0 ? sn : qi == 1 ? cs : qi == 2 ? -sn : -cs;
-    c = qi == 0 ? cs : qi == 1 ? -sn : qi == 2 ? -cs : sn;
+    // quadrant qi: (s, c) = (sn, cs), (cs, -sn), (-sn, -cs), (-cs, sn) -- a swap
+    // for odd qi and sign flips, as selects and xors rather than branches
+    const bool odd = qi & 1;
+    s = __uint_as_float(__float_as_uint(odd ? cs : sn) ^ ((uint32_t)(qi & 2) << 30));
+    c = __uint_as_float(__float_as_uint(odd ? sn : cs) ^ ((uint32_t)((qi + 1) & 2) << 30));
+}
+
+// The f32 normals' own forms of log and sqrt.  Their inputs come from
+// unit_f32, which gives 0, 1 or a normal number in between, so these skip
+// the general routines' scaling and special cases and their control flow,
+// and replace the expanded IEEE divide and square root with short
+// sequences.  Each returns the bits of the general form (bm_logf with its
+// IEEE divide, the correctly rounded sqrtf) for every one of the 2^32 stream
+// words, checked exhaustively on the device.
+
+// f / d for d = 2 + f in [1.70, 2.42] and f = 0 or 2^-23 <= |f| < 0.42: the
+// hardware reciprocal refined by one Newton step, the quotient corrected once
+// by its residual (exact in the fma).
+__device__ __forceinline__ float bm_div(float f, float d)
+{
+    float r = __builtin_amdgcn_rcpf(d);
+    r = __builtin_fmaf(__builtin_fmaf(-d, r, 1.0f), r, r);
+    const float q = f * r;
+    return __builtin_fmaf(__builtin_fmaf(-d, q, f), r, q);
+}
+
+// bm_logf for x = 0, 1 or a normal number between.
+__device__ __forceinline__ float bm_logf_unit(float x)
+{
+    const uint32_t b = __float_as_uint(x);
+    float m = __uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    const bool up = m > 0x1.6a09e6p+0f;
+    m = up ? m * 0.5f : m;
+    const int e = (int)(b >> 23) - 127 + (up ? 1 : 0);
+    const float f = m - 1.0f;
+    const float s = bm_div(f, 2.0f + f);
+    const float z = s * s;
+    float p = 0x1.745d18p-3f;
+    p = __builtin_fmaf(p, z, 0x1.c71c72p-3f);
+    p = __builtin_fmaf(p, z, 0x1.24924ap-2f);
+    p = __builtin_fmaf(p, z, 0x1.99999ap-2f);
+    p = __builtin_fmaf(p, z, 0x1.555556p-1f);
+    const float logm = __builtin_fmaf(s * z, p, 2.0f * s);
+    const float de = (float)e;
+    const float l = __builtin_fmaf(de, 0x1.62e4p-1f, __builtin_fmaf(de, 0x1.7f7d1cp-20f, logm));
+    return x == 0.0f ? -__builtin_inff() : l;
+}
+
+// Correctly rounded sqrt of y = -2 log(u): -0, +inf or a normal number.  The
+// hardware square root s lies within an ulp of sqrt(y); y <= s_dn * s (the
+// square of the midpoint below s, less ulp^2 / 4) picks the neighbour below,
+// y > s_up * s the one above.
+__device__ __forceinline__ float bm_sqrtf(float y)
+{
+    const float s = __builtin_amdgcn_sqrtf(y);
+    const float sd = __uint_as_float(__float_as_uint(s) - 1u), su = __uint_as_float(__float_as_uint(s) + 1u);
+    float t = __builtin_fmaf(-sd, s, y) <= 0.0f ? sd : s;
+    t = __builtin_fmaf(-su, s, y) > 0.0f ? su : t;
+    return y == 0.0f || y == __builtin_inff() ? y : t;
 }
 
 __device__ __forceinline__ void box_muller_f32(float ua, float ub, float& n0, float& n1)
 {
-    const float r = __builtin_sqrtf(-2.0f * bm_logf(ua));
+    const float r = bm_sqrtf(-2.0f * bm_logf_unit(ua));
     float s, c;
     bm_sincos2pif(ub, s, c);
     n0 = r * s;

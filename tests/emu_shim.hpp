@@ -33,3 +33,8 @@ inline uint32_t emu_bitop3(uint32_t a, uint32_t b, uint32_t c, unsigned imm)
     return r;
 }
 #define __builtin_amdgcn_bitop3_b32(a, b, c, imm) emu_bitop3((a), (b), (c), (imm))
+// v_rcp_f32 / v_sqrt_f32 are within an ulp on the device; the correctly rounded
+// host operations are within that bound
+inline float emu_rcpf(float x) { return 1.0f / x; }
+#define __builtin_amdgcn_rcpf(x) emu_rcpf(x)
+#define __builtin_amdgcn_sqrtf(x) sqrtf(x)
