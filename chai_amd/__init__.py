@@ -64,6 +64,7 @@ def lib():
         L.chai_hip_timer_elapsed_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
         L.chai_hip_timer_destroy.argtypes = [vp]
         L.chai_hip_rng_fill.argtypes = [i, i, i, ctypes.c_uint64, sz, sz, sz, i, i, d, d, vp, vp]
+        L.chai_hip_rng_check_normals.argtypes = [ctypes.POINTER(ctypes.c_uint64), vp]
         L.chai_hip_reduce.argtypes = [i, i, vp, sz, sz, vp, vp, vp]
         L.chai_hip_gemm.argtypes = [ctypes.POINTER(GemmDesc), vp, vp, vp, vp]
         L.chai_hip_gemm_num_variants.argtypes = [i, ctypes.POINTER(i)]
@@ -127,6 +128,15 @@ def rng_fill(out, variant=PHILOX, seed=0, vec_len=0, normal=False, limits=None,
                                   int(limits is not None), lo, hi, _ptr(out), _stream()),
           "chai_hip_rng_fill")
     return out
+
+
+def rng_check_normals():
+    """Run every 32-bit stream word through the f32 normals' short log / sqrt
+    forms and the IEEE ones; returns (log mismatches, sqrt mismatches)."""
+    init()
+    out = (ctypes.c_uint64 * 2)()
+    check(lib().chai_hip_rng_check_normals(out, _stream()), "chai_hip_rng_check_normals")
+    return int(out[0]), int(out[1])
 
 
 # --------------------------------------------------------------- reductions

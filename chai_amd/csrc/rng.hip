@@ -181,9 +181,47 @@ static int dispatch(int prec, int vl, int normal, const RngGeom& g, void* out, h
     return CHAI_OK;
 }
 
+// Every 32-bit stream word through the f32 normals' log and sqrt and through
+// the general forms (bm_logf with its IEEE divide, the correctly rounded
+// sqrtf): cnt[0] counts the words whose logs differ in any bit, cnt[1] those
+// whose square roots do.
+__global__ __launch_bounds__(256) void bm_check_kernel(unsigned long long* cnt)
+{
+    const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
+    unsigned long long bl = 0, bs = 0;
+    for (uint64_t w = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; w < (1ull << 32); w += nthr) {
+        const float u = unit_f32((uint32_t)w);
+        const float l = bm_logf(u);
+        bl += __float_as_uint(l) != __float_as_uint(bm_logf_unit(u));
+        const float y = -2.0f * l;
+        bs += __float_as_uint(__builtin_sqrtf(y)) != __float_as_uint(bm_sqrtf(y));
+    }
+    if (bl) atomicAdd(&cnt[0], bl);
+    if (bs) atomicAdd(&cnt[1], bs);
+}
+
 }  // namespace chai_hip
 
 using namespace chai_hip;
+
+extern "C" int chai_hip_rng_check_normals(uint64_t* mismatches, void* stream)
+{
+    if (!mismatches) { set_error("rng_check_normals: null output"); return CHAI_ERR_ARG; }
+    unsigned long long* d = nullptr;
+    CHAI_HIP_TRY(hipMalloc(&d, 2 * sizeof(unsigned long long)));
+    hipStream_t st = as_stream(stream);
+    hipError_t e = hipMemsetAsync(d, 0, 2 * sizeof(unsigned long long), st);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(bm_check_kernel, dim3(2048), dim3(256), 0, st, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(mismatches, d, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    const hipError_t ef = hipFree(d);
+    if (e == hipSuccess) e = ef;
+    if (e != hipSuccess) return hip_fail(e, "rng_check_normals");
+    return CHAI_OK;
+}
 
 extern "C" int chai_hip_rng_fill(int variant, int prec, int vec_len, uint64_t seed, size_t W,
                                  size_t H, size_t slots, int normal, int has_limits, double lo,

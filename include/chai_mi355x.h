@@ -100,6 +100,14 @@ int chai_hip_rng_fill(int variant, int prec, int vec_len, uint64_t seed,
                       int has_limits, double lo, double hi,
                       void* out, void* stream);
 
+/* Self-check of the f32 normals' arithmetic; no counterpart in the reference.
+ * Box-Muller's log and sqrt in the f32 streams use short forms valid for
+ * the uniforms a stream word yields; this runs all 2^32 words through them
+ * and through the general IEEE forms and writes the number of words whose
+ * log (mismatches[0]) or sqrt (mismatches[1]) differ in any bit: both zero
+ * on a correct device.  Takes about 20 ms; allocates 16 bytes. */
+int chai_hip_rng_check_normals(uint64_t* mismatches, void* stream);
+
 /* ------------------------------------------------------------- reductions */
 /* out[s] = sum / mean of in[s][0..n) (interp/InterpAccum.cpp:13-70 semantics:
  * accumulation is exact-in-FP64-combine, result in prec's type).  Replaces the

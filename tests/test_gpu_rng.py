@@ -1,8 +1,8 @@
 """GPU parity: chai_hip_rng_fill against the oracle's stream, bit-exact.
 
 Integer words, uniforms and normals are all compared bit for bit: the
-normals' log / sin / cos are the stream's own double-precision routines on
-both sides (see oracle/rng.c)."""
+normals' log / sin / cos are the stream's own routines on both sides, in
+double for the f64 streams and in f32 for the f32 streams (see oracle/rng.c)."""
 import numpy as np
 import pytest
 import torch
@@ -60,6 +60,15 @@ def test_full_size_normals_2_24(chai, oracle):
     assert np.array_equal(g.view(np.uint32), o.view(np.uint32))
     x = g.astype(np.float64).ravel()
     assert abs(x.mean()) < 5.0 / np.sqrt(n) and abs(x.var() - 1.0) < 2e-3
+
+
+def test_normals_short_forms_exhaustive(chai):
+    """The f32 normals' log (branch-free, reciprocal-plus-correction divide)
+    and sqrt (hardware root moved by fma residuals) give the bits of bm_logf
+    with its IEEE divide and of the correctly rounded sqrtf for every one of
+    the 2^32 stream words -- the whole domain unit_f32 can feed them."""
+    print("case exhaustive normals check", flush=True)
+    assert chai.rng_check_normals() == (0, 0)
 
 
 def test_rejects_bad_arguments(chai):
